@@ -1,0 +1,175 @@
+"""Benchmark: batched code-phase/Doppler search throughput on one MI355X.
+
+Workload (default): GPS + SBAS + QZSS L1 C/A, 24 antennas, 2-bit real samples at
+16.368 MHz, 10 ms records (163680 samples); seeded signals: 8 GPS, 2 SBAS and 1 QZSS at
+C/N0 uniform in [38, 50] dB-Hz with per-SV Doppler and per-antenna carrier offsets from a
+3 m disk; search 38 + 39 + 23 PRNs x 41 Doppler bins (+-10 kHz, 500 Hz step), N = 2P
+(reference form, two hypotheses), FP32, plus ACR C/N0 on the host.
+
+A step = one tga_search_device call on an observation already resident in HBM: decode +
+de-mean, forward spectra, all correlation rows, selection, copy of the 2400 peaks to the
+host, and the f64 ACR lookups.  Metric: search cells/s = sum(n_ant * n_prn * n_bin * P) / t.
+
+Prints one JSON line (see the repository's measurement contract).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "tart-gnss_amd"))
+sys.path.insert(0, REPO)
+
+METRIC = "search cells/s (ant×PRN×Doppler×phase): FP16-MFMA vs FP32 GPU vs CPU"
+FS = 16.368e6
+IF = 4.092e6
+HBM_PEAK_GBS = 8000.0
+
+
+def config2_signals(rng):
+    sv = []
+    gps = rng.choice(np.arange(1, 33), 8, replace=False)
+    sbas = rng.choice(np.arange(120, 159), 2, replace=False)
+    qz = rng.choice(np.arange(184, 207), 1, replace=False)
+    for sysname, prns in (("gps", gps), ("sbas", sbas), ("qzss", qz)):
+        for prn in prns:
+            sv.append(dict(system=sysname, prn=int(prn), cn0=float(rng.uniform(38, 50)),
+                           doppler=float(rng.uniform(-9000, 9000)),
+                           delay=int(rng.integers(0, 16368)), phase=float(rng.uniform(0, 6.28)),
+                           az=float(rng.uniform(0, 360)), el=float(rng.uniform(15, 90))))
+    return sv
+
+
+def load_traffic(path, kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
+    from the committed summary; None if absent."""
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return float(d["kernels"][kernel]["hbm_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU sample")
+    ap.add_argument("--seed", type=int, default=0x5EED_2026 + 2)
+    ap.add_argument("--traffic", default=os.path.join(REPO, "profiles", "pmc_traffic.json"))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the search "
+              "shards by antenna with no collective, see the design notes", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import tga
+    from tga import synth
+
+    global np
+    import numpy as np
+
+    assert torch.cuda.is_available(), "no GPU visible"
+    rng = np.random.default_rng(args.seed)
+    n_ant, L = 24, 163680
+    signals = config2_signals(rng)
+    t0 = time.perf_counter()
+    packed = synth.synth_observation(n_ant, L, FS, signals, fmt=2, seed=args.seed)
+    t_synth = time.perf_counter() - t0
+    fc = [IF - 10000.0 + 500.0 * i for i in range(41)]
+    systems = [tga.SystemSearch("gps", fc), tga.SystemSearch("sbas", fc),
+               tga.SystemSearch("qzss", fc)]
+    t0 = time.perf_counter()
+    plan = tga.Plan(FS, n_ant, L, systems, input_format=2)
+    t_plan = time.perf_counter() - t0
+    n_prn = sum(len(p) for p in plan.prns)
+    P = 16368
+    cells = n_ant * n_prn * len(fc) * P
+
+    dev = torch.from_numpy(packed).cuda()
+    torch.cuda.synchronize()
+    ptr = dev.data_ptr()
+
+    def step():
+        peaks = plan.search_device(ptr)
+        # ACR C/N0 per (antenna, PRN) on the host in f64 (acquisition.rs:341-357)
+        ok = 0
+        for arr in peaks:
+            vm = arr["best_peak"].astype(np.float64)
+            vs = arr["second_peak"].astype(np.float64)
+            for a in range(vm.shape[0]):
+                for j in range(vm.shape[1]):
+                    if vs[a, j] > 0 and vm[a, j] > vs[a, j] and tga.estimate_cn0(
+                            (vm[a, j] / vs[a, j]) ** 2, "GPS_L1_CA_ACR_TABLE") is not None:
+                        ok += 1
+        return peaks, ok
+
+    for _ in range(args.warmup):
+        peaks, _ = step()
+    plan.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        peaks, n_cn0 = step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_step = dt * 1000.0 / args.steps
+    value = cells * args.steps / dt
+
+    rows_ms, rows_n, rows_bytes = plan.timing(0)
+    fwd_ms, fwd_n, _ = plan.timing(1)
+    sel_ms, sel_n, _ = plan.timing(2)
+    avg_rows_s = rows_ms / 1000.0 / max(rows_n, 1)
+    achieved = rows_bytes / avg_rows_s / 1e9
+    traffic = load_traffic(args.traffic, plan.kernel_name(0))
+
+    # detections: every injected SV found at its code phase and nearest bin on most antennas
+    found = 0
+    for s in signals:
+        si = {"gps": 0, "sbas": 1, "qzss": 2}[s["system"]]
+        j = list(plan.prns[si]).index(s["prn"])
+        cp = (P - s["delay"]) % P
+        hits = sum(int(peaks[si][a, j]["best_codephase"]) == cp for a in range(n_ant))
+        found += hits >= n_ant // 2
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import cpu_port
+        workers = min(16, os.cpu_count() or 1)
+        levels = synth.unpack_levels(packed[:1], L, 2)[0]
+        t_cells, t_s, n_done = cpu_port.time_sample(levels, FS, "gps", list(range(1, 39)), fc,
+                                                    workers, budget_s=args.cpu_budget)
+        cpu = {"value": t_cells / t_s, "unit": "cells/s", "cores": workers, "kind": "port",
+               "sample": f"antenna 0, {n_done} GPS PRNs x 41 bins, N=32736, complex64 "
+                         f"pocketfft over {workers} threads, reference work structure "
+                         f"(2 replica + 41x3 FFTs per PRN), {t_s:.1f} s"}
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "GPS+SBAS+QZSS L1 C/A, 24 antennas, 2-bit, 16.368 MHz, 10 ms; "
+                               "100 PRNs x 41 bins (+-10 kHz, 500 Hz), N=2P=32736, FP32 + ACR",
+                   "n_ant": n_ant, "n_prn": n_prn, "n_bins": len(fc), "code_period": P,
+                   "cells": cells, "form": "reference (dual hypothesis, folded P-point rows)"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": plan.kernel_name(0), "kernel_ms": avg_rows_s * 1000.0,
+                     "alg_bytes_per_launch": rows_bytes},
+        "cpu_baseline": cpu,
+        "extra": {"forward_ms": fwd_ms / max(fwd_n, 1), "select_ms": sel_ms / max(sel_n, 1),
+                  "plan_create_s": t_plan, "synth_s": t_synth, "cn0_valid": n_cn0,
+                  "injected_found": f"{found}/{len(signals)}"},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

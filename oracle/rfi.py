@@ -1,0 +1,62 @@
+"""Narrowband RFI report (oracle restatement of rfi.rs:83-184)."""
+import numpy as np
+
+MIN_EXCESS_DB = 15.0   # rfi.rs:36
+CLUSTER_HZ = 10_000.0  # rfi.rs:41
+MAX_LINES = 3          # rfi.rs:43
+
+
+def autocorrelation(z, lag):
+    """rfi.rs:83-86: sum z[i] z[i+lag] / (n - lag)."""
+    n = len(z)
+    return float(np.dot(z[:n - lag], z[lag:])) / (n - lag)
+
+
+def power_spectrum(z):
+    """|FFT_f64(z)|^2 for k < n/2 (rfi.rs:96-103)."""
+    X = np.fft.fft(np.asarray(z, dtype=np.float64))
+    half = len(z) // 2
+    return X.real[:half] ** 2 + X.imag[:half] ** 2
+
+
+def detect_lines(z, fs):
+    """rfi.rs:91-138."""
+    n = len(z)
+    if n < 8:
+        return []
+    half = n // 2
+    bin_hz = fs / n
+    p = power_spectrum(z)
+    floor = np.sort(p)[half // 2]
+    lines = []
+    k = np.arange(1, half - 1)
+    is_max = (p[k] > p[k - 1]) & (p[k] >= p[k + 1])
+    for kk in k[is_max]:
+        with np.errstate(divide="ignore"):
+            excess = 10.0 * np.log10(p[kk] / floor)
+        if excess > MIN_EXCESS_DB:
+            lines.append({"frequency": kk * bin_hz, "excess_db": float(excess)})
+    lines.sort(key=lambda l: -l["excess_db"])  # stable, like Vec::sort_by
+    out = []
+    for ln in lines:
+        if all(abs(c["frequency"] - ln["frequency"]) > CLUSTER_HZ for c in out):
+            out.append(ln)
+            if len(out) == MAX_LINES:
+                break
+    return out
+
+
+def analyze_channel(antenna, bipolar, fs):
+    """rfi.rs:142-168."""
+    b = np.asarray(bipolar, dtype=np.float64)
+    z = b - b.sum() / len(b)
+    var = float((z * z).sum()) / len(z)
+    if var < 1e-9:
+        return {"antenna": antenna, "dead": True, "acf_lag1": None, "acf_lag3": None,
+                "acf_lag16": None, "period_samples": None, "lines": []}
+    lines = detect_lines(z, fs)
+    return {"antenna": antenna, "dead": False,
+            "acf_lag1": autocorrelation(z, 1), "acf_lag3": autocorrelation(z, 3),
+            "acf_lag16": autocorrelation(z, 16),
+            "period_samples": (fs / lines[0]["frequency"]) if lines else None,
+            "lines": lines}

@@ -1,0 +1,341 @@
+// Generic batched mixed-radix Stockham FFT through global memory (gfx950).
+//
+// One kernel launch per radix pass; each thread computes one radix-R butterfly of one row:
+//   j in [0, n/R), k = j mod ns (ns = product of the radices already applied)
+//   v[r] = src[j + r n/R] * W_{ns R}^{r k}           (twiddle skipped in the first pass)
+//   v    = DFT_R(v)
+//   dst[(j / ns) ns R + k + r ns] = v[r]
+// Reads of consecutive j are consecutive addresses; so are the writes once ns > 1.  Radices
+// 16, 8, 4, 2, 3, 5, 7, 11, 13, 31 have register codelets; any other prime factor p runs
+// through a direct O(p^2) butterfly.  This engine serves every length that has no fused LDS
+// kernel: level-1 calls of any N, replica spectra, RFI records, long code periods.
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <mutex>
+
+#include "fft_device.h"
+#include "internal.h"
+
+namespace tga {
+
+struct FftPlan {
+  size_t n = 0;
+  std::vector<int> radix;  // pass order
+  int split = 1;
+  DevBuf tw;  // tw1[split] then tw2[ceil(n/split)]
+  std::vector<DevBuf> prime_roots;  // per pass with a generic radix: W_R^m, m < R
+};
+
+struct FftPlanD {
+  size_t n = 0;
+  std::vector<int> radix;
+  int split = 1;
+  DevBuf tw;
+  std::vector<DevBuf> prime_roots;
+};
+
+namespace {
+
+constexpr int kBlock = 256;
+
+bool has_codelet(int r) {
+  switch (r) {
+    case 2: case 3: case 4: case 5: case 7: case 8: case 11: case 13: case 16: case 31:
+      return true;
+  }
+  return false;
+}
+
+void push_odd_primes(size_t m, std::vector<int>& out) {
+  for (size_t p = 3; p * p <= m; p += 2)
+    while (m % p == 0) { out.push_back(static_cast<int>(p)); m /= p; }
+  if (m > 1) out.push_back(static_cast<int>(m));
+}
+
+size_t strip_pow2(size_t m) {
+  while (m % 2 == 0) m /= 2;
+  return m;
+}
+
+// Single precision pass order: odd radices first, then 16s and one 8/4/2 remainder.
+std::vector<int> factorize(size_t n) {
+  std::vector<int> out;
+  push_odd_primes(strip_pow2(n), out);
+  size_t m = n / strip_pow2(n);
+  while (m % 16 == 0) { out.push_back(16); m /= 16; }
+  if (m > 1) out.push_back(static_cast<int>(m));  // 8, 4 or 2
+  return out;
+}
+
+// Double precision pass order: radix 4 (butterfly path), one 2, then odd primes (direct).
+std::vector<int> factorize_d(size_t n) {
+  std::vector<int> out;
+  size_t m = n / strip_pow2(n);
+  while (m % 4 == 0) { out.push_back(4); m /= 4; }
+  if (m == 2) out.push_back(2);
+  push_odd_primes(strip_pow2(n), out);
+  return out;
+}
+
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_pass(const float2* __restrict__ src,
+                                                 float2* __restrict__ dst, int n, int ns,
+                                                 long long nb_total, const float2* tw1,
+                                                 const float2* tw2, int split) {
+  const long long gid = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (gid >= nb_total) return;
+  const int nb = n / R;
+  const long long row = gid / nb;
+  const int j = static_cast<int>(gid - row * nb);
+  const float2* s = src + row * n;
+  float2* d = dst + row * n;
+  float2 v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = s[j + r * nb];
+  const int k = j % ns;
+  if (ns > 1) {
+    const int e1 = k * (n / (ns * R));
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twiddle2(tw1, tw2, split, r * e1));
+  }
+  Dft<R>::run(v);
+  const int base = (j / ns) * ns * R + k;
+#pragma unroll
+  for (int r = 0; r < R; ++r) d[base + r * ns] = v[r];
+}
+
+// Direct butterfly for a prime radix without a codelet.
+__global__ __launch_bounds__(kBlock) void k_pass_generic(const float2* __restrict__ src,
+                                                         float2* __restrict__ dst, int n,
+                                                         int ns, int R, long long nb_total,
+                                                         const float2* tw1, const float2* tw2,
+                                                         int split, const float2* roots) {
+  const long long gid = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (gid >= nb_total) return;
+  const int nb = n / R;
+  const long long row = gid / nb;
+  const int j = static_cast<int>(gid - row * nb);
+  const float2* s = src + row * n;
+  float2* d = dst + row * n;
+  const int k = j % ns;
+  const int e1 = k * (n / (ns * R));
+  const int base = (j / ns) * ns * R + k;
+  for (int q = 0; q < R; ++q) {
+    float2 acc = make_float2(0.f, 0.f);
+    for (int r = 0; r < R; ++r) {
+      float2 x = s[j + r * nb];
+      if (ns > 1 && r > 0) x = cmul(x, twiddle2(tw1, tw2, split, r * e1));
+      acc = cadd(acc, cmul(x, roots[(static_cast<long long>(r) * q) % R]));
+    }
+    d[base + q * ns] = acc;
+  }
+}
+
+// ---- double precision twins (RFI spectrum) ------------------------------------------------
+__device__ __forceinline__ double2 dcmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pass_d(const double2* __restrict__ src,
+                                                   double2* __restrict__ dst, int n, int ns,
+                                                   int R, long long nb_total,
+                                                   const double2* tw1, const double2* tw2,
+                                                   int split, const double2* roots) {
+  const long long gid = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (gid >= nb_total) return;
+  const int nb = n / R;
+  const long long row = gid / nb;
+  const int j = static_cast<int>(gid - row * nb);
+  const double2* s = src + row * n;
+  double2* d = dst + row * n;
+  const int k = j % ns;
+  const int e1 = k * (n / (ns * R));
+  const int base = (j / ns) * ns * R + k;
+  if (R == 4) {
+    double2 v[4];
+    for (int r = 0; r < 4; ++r) {
+      v[r] = s[j + r * nb];
+      if (ns > 1 && r > 0) {
+        const int e = r * e1, hi = e / split, lo = e - hi * split;
+        v[r] = dcmul(v[r], dcmul(tw1[lo], tw2[hi]));
+      }
+    }
+    double2 a = make_double2(v[0].x + v[2].x, v[0].y + v[2].y);
+    double2 b = make_double2(v[0].x - v[2].x, v[0].y - v[2].y);
+    double2 c = make_double2(v[1].x + v[3].x, v[1].y + v[3].y);
+    double2 e = make_double2(v[1].x - v[3].x, v[1].y - v[3].y);
+    d[base] = make_double2(a.x + c.x, a.y + c.y);
+    d[base + ns] = make_double2(b.x + e.y, b.y - e.x);
+    d[base + 2 * ns] = make_double2(a.x - c.x, a.y - c.y);
+    d[base + 3 * ns] = make_double2(b.x - e.y, b.y + e.x);
+    return;
+  }
+  for (int q = 0; q < R; ++q) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int r = 0; r < R; ++r) {
+      double2 x = s[j + r * nb];
+      if (ns > 1 && r > 0) {
+        const int e = r * e1, hi = e / split, lo = e - hi * split;
+        x = dcmul(x, dcmul(tw1[lo], tw2[hi]));
+      }
+      const double2 w = roots[(static_cast<long long>(r) * q) % R];
+      acc = make_double2(acc.x + x.x * w.x - x.y * w.y, acc.y + x.x * w.y + x.y * w.x);
+    }
+    d[base + q * ns] = acc;
+  }
+}
+
+template <class T>
+std::vector<T> host_roots(size_t n, size_t count, size_t step) {
+  // exp(-2 pi i (m * step) / n) for m < count, computed in long double-free f64 with exact
+  // integer reduction of the angle.
+  std::vector<T> out(count);
+  for (size_t m = 0; m < count; ++m) {
+    const unsigned long long e = (static_cast<unsigned long long>(m) * step) % n;
+    const double a = -2.0 * kPiD * static_cast<double>(e) / static_cast<double>(n);
+    out[m].x = static_cast<decltype(out[m].x)>(std::cos(a));
+    out[m].y = static_cast<decltype(out[m].y)>(std::sin(a));
+  }
+  return out;
+}
+
+template <class Plan, class T>
+void init_plan(Plan& p, size_t n, bool codelets) {
+  p.n = n;
+  p.radix = codelets ? factorize(n) : factorize_d(n);
+  p.split = static_cast<int>(std::ceil(std::sqrt(static_cast<double>(n))));
+  if (p.split < 1) p.split = 1;
+  const size_t n_hi = (n + p.split - 1) / p.split + 1;
+  std::vector<T> t1 = host_roots<T>(n, p.split, 1);
+  std::vector<T> t2 = host_roots<T>(n, n_hi, p.split);
+  p.tw.alloc((t1.size() + t2.size()) * sizeof(T));
+  TGA_HIP(hipMemcpy(p.tw.p, t1.data(), t1.size() * sizeof(T), hipMemcpyHostToDevice));
+  TGA_HIP(hipMemcpy(static_cast<T*>(p.tw.p) + t1.size(), t2.data(), t2.size() * sizeof(T),
+                    hipMemcpyHostToDevice));
+  for (int r : p.radix) {
+    if (codelets && has_codelet(r)) {
+      p.prime_roots.emplace_back();
+      continue;
+    }
+    std::vector<T> roots = host_roots<T>(r, r, 1);
+    DevBuf b(roots.size() * sizeof(T));
+    TGA_HIP(hipMemcpy(b.p, roots.data(), roots.size() * sizeof(T), hipMemcpyHostToDevice));
+    p.prime_roots.push_back(std::move(b));
+  }
+}
+
+std::mutex g_plan_mu;
+std::map<std::pair<int, size_t>, std::unique_ptr<FftPlan>> g_plans;
+std::map<std::pair<int, size_t>, std::unique_ptr<FftPlanD>> g_plans_d;
+
+template <int R>
+void launch_pass(const float2* src, float2* dst, int n, int ns, long long nbt,
+                 const float2* tw1, const float2* tw2, int split, hipStream_t st) {
+  const unsigned grid = static_cast<unsigned>((nbt + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_pass<R>, dim3(grid), dim3(kBlock), 0, st, src, dst, n, ns, nbt, tw1,
+                     tw2, split);
+}
+
+}  // namespace
+
+const FftPlan* fft_plan(size_t n) {
+  if (n == 0) fail(TGA_ERR_INVALID_ARGUMENT, "FFT length must be positive");
+  if (n > (1u << 30)) fail(TGA_ERR_UNSUPPORTED, "FFT length too large");
+  const int dev = current_device_checked();
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  auto key = std::make_pair(dev, n);
+  auto it = g_plans.find(key);
+  if (it != g_plans.end()) return it->second.get();
+  auto p = std::make_unique<FftPlan>();
+  init_plan<FftPlan, float2>(*p, n, true);
+  const FftPlan* raw = p.get();
+  g_plans.emplace(key, std::move(p));
+  return raw;
+}
+
+size_t fft_plan_size(const FftPlan* plan) { return plan->n; }
+
+TwiddleView fft_twiddles(const FftPlan* plan) {
+  TwiddleView v;
+  v.tw1 = plan->tw.as<float2>();
+  v.tw2 = plan->tw.as<float2>() + plan->split;
+  v.split = plan->split;
+  return v;
+}
+
+float2* fft_forward(const FftPlan* plan, float2* data, float2* scratch, size_t batch,
+                    hipStream_t st) {
+  const int n = static_cast<int>(plan->n);
+  if (n == 1 || batch == 0) return data;
+  const float2* tw1 = plan->tw.as<float2>();
+  const float2* tw2 = tw1 + plan->split;
+  float2* src = data;
+  float2* dst = scratch;
+  int ns = 1;
+  for (size_t pi = 0; pi < plan->radix.size(); ++pi) {
+    const int R = plan->radix[pi];
+    const long long nbt = static_cast<long long>(n / R) * static_cast<long long>(batch);
+    switch (R) {
+      case 2: launch_pass<2>(src, dst, n, ns, nbt, tw1, tw2, plan->split, st); break;
+      case 3: launch_pass<3>(src, dst, n, ns, nbt, tw1, tw2, plan->split, st); break;
+      case 4: launch_pass<4>(src, dst, n, ns, nbt, tw1, tw2, plan->split, st); break;
+      case 5: launch_pass<5>(src, dst, n, ns, nbt, tw1, tw2, plan->split, st); break;
+      case 7: launch_pass<7>(src, dst, n, ns, nbt, tw1, tw2, plan->split, st); break;
+      case 8: launch_pass<8>(src, dst, n, ns, nbt, tw1, tw2, plan->split, st); break;
+      case 11: launch_pass<11>(src, dst, n, ns, nbt, tw1, tw2, plan->split, st); break;
+      case 13: launch_pass<13>(src, dst, n, ns, nbt, tw1, tw2, plan->split, st); break;
+      case 16: launch_pass<16>(src, dst, n, ns, nbt, tw1, tw2, plan->split, st); break;
+      case 31: launch_pass<31>(src, dst, n, ns, nbt, tw1, tw2, plan->split, st); break;
+      default: {
+        const unsigned grid = static_cast<unsigned>((nbt + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_pass_generic, dim3(grid), dim3(kBlock), 0, st, src, dst, n, ns, R,
+                           nbt, tw1, tw2, plan->split, plan->prime_roots[pi].as<float2>());
+      }
+    }
+    TGA_HIP(hipGetLastError());
+    ns *= R;
+    std::swap(src, dst);
+  }
+  return src;
+}
+
+const FftPlanD* fft_plan_d(size_t n) {
+  if (n == 0) fail(TGA_ERR_INVALID_ARGUMENT, "FFT length must be positive");
+  if (n > (1u << 30)) fail(TGA_ERR_UNSUPPORTED, "FFT length too large");
+  const int dev = current_device_checked();
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  auto key = std::make_pair(dev, n);
+  auto it = g_plans_d.find(key);
+  if (it != g_plans_d.end()) return it->second.get();
+  auto p = std::make_unique<FftPlanD>();
+  init_plan<FftPlanD, double2>(*p, n, false);
+  const FftPlanD* raw = p.get();
+  g_plans_d.emplace(key, std::move(p));
+  return raw;
+}
+
+double2* fft_forward_d(const FftPlanD* plan, double2* data, double2* scratch, size_t batch,
+                       hipStream_t st) {
+  const int n = static_cast<int>(plan->n);
+  if (n == 1 || batch == 0) return data;
+  const double2* tw1 = plan->tw.as<double2>();
+  const double2* tw2 = tw1 + plan->split;
+  double2* src = data;
+  double2* dst = scratch;
+  int ns = 1;
+  for (size_t pi = 0; pi < plan->radix.size(); ++pi) {
+    const int R = plan->radix[pi];
+    const long long nbt = static_cast<long long>(n / R) * static_cast<long long>(batch);
+    const unsigned grid = static_cast<unsigned>((nbt + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_pass_d, dim3(grid), dim3(kBlock), 0, st, src, dst, n, ns, R, nbt, tw1,
+                       tw2, plan->split, plan->prime_roots[pi].as<double2>());
+    TGA_HIP(hipGetLastError());
+    ns *= R;
+    std::swap(src, dst);
+  }
+  return src;
+}
+
+}  // namespace tga

@@ -1,0 +1,437 @@
+// Level-3 batched search plans: every antenna, PRN, Doppler bin and hypothesis of every
+// system of one observation in a handful of launches (replaces the rayon loops of
+// acquire_all_* -- acquisition.rs:261-396 and its five siblings).
+//
+// Systems that share (code period P, coherent length N, bin grid) share one group: one
+// forward-spectrum buffer and one rows launch over all their PRNs (GPS, SBAS and QZSS on the
+// reference's common grid form one group of 100 PRNs).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "internal.h"
+#include "plan.h"
+#include "search.h"
+
+namespace tga {
+
+namespace {
+
+template <class T>
+void upload(DevBuf& b, const std::vector<T>& h, hipStream_t st) {
+  b.alloc(h.size() * sizeof(T));
+  TGA_HIP(hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+  TGA_HIP(hipStreamSynchronize(st));  // the host vector may be a temporary
+}
+
+std::vector<float2> tau_table(size_t P) {
+  std::vector<float2> t(P);
+  for (size_t n = 0; n < P; ++n) {
+    const double a = kPi * static_cast<double>(n) / static_cast<double>(P);
+    t[n] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(-std::sin(a)));
+  }
+  return t;
+}
+
+}  // namespace
+
+Plan::~Plan() {
+  for (auto& t : timers) {
+    if (t.start) (void)hipEventDestroy(t.start);
+    if (t.stop) (void)hipEventDestroy(t.stop);
+  }
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Plan::build(const tga_plan_desc& d) {
+  if (!(d.sampling_freq > 0.0) || !std::isfinite(d.sampling_freq))
+    fail(TGA_ERR_INVALID_ARGUMENT, "sampling_freq must be positive");
+  if (d.n_ant < 1) fail(TGA_ERR_INVALID_ARGUMENT, "n_ant must be >= 1");
+  if (d.record_samples < 1) fail(TGA_ERR_INVALID_ARGUMENT, "record_samples must be >= 1");
+  if (d.input_format != TGA_INPUT_1BIT && d.input_format != TGA_INPUT_2BIT)
+    fail(TGA_ERR_INVALID_ARGUMENT, "input_format must be TGA_INPUT_1BIT or TGA_INPUT_2BIT");
+  if (d.n_systems < 1 || d.systems == nullptr)
+    fail(TGA_ERR_INVALID_ARGUMENT, "a plan needs at least one system");
+  current_device_checked();
+  fs = d.sampling_freq;
+  n_ant = d.n_ant;
+  L = d.record_samples;
+  fmt = d.input_format;
+  const int bits = fmt == TGA_INPUT_1BIT ? 1 : 2;
+  row_bytes = static_cast<size_t>((L * bits + 7) / 8);
+  phase_const = (2.0 * kPi) / fs;  // TAU / fs (acquisition.rs:289)
+  TGA_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+
+  // ---- systems -> groups ----------------------------------------------------------------
+  size_t peak_off = 0;
+  for (int s = 0; s < d.n_systems; ++s) {
+    const tga_system_desc& sd = d.systems[s];
+    const SystemInfo& info = system_info(sd.system);
+    SysEntry e;
+    e.system = sd.system;
+    if (sd.n_prn == 0 && sd.prns == nullptr) {
+      for (int p = info.prn_first; p <= info.prn_last; ++p) e.prns.push_back(p);
+    } else {
+      if (sd.n_prn < 1 || sd.prns == nullptr)
+        fail(TGA_ERR_INVALID_ARGUMENT, "n_prn / prns inconsistent");
+      e.prns.assign(sd.prns, sd.prns + sd.n_prn);
+    }
+    for (int p : e.prns) check_prn(sd.system, p);
+    if (sd.n_fc < 1 || sd.fc == nullptr)
+      fail(TGA_ERR_EMPTY_BINS, "frequency bin list must be non-empty");
+    std::vector<double> fc(sd.fc, sd.fc + sd.n_fc);
+    for (double f : fc)
+      if (!std::isfinite(f)) fail(TGA_ERR_INVALID_ARGUMENT, "bin frequencies must be finite");
+    const size_t P = samples_per_period(sd.system, fs);
+    if (P == 0) fail(TGA_ERR_INVALID_ARGUMENT, "sampling rate below one sample per period");
+    size_t N = sd.num_samples > 0 ? static_cast<size_t>(sd.num_samples)
+                                  : std::min<size_t>(2 * P, static_cast<size_t>(L));
+    if (N > static_cast<size_t>(L))
+      fail(TGA_ERR_INVALID_ARGUMENT, "num_samples exceeds the record length");
+    if (N > (1u << 26)) fail(TGA_ERR_UNSUPPORTED, "coherent length too large");
+    e.peak_offset = peak_off;
+    peak_off += static_cast<size_t>(n_ant) * e.prns.size();
+    // replica of N samples (acquisition.rs:219); folding needs c[n+P] == c[n]
+    std::vector<std::vector<double>> codes;
+    bool fold = (N == 2 * P);
+    for (int p : e.prns) {
+      std::vector<double> c(N);
+      resample(code_chips(sd.system, p), static_cast<double>(P), N, c.data());
+      if (fold)
+        for (size_t n = 0; n < P; ++n)
+          if (c[n + P] != c[n]) { fold = false; break; }
+      codes.push_back(std::move(c));
+    }
+    // find or create the group
+    Group* g = nullptr;
+    for (auto& gg : groups)
+      if (gg.P == P && gg.N == N && gg.fc == fc && gg.folded == fold) g = &gg;
+    if (!g) {
+      groups.emplace_back();
+      g = &groups.back();
+      g->P = P;
+      g->N = N;
+      g->fc = fc;
+      g->folded = fold;
+      g->fused = fold && fused_supported(P, N);
+      g->M = fold ? P : N;
+    }
+    e.group = static_cast<int>(g - groups.data());
+    e.first_prn = static_cast<int>(g->codes.size());
+    for (auto& c : codes) g->codes.push_back(std::move(c));
+    for (size_t i = 0; i < e.prns.size(); ++i) {
+      PeakSlot sl;
+      sl.base = static_cast<int32_t>(e.peak_offset);
+      sl.stride = static_cast<int32_t>(e.prns.size());
+      sl.local = static_cast<int32_t>(i);
+      sl.pad = 0;
+      g->slots.push_back(sl);
+    }
+    systems.push_back(std::move(e));
+  }
+  n_peaks = peak_off;
+
+  // ---- device state per group -------------------------------------------------------------
+  d_means.alloc(static_cast<size_t>(n_ant) * sizeof(double));
+  d_input.alloc(static_cast<size_t>(n_ant) * row_bytes);
+  d_peaks.alloc(n_peaks * sizeof(tga_peak));
+  d_nan.alloc(sizeof(int));
+  for (auto& g : groups) {
+    const int n_prn = static_cast<int>(g.codes.size());
+    const int n_bin = static_cast<int>(g.fc.size());
+    const size_t M = g.M;
+    g.scale = 1.0f / std::sqrt(static_cast<float>(g.N));  // correlate.rs:123
+    g.cw = static_cast<int>(std::ceil(fs / 1.023e6));     // correlate.rs:157
+    upload(g.d_fc, g.fc, stream);
+    upload(g.d_slots, g.slots, stream);
+    // replica time series -> spectra (plan creation only)
+    std::vector<float2> h(static_cast<size_t>(n_prn) * 2 * M);
+    std::vector<float2> tau;
+    if (g.folded) {
+      tau = tau_table(g.P);
+      upload(g.d_tau, tau, stream);
+    }
+    for (int p = 0; p < n_prn; ++p) {
+      const std::vector<double>& c = g.codes[p];
+      float2* h0 = h.data() + (static_cast<size_t>(p) * 2) * M;
+      float2* h1 = h0 + M;
+      for (size_t n = 0; n < M; ++n) {
+        const float cv = static_cast<float>(c[n]);
+        h0[n] = make_float2(cv, 0.f);
+        if (g.folded) {
+          h1[n] = make_float2(cv * tau[n].x, cv * tau[n].y);
+        } else {
+          const float a = n < std::min(g.P, g.N) ? cv : -cv;  // correlate.rs:53-56
+          h1[n] = make_float2(a, 0.f);
+        }
+      }
+    }
+    DevBuf a;
+    upload(a, h, stream);
+    DevBuf sc(h.size() * sizeof(float2));
+    float2* r = fft_forward(fft_plan(M), a.as<float2>(), sc.as<float2>(),
+                            static_cast<size_t>(n_prn) * 2, stream);
+    TGA_HIP(hipStreamSynchronize(stream));
+    g.d_rep = std::move(r == a.as<float2>() ? a : sc);
+    g.codes.clear();
+    g.codes.shrink_to_fit();
+    // forward spectra and per-row results
+    g.n_xrow = static_cast<size_t>(n_ant) * n_bin * (g.folded ? 2 : 1);
+    g.d_X.alloc(g.n_xrow * M * sizeof(float2));
+    g.n_rows = static_cast<long long>(n_ant) * n_prn * n_bin * 2;
+    g.d_rows.alloc(static_cast<size_t>(g.n_rows) * sizeof(RowOut));
+    g.n_prn = n_prn;
+    g.n_bin = n_bin;
+    if (g.fused) {
+      upload(g.d_tw, fused_twiddles(), stream);
+    } else {
+      g.d_Xs.alloc(g.n_xrow * M * sizeof(float2));
+      // rows are processed in chunks of at most ~32 Mi complex values (256 MiB) per buffer
+      const long long cap = std::max<long long>(1, (32LL << 20) / static_cast<long long>(M));
+      g.chunk_rows = std::min<long long>(cap, g.n_rows);
+      g.d_Z.alloc(static_cast<size_t>(g.chunk_rows) * M * sizeof(float2));
+      g.d_Zs.alloc(static_cast<size_t>(g.chunk_rows) * M * sizeof(float2));
+    }
+    // algorithmic bytes of the rows stage: each row reads its forward spectrum and replica
+    // spectrum once (M complex f32 each)
+    g.row_bytes_alg = static_cast<double>(g.n_rows) * 2.0 * static_cast<double>(M) * 8.0;
+    g.fwd_bytes_alg = static_cast<double>(g.n_xrow) * static_cast<double>(M) * 8.0;
+  }
+  TGA_HIP(hipStreamSynchronize(stream));
+  for (auto& t : timers) {
+    TGA_HIP(hipEventCreate(&t.start));
+    TGA_HIP(hipEventCreate(&t.stop));
+  }
+}
+
+void Plan::time_begin(int slot, hipStream_t st) {
+  if (timing) TGA_HIP(hipEventRecord(timers[slot].start, st));
+}
+
+void Plan::time_end(int slot, hipStream_t st, double bytes) {
+  if (!timing) return;
+  TGA_HIP(hipEventRecord(timers[slot].stop, st));
+  timers[slot].pending = true;
+  timers[slot].bytes = bytes;
+}
+
+void Plan::time_collect() {
+  if (!timing) return;
+  for (auto& t : timers) {
+    if (!t.pending) continue;
+    float ms = 0.f;
+    TGA_HIP(hipEventElapsedTime(&ms, t.start, t.stop));
+    t.total_ms += ms;
+    t.launches += 1;
+    t.pending = false;
+  }
+}
+
+void Plan::search(const uint8_t* d_in, hipStream_t st) {
+  if (st == nullptr) st = stream;
+  TGA_HIP(hipMemsetAsync(d_nan.p, 0, sizeof(int), st));
+  launch_ant_sums(d_in, row_bytes, n_ant, L, fmt, d_means.as<double>(), st);
+  double rows_bytes = 0.0, fwd_bytes = 0.0;
+  for (auto& g : groups) {
+    rows_bytes += g.row_bytes_alg;
+    fwd_bytes += g.fwd_bytes_alg;
+  }
+  // forward spectra of every group
+  time_begin(1, st);
+  for (auto& g : groups) {
+    const int M = static_cast<int>(g.M);
+    if (g.fused) {
+      launch_fused_forward(d_in, row_bytes, fmt, d_means.as<double>(), g.d_fc.as<double>(),
+                           n_ant, g.n_bin, phase_const, g.d_tau.as<float2>(),
+                           g.d_tw.as<float2>(), g.d_X.as<float2>(), st);
+      g.X = g.d_X.as<float2>();
+    } else {
+      if (g.folded)
+        launch_fold_obs(d_in, row_bytes, fmt, d_means.as<double>(), g.d_fc.as<double>(), n_ant,
+                        g.n_bin, M, phase_const, g.d_tau.as<float2>(), g.d_X.as<float2>(), st);
+      else
+        launch_wipe_obs(d_in, row_bytes, fmt, d_means.as<double>(), g.d_fc.as<double>(), n_ant,
+                        g.n_bin, M, phase_const, g.d_X.as<float2>(), st);
+      g.X = fft_forward(fft_plan(g.M), g.d_X.as<float2>(), g.d_Xs.as<float2>(), g.n_xrow, st);
+    }
+  }
+  time_end(1, st, fwd_bytes);
+  // correlation rows of every group
+  time_begin(0, st);
+  for (auto& g : groups) {
+    const int M = static_cast<int>(g.M);
+    if (g.fused) {
+      launch_fused_rows(g.X, g.d_rep.as<float2>(), g.d_tw.as<float2>(), n_ant, g.n_bin, g.n_prn,
+                        g.cw, g.d_rows.as<RowOut>(), st);
+    } else {
+      const FftPlan* fp = fft_plan(g.M);
+      for (long long r0 = 0; r0 < g.n_rows; r0 += g.chunk_rows) {
+        const long long r1 = std::min(g.n_rows, r0 + g.chunk_rows);
+        launch_mul_rows(g.X, g.d_rep.as<float2>(), M, g.n_prn, g.n_bin, g.folded, r0, r1,
+                        g.d_Z.as<float2>(), st);
+        float2* z = fft_forward(fp, g.d_Z.as<float2>(), g.d_Zs.as<float2>(),
+                                static_cast<size_t>(r1 - r0), st);
+        launch_row_reduce(z, M, static_cast<int>(g.P), g.cw, r1 - r0,
+                          g.d_rows.as<RowOut>() + r0, st);
+      }
+    }
+  }
+  time_end(0, st, rows_bytes);
+  time_begin(2, st);
+  for (auto& g : groups)
+    launch_select(g.d_rows.as<RowOut>(), g.d_slots.as<PeakSlot>(), n_ant, g.n_prn, g.n_bin,
+                  g.folded ? 2.0f : 1.0f, g.scale, d_peaks.as<tga_peak>(), d_nan.as<int>(), st);
+  time_end(2, st, 0.0);
+}
+
+void Plan::finish(tga_peak* out, hipStream_t st) {
+  if (st == nullptr) st = stream;
+  int nan = 0;
+  TGA_HIP(hipMemcpyAsync(out, d_peaks.p, n_peaks * sizeof(tga_peak), hipMemcpyDeviceToHost, st));
+  TGA_HIP(hipMemcpyAsync(&nan, d_nan.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  TGA_HIP(hipStreamSynchronize(st));
+  time_collect();
+  if (nan)
+    fail(TGA_ERR_NAN, "called `Option::unwrap()` on a `None` value (NaN in the correlation)");
+}
+
+std::string Plan::describe() const {
+  std::string s;
+  for (const auto& e : systems) {
+    const Group& g = groups[e.group];
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "%s: P=%zu N=%zu bins=%zu prns=%zu group=%d path=%s\n",
+                  system_info(e.system).key, g.P, g.N, g.fc.size(), e.prns.size(), e.group,
+                  g.fused ? "fused16368" : (g.folded ? "generic-folded" : "generic-unfolded"));
+    s += buf;
+  }
+  return s;
+}
+
+}  // namespace tga
+
+using namespace tga;
+
+extern "C" {
+
+int tga_plan_create(const tga_plan_desc* desc, tga_plan** plan) {
+  return plan_guard([&] {
+    if (!desc || !plan) fail(TGA_ERR_INVALID_ARGUMENT, "desc and plan must not be null");
+    *plan = nullptr;
+    auto p = std::make_unique<Plan>();
+    p->build(*desc);
+    *plan = reinterpret_cast<tga_plan*>(p.release());
+  });
+}
+
+int tga_plan_destroy(tga_plan* plan) {
+  return plan_guard([&] { delete reinterpret_cast<Plan*>(plan); });
+}
+
+int tga_plan_row_bytes(const tga_plan* plan, size_t* row_bytes) {
+  return plan_guard([&] {
+    if (!plan || !row_bytes) fail(TGA_ERR_INVALID_ARGUMENT, "null argument");
+    *row_bytes = reinterpret_cast<const Plan*>(plan)->row_bytes;
+  });
+}
+
+int tga_plan_peak_count(const tga_plan* plan, size_t* count) {
+  return plan_guard([&] {
+    if (!plan || !count) fail(TGA_ERR_INVALID_ARGUMENT, "null argument");
+    *count = reinterpret_cast<const Plan*>(plan)->n_peaks;
+  });
+}
+
+int tga_plan_prns(const tga_plan* plan, int32_t sys_index, int32_t* prns, size_t cap,
+                  size_t* n_prn) {
+  return plan_guard([&] {
+    if (!plan || !n_prn) fail(TGA_ERR_INVALID_ARGUMENT, "null argument");
+    const Plan* p = reinterpret_cast<const Plan*>(plan);
+    if (sys_index < 0 || sys_index >= static_cast<int>(p->systems.size()))
+      fail(TGA_ERR_INVALID_ARGUMENT, "system index out of range");
+    const auto& v = p->systems[sys_index].prns;
+    *n_prn = v.size();
+    if (prns) {
+      if (cap < v.size()) fail(TGA_ERR_INVALID_ARGUMENT, "capacity too small");
+      std::copy(v.begin(), v.end(), prns);
+    }
+  });
+}
+
+int tga_search(tga_plan* plan, const uint8_t* packed, size_t n_ant, size_t row_bytes,
+               tga_peak* out) {
+  return plan_guard([&] {
+    if (!plan || !packed || !out) fail(TGA_ERR_INVALID_ARGUMENT, "null argument");
+    Plan* p = reinterpret_cast<Plan*>(plan);
+    if (n_ant != static_cast<size_t>(p->n_ant))
+      fail(TGA_ERR_ANTENNA_RANGE, "antenna count " + std::to_string(n_ant) +
+                                      " does not match the plan's " + std::to_string(p->n_ant));
+    if (row_bytes != p->row_bytes)
+      fail(TGA_ERR_INVALID_ARGUMENT, "row_bytes " + std::to_string(row_bytes) +
+                                         " does not match the plan's " +
+                                         std::to_string(p->row_bytes));
+    TGA_HIP(hipMemcpyAsync(p->d_input.p, packed, n_ant * row_bytes, hipMemcpyHostToDevice,
+                           p->stream));
+    p->search(p->d_input.as<uint8_t>(), p->stream);
+    p->finish(out, p->stream);
+  });
+}
+
+int tga_search_device(tga_plan* plan, const void* d_packed, void* stream, tga_peak* out) {
+  return plan_guard([&] {
+    if (!plan || !d_packed || !out) fail(TGA_ERR_INVALID_ARGUMENT, "null argument");
+    Plan* p = reinterpret_cast<Plan*>(plan);
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : p->stream;
+    p->search(static_cast<const uint8_t*>(d_packed), st);
+    p->finish(out, st);
+  });
+}
+
+int tga_plan_set_timing(tga_plan* plan, int32_t enable) {
+  return plan_guard([&] {
+    if (!plan) fail(TGA_ERR_INVALID_ARGUMENT, "null plan");
+    Plan* p = reinterpret_cast<Plan*>(plan);
+    p->timing = enable != 0;
+    for (auto& t : p->timers) {
+      t.total_ms = 0;
+      t.launches = 0;
+      t.pending = false;
+    }
+  });
+}
+
+int tga_plan_get_timing(tga_plan* plan, int32_t kernel, double* total_ms, int64_t* launches,
+                        double* bytes_per_launch) {
+  return plan_guard([&] {
+    if (!plan || !total_ms || !launches) fail(TGA_ERR_INVALID_ARGUMENT, "null argument");
+    Plan* p = reinterpret_cast<Plan*>(plan);
+    if (kernel < 0 || kernel >= static_cast<int>(p->timers.size()))
+      fail(TGA_ERR_INVALID_ARGUMENT, "timing slot out of range");
+    *total_ms = p->timers[kernel].total_ms;
+    *launches = p->timers[kernel].launches;
+    if (bytes_per_launch) *bytes_per_launch = p->timers[kernel].bytes;
+  });
+}
+
+const char* tga_plan_kernel_name(const tga_plan* plan, int32_t kernel) {
+  const Plan* p = reinterpret_cast<const Plan*>(plan);
+  const bool fused = p && !p->groups.empty() && p->groups[0].fused;
+  switch (kernel) {
+    case 0: return fused ? "k_rows" : "k_mul_rows+k_pass+k_row_reduce";
+    case 1: return fused ? "k_forward" : "k_wipe_obs+k_pass";
+    case 2: return "k_select";
+  }
+  return "";
+}
+
+int tga_plan_describe(const tga_plan* plan, char* buf, size_t cap) {
+  return plan_guard([&] {
+    if (!plan || !buf || cap == 0) fail(TGA_ERR_INVALID_ARGUMENT, "null argument");
+    std::string s = reinterpret_cast<const Plan*>(plan)->describe();
+    std::strncpy(buf, s.c_str(), cap - 1);
+    buf[cap - 1] = 0;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,76 @@
+// Batched search plan (level 3) shared by plan.cpp and the JSON host layer.
+#pragma once
+
+#include <array>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+#include "search.h"
+
+namespace tga {
+
+constexpr double kPi = 3.14159265358979323846;
+
+struct Group {
+  size_t P = 0, N = 0, M = 0;  // code period, coherent length, transform length
+  std::vector<double> fc;
+  bool folded = false, fused = false;
+  std::vector<std::vector<double>> codes;  // replicas, plan creation only
+  std::vector<PeakSlot> slots;
+  int n_prn = 0, n_bin = 0;
+  size_t n_xrow = 0;
+  long long n_rows = 0, chunk_rows = 0;
+  float scale = 1.f;
+  int cw = 1;
+  DevBuf d_fc, d_slots, d_tau, d_rep, d_X, d_Xs, d_rows, d_Z, d_Zs, d_tw;
+  float2* X = nullptr;  // where the forward spectra of the last search landed
+  double row_bytes_alg = 0, fwd_bytes_alg = 0;
+};
+
+struct SysEntry {
+  int system = 0;
+  std::vector<int> prns;
+  int group = 0;
+  int first_prn = 0;
+  size_t peak_offset = 0;
+};
+
+struct Timer {
+  hipEvent_t start = nullptr, stop = nullptr;
+  bool pending = false;
+  double total_ms = 0.0;
+  long long launches = 0;
+  double bytes = 0.0;
+};
+
+struct Plan {
+  double fs = 0, phase_const = 0;
+  int n_ant = 0, fmt = 1;
+  long long L = 0;
+  size_t row_bytes = 0, n_peaks = 0;
+  std::vector<SysEntry> systems;
+  std::vector<Group> groups;
+  DevBuf d_means, d_input, d_peaks, d_nan;
+  hipStream_t stream = nullptr;
+  bool timing = false;
+  std::array<Timer, 3> timers;
+
+  ~Plan();
+  void build(const tga_plan_desc& d);
+  void search(const uint8_t* d_in, hipStream_t st);
+  void finish(tga_peak* out, hipStream_t st);
+  std::string describe() const;
+  void time_begin(int slot, hipStream_t st);
+  void time_end(int slot, hipStream_t st, double bytes);
+  void time_collect();
+};
+
+// Error-to-status wrapper shared by the C ABI translation units.
+int plan_guard_impl(void (*fn)(void*), void* ctx);
+template <class F>
+int plan_guard(F&& f) {
+  return plan_guard_impl([](void* c) { (*static_cast<F*>(c))(); }, &f);
+}
+
+}  // namespace tga

@@ -1,0 +1,72 @@
+// Kernel launchers of the acquisition search (search.hip), called by the host plan code.
+#pragma once
+
+#include "internal.h"
+
+namespace tga {
+
+// Per-row result of one (antenna, PRN, bin, hypothesis) correlation row, before scaling:
+// p1 = max |z|^2 over code phases, i1 its index (last maximum on ties), p2 = max |z|^2 more
+// than `cw` samples (ring distance modulo the code period) from i1 (-1 if none),
+// nan = 1 if any |z|^2 was NaN.
+struct RowOut {
+  float p1;
+  float p2;
+  int32_t i1;
+  int32_t nan;
+};
+
+// Maps the global PRN index of a group to its tga_peak slot: out = base + ant * stride + local.
+struct PeakSlot {
+  int32_t base;
+  int32_t stride;
+  int32_t local;
+  int32_t pad;
+};
+
+// ---- decoding / de-meaning --------------------------------------------------------------
+void launch_ant_sums(const uint8_t* packed, size_t row_bytes, int n_ant, int64_t L, int fmt,
+                     double* means, hipStream_t st);
+
+// ---- fused path, code period P = 16368 (16.368 MHz C/A) --------------------------------
+constexpr int kFusedP = 16368;
+bool fused_supported(size_t P, size_t N);
+// X[(ant*n_bin + bin)*2 + h][P]: FFT of the folded sum (h=0) / twisted difference (h=1).
+void launch_fused_forward(const uint8_t* packed, size_t row_bytes, int fmt, const double* means,
+                          const double* fc, int n_ant, int n_bin, double phase_const,
+                          const float2* tau, const float2* tw, float2* X, hipStream_t st);
+// rows[((ant*n_prn + prn)*n_bin + bin)*2 + h]
+void launch_fused_rows(const float2* X, const float2* rep, const float2* tw, int n_ant,
+                       int n_bin, int n_prn, int cw, RowOut* rows, hipStream_t st);
+// Twiddle table of the fused kernels: W_P^i (i < 128) then W_P^{128 i} (i < 128).
+std::vector<float2> fused_twiddles();
+
+// ---- generic path -------------------------------------------------------------------------
+// folded input rows (S / twisted E) for any P, N = 2P: out[(ant*n_bin + bin)*2 + h][P]
+void launch_fold_obs(const uint8_t* packed, size_t row_bytes, int fmt, const double* means,
+                     const double* fc, int n_ant, int n_bin, int P, double phase_const,
+                     const float2* tau, float2* out, hipStream_t st);
+// unfolded wiped rows for N-point correlation: out[ant*n_bin + bin][N]
+void launch_wipe_obs(const uint8_t* packed, size_t row_bytes, int fmt, const double* means,
+                     const double* fc, int n_ant, int n_bin, int N, double phase_const,
+                     float2* out, hipStream_t st);
+// level 1: out[bin][N] from f32 samples and f64 phase points
+void launch_wipe_signal(const float* sig, const double* pp, const double* fc, int n_bin, int N,
+                        float2* out, hipStream_t st);
+// Z[r][i] = conj(X[xrow(r)][i]) * rep[rep(r)][i] for rows r in [r0, r1) of the ordering
+// r = ((ant*n_prn + prn)*n_bin + bin)*2 + h; folded: xrow = (ant*n_bin+bin)*2+h, else
+// xrow = ant*n_bin+bin; rep index = prn*2 + h.
+void launch_mul_rows(const float2* X, const float2* rep, int M, int n_prn, int n_bin,
+                     bool folded, long long r0, long long r1, float2* Z, hipStream_t st);
+// One block per row of Z: argmax / masked second maximum of |z|^2 (ring distance mod P).
+void launch_row_reduce(const float2* Z, int M, int P, int cw, long long n_rows, RowOut* out,
+                       hipStream_t st);
+
+// ---- selection ----------------------------------------------------------------------------
+// Per (ant, prn): last-max over bins of the row peak, [c,-c] only if strictly larger; writes
+// tga_peak through `slots`; sets *nan_flag if any row saw a NaN.  corr = sqrt(p)*gain*scale
+// with gain 2 for folded rows, 1 for unfolded ones.
+void launch_select(const RowOut* rows, const PeakSlot* slots, int n_ant, int n_prn, int n_bin,
+                   float gain, float scale, tga_peak* out, int* nan_flag, hipStream_t st);
+
+}  // namespace tga

@@ -1,0 +1,598 @@
+// Acquisition-search kernels for gfx950.
+//
+// The reference correlates, per (antenna, PRN, bin, hypothesis h), an N = 2P point wiped
+// signal against the replica [c,c] (h=0) or [c,-c] (h=1) (correlate.rs:107-146).  In exact
+// arithmetic both reduce to P-point cyclic problems:
+//   h=0:  |y[m]| = |IFFT_P(FFT_P(s) . conj FFT_P(c))[m]|,      s[n] = x[n] + x[n+P]
+//   h=1:  |y[m]| = |IFFT_P(FFT_P(e) . conj FFT_P(c tau))[m]|,  e[n] = (x[n] - x[n+P]) tau[n]
+// with tau[n] = exp(-i pi n / P), |y[m+P]| = |y[m]|, and an extra factor 2 on the P-point
+// results (unnormalised 2P-point transforms carry twice the gain).  For P = 16368 = 31*11*3*16 a whole
+// P-point transform fits the 160 KB LDS of a gfx950 CU (128 KB of complex f32), so one
+// workgroup runs the four Stockham passes on chip:
+//   forward kernel  (antenna, bin, h): decode + de-mean + wipe-off + fold + FFT -> X in HBM
+//   rows kernel     (antenna, bin, h) x PRN group: conj(X) . R -> FFT -> |.|^2 -> argmax and
+//                   masked second maximum, X kept in registers across the group's PRNs.
+// |IFFT(Z)| = |FFT(conj Z)| so every transform here is a forward one.
+#include "fft_device.h"
+#include "search.h"
+
+namespace tga {
+
+namespace {
+
+// ---- sample decoding ------------------------------------------------------------------------
+// 1-bit: MSB-first, bit 1 -> +1, 0 -> -1 (observation.rs:140-151, 232-240).
+// 2-bit: MSB-first pairs (sign, magnitude) -> (sign ? +1 : -1) * (magnitude ? 3 : 1).
+__device__ __forceinline__ int decode_level(const uint8_t* row, int fmt, long long n) {
+  if (fmt == TGA_INPUT_1BIT) {
+    const uint8_t b = row[n >> 3];
+    return ((b >> (7 - static_cast<int>(n & 7))) & 1) ? 1 : -1;
+  }
+  const uint8_t b = row[n >> 2];
+  const int pair = (b >> (6 - 2 * static_cast<int>(n & 3))) & 3;
+  const int mag = (pair & 1) ? 3 : 1;
+  return (pair & 2) ? mag : -mag;
+}
+
+// Wiped sample x[n] = s[n] e^{+j phi}, phi = f32(f64(pc * n) * f), s = f32(level - mean)
+// (correlate.rs:110-115, acquisition.rs:284-291); cos/sin of the f32 phase evaluated in f64
+// and rounded once, i.e. correctly rounded like a <1 ULP libm cosf/sinf.
+__device__ __forceinline__ float2 wipe_value(float s, double pp, double f) {
+  const float ph = __double2float_rn(__dmul_rn(pp, f));
+  double sd, cd;
+  sincos(static_cast<double>(ph), &sd, &cd);
+  const float c = __double2float_rn(cd);
+  const float sn = __double2float_rn(sd);
+  return make_float2(c * s, sn * s);
+}
+
+__device__ __forceinline__ float2 wipe_obs(const uint8_t* row, int fmt, long long n, double mean,
+                                           double pc, double f) {
+  const float s = __double2float_rn(static_cast<double>(decode_level(row, fmt, n)) - mean);
+  return wipe_value(s, __dmul_rn(pc, static_cast<double>(n)), f);
+}
+
+// ---- block reductions (wave64 shuffles, then LDS across waves) --------------------------------
+__device__ __forceinline__ void arg_combine(float& p, int& i, float q, int j) {
+  if (q > p || (q == p && j > i)) {
+    p = q;
+    i = j;
+  }
+}
+
+template <int NT>
+__device__ __forceinline__ void block_argmax(float& p, int& i, float* sp, int* si) {
+  constexpr int NW = (NT + 63) / 64;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const float q = __shfl_xor(p, off, 64);
+    const int j = __shfl_xor(i, off, 64);
+    arg_combine(p, i, q, j);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    sp[wave] = p;
+    si[wave] = i;
+  }
+  __syncthreads();
+  p = sp[0];
+  i = si[0];
+#pragma unroll
+  for (int w = 1; w < NW; ++w) arg_combine(p, i, sp[w], si[w]);
+  __syncthreads();
+}
+
+template <int NT>
+__device__ __forceinline__ float block_max(float v, float* sp) {
+  constexpr int NW = (NT + 63) / 64;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) sp[wave] = v;
+  __syncthreads();
+  v = sp[0];
+#pragma unroll
+  for (int w = 1; w < NW; ++w) v = fmaxf(v, sp[w]);
+  __syncthreads();
+  return v;
+}
+
+__device__ __forceinline__ int ring_dist(int a, int b, int P) {
+  int d = a > b ? a - b : b - a;
+  return d < P - d ? d : P - d;
+}
+
+// ---- antenna sums ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_ant_sums(const uint8_t* __restrict__ packed,
+                                                  size_t row_bytes, long long L, int fmt,
+                                                  double* __restrict__ means) {
+  __shared__ long long red[4];
+  const uint8_t* row = packed + static_cast<size_t>(blockIdx.x) * row_bytes;
+  long long acc = 0;
+  for (long long n = threadIdx.x; n < L; n += 256) acc += decode_level(row, fmt, n);
+  // reduce 64-bit sums
+  for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const long long s = red[0] + red[1] + red[2] + red[3];
+    means[blockIdx.x] = static_cast<double>(s) / static_cast<double>(L);
+  }
+}
+
+// ================================ fused P = 16368 path ===================================
+namespace fused {
+
+constexpr int P = kFusedP;  // 31 * 11 * 3 * 16
+// 512 threads = 8 waves = 2 per SIMD: every SIMD gets the same number of waves, and the
+// register budget is 256 VGPRs (no spills).  Per pass the butterflies per thread are
+// R31: 1 (+ one direct output of the 16 left-over butterflies), R11: 3, R3: 11, R16: 2.
+constexpr int T = 512;
+constexpr int NB1 = P / 31;  // 528
+constexpr int NB4 = P / 16;  // 1023
+constexpr int NW = T / 64;
+constexpr int LEFT = NB1 - T;  // 16 radix-31 butterflies beyond one per thread
+
+__device__ __forceinline__ float2 tw_lds(const float2* stw, int e) {
+  return cmul(stw[e & 127], stw[128 + (e >> 7)]);
+}
+
+// In-LDS Stockham pass of radix R after NS points of sub-transforms (in place: every thread
+// holds its butterflies in registers across the barrier).
+template <int R, int NS>
+__device__ __forceinline__ void lds_pass(float2* buf, const float2* stw, int tid) {
+  // Opaque copy of the thread index: inside the PRN loop of the rows kernel this keeps the
+  // LDS addresses and twiddle loads of every pass from being hoisted out of the loop as
+  // loop-invariant registers.
+  asm volatile("" : "+v"(tid));
+  constexpr int NB = P / R;
+  constexpr int BPT = (NB + T - 1) / T;
+  constexpr int STRIDE = P / (NS * R);
+  float2 v[BPT][R];
+#pragma unroll
+  for (int b = 0; b < BPT; ++b) {
+    const int j = tid + b * T;
+    if ((b + 1) * T <= NB || j < NB) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b][r] = buf[j + r * NB];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < BPT; ++b) {
+    const int j = tid + b * T;
+    if ((b + 1) * T <= NB || j < NB) {
+      const int k = j % NS;
+      if constexpr (NS > 1) {
+        const int e1 = k * STRIDE;
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], tw_lds(stw, r * e1));
+      }
+      Dft<R>::run(v[b]);
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[base + r * NS] = v[b][r];
+    }
+  }
+  __syncthreads();
+}
+
+// First pass, radix 31 (NS = 1, no twiddles): butterfly j = tid by the symmetric codelet, and
+// the 16 left-over butterflies 512..527 as 496 single outputs (one per thread) by direct sums,
+// so that no wave runs a second full radix-31 round.
+__device__ __forceinline__ void lds_pass31(float2* buf, const float2* stw, int tid) {
+  asm volatile("" : "+v"(tid));
+  float2 v[31];
+#pragma unroll
+  for (int r = 0; r < 31; ++r) v[r] = buf[tid + r * NB1];
+  const bool left = tid < LEFT * 31;
+  const int q = T + tid / 31;
+  const int kk = tid - (tid / 31) * 31;
+  float2 w[31];
+  if (left) {
+#pragma unroll
+    for (int r = 0; r < 31; ++r) w[r] = buf[q + r * NB1];
+  }
+  __syncthreads();
+  Dft<31>::run(v);
+#pragma unroll
+  for (int r = 0; r < 31; ++r) buf[tid * 31 + r] = v[r];
+  if (left) {
+    float2 acc = w[0];
+    int e = 0;  // r * kk mod 31, as a multiple of P/31 in the W_P table
+#pragma unroll
+    for (int r = 1; r < 31; ++r) {
+      e += kk;
+      if (e >= 31) e -= 31;
+      const float2 t = cmul(w[r], tw_lds(stw, e * NB1));
+      acc = cadd(acc, t);
+    }
+    buf[q * 31 + kk] = acc;
+  }
+  __syncthreads();
+}
+
+// Passes 31, 11, 3 in LDS; the caller runs the last radix-16 pass.
+__device__ __forceinline__ void passes_to_last(float2* buf, const float2* stw, int tid) {
+  lds_pass31(buf, stw, tid);
+  lds_pass<11, 31>(buf, stw, tid);
+  lds_pass<3, 341>(buf, stw, tid);
+}
+
+struct FwdArgs {
+  const uint8_t* packed;
+  size_t row_bytes;
+  int fmt;
+  const double* means;
+  const double* fc;
+  int n_bin;
+  double pc;
+  const float2* tau;
+  const float2* tw;
+  float2* X;
+};
+
+__global__ __launch_bounds__(T) void k_forward(FwdArgs a) {
+  __shared__ float2 buf[P];
+  __shared__ float2 stw[256];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < 256; i += T) stw[i] = a.tw[i];
+  const int xrow = blockIdx.x;
+  const int h = xrow & 1;
+  const int bin = (xrow >> 1) % a.n_bin;
+  const int ant = (xrow >> 1) / a.n_bin;
+  const uint8_t* row = a.packed + static_cast<size_t>(ant) * a.row_bytes;
+  const double mean = a.means[ant];
+  const double f = a.fc[bin];
+  for (int n = tid; n < P; n += T) {
+    const float2 x0 = wipe_obs(row, a.fmt, n, mean, a.pc, f);
+    const float2 x1 = wipe_obs(row, a.fmt, n + P, mean, a.pc, f);
+    buf[n] = h == 0 ? cadd(x0, x1) : cmul(csub(x0, x1), a.tau[n]);
+  }
+  __syncthreads();
+  passes_to_last(buf, stw, tid);
+  // last pass: radix 16 after 1023, output in natural order
+  float2* out = a.X + static_cast<size_t>(xrow) * P;
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    const int j = tid + b * T;
+    if (j < NB4) {
+      float2 v[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) v[r] = buf[j + r * NB4];
+#pragma unroll
+      for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], tw_lds(stw, r * j));
+      Dft<16>::run(v);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) out[j + r * NB4] = v[r];
+    }
+  }
+}
+
+struct RowArgs {
+  const float2* X;
+  const float2* rep;
+  const float2* tw;
+  int n_bin;
+  int n_prn;
+  int groups;
+  int per_group;
+  int cw;
+  RowOut* rows;
+};
+
+__global__ __launch_bounds__(T) void k_rows(RowArgs a) {
+  __shared__ float2 buf[P];
+  __shared__ float2 stw[256];
+  __shared__ float red_p[NW];
+  __shared__ int red_i[NW];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < 256; i += T) stw[i] = a.tw[i];
+  // Blocks are dealt round-robin to the 8 XCDs: the PRN group follows blockIdx % groups so the
+  // replica spectra a group touches stay in one XCD's L2.
+  const int g = blockIdx.x % a.groups;
+  const int xrow = blockIdx.x / a.groups;
+  const int h = xrow & 1;
+  const int bin = (xrow >> 1) % a.n_bin;
+  const int ant = (xrow >> 1) / a.n_bin;
+  const int p0 = g * a.per_group;
+  const int p1 = min(p0 + a.per_group, a.n_prn);
+  const float2* X = a.X + static_cast<size_t>(xrow) * P;
+  for (int prn = p0; prn < p1; ++prn) {
+    const float2* R = a.rep + (static_cast<size_t>(prn) * 2 + h) * P;
+    int t0 = tid;
+    asm volatile("" : "+v"(t0));
+    // spectrum product conj(X) . R staged through LDS in natural order (coalesced loads)
+    for (int i = t0; i < P; i += T) buf[i] = cconjmul(X[i], R[i]);
+    __syncthreads();
+    passes_to_last(buf, stw, tid);
+    // last pass: radix 16 after 1023; |z|^2 goes back to LDS as a float row
+    int t4 = tid;
+    asm volatile("" : "+v"(t4));
+    float2 v[2][16];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int j = t4 + b * T;
+      if (j < NB4) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[b][r] = buf[j + r * NB4];
+      }
+    }
+    __syncthreads();
+    float* pw = reinterpret_cast<float*>(buf);
+    float bp = -1.0f;
+    int bi = -1;
+    int nan = 0;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int j = t4 + b * T;
+      if (j < NB4) {
+#pragma unroll
+        for (int r = 1; r < 16; ++r) v[b][r] = cmul(v[b][r], tw_lds(stw, r * j));
+        Dft<16>::run(v[b]);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const float p = v[b][r].x * v[b][r].x + v[b][r].y * v[b][r].y;
+          pw[j + r * NB4] = p;
+          nan |= (p != p);
+          arg_combine(bp, bi, p, j + r * NB4);
+        }
+      }
+    }
+    block_argmax<T>(bp, bi, red_p, red_i);  // its barriers also publish pw
+    float m2 = -1.0f;
+    int t5 = tid;
+    asm volatile("" : "+v"(t5));
+    for (int i = t5; i < P; i += T)
+      if (ring_dist(i, bi, P) > a.cw) m2 = fmaxf(m2, pw[i]);
+    m2 = block_max<T>(m2, red_p);
+    nan = __syncthreads_or(nan);
+    if (tid == 0) {
+      RowOut o;
+      o.p1 = bp;
+      o.p2 = m2;
+      o.i1 = bi;
+      o.nan = nan;
+      a.rows[((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) * 2 + h] = o;
+    }
+  }
+}
+
+}  // namespace fused
+
+// ================================ generic path ===========================================
+__global__ void k_fold_obs(const uint8_t* __restrict__ packed, size_t row_bytes, int fmt,
+                           const double* __restrict__ means, const double* __restrict__ fc,
+                           int n_bin, int P, long long total, double pc,
+                           const float2* __restrict__ tau, float2* __restrict__ out) {
+  const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= total) return;
+  const long long xrow = gid / P;
+  const int n = static_cast<int>(gid - xrow * P);
+  const int h = static_cast<int>(xrow & 1);
+  const int bin = static_cast<int>((xrow >> 1) % n_bin);
+  const int ant = static_cast<int>((xrow >> 1) / n_bin);
+  const uint8_t* row = packed + static_cast<size_t>(ant) * row_bytes;
+  const float2 x0 = wipe_obs(row, fmt, n, means[ant], pc, fc[bin]);
+  const float2 x1 = wipe_obs(row, fmt, static_cast<long long>(n) + P, means[ant], pc, fc[bin]);
+  out[gid] = h == 0 ? cadd(x0, x1) : cmul(csub(x0, x1), tau[n]);
+}
+
+__global__ void k_wipe_obs(const uint8_t* __restrict__ packed, size_t row_bytes, int fmt,
+                           const double* __restrict__ means, const double* __restrict__ fc,
+                           int n_bin, int N, long long total, double pc,
+                           float2* __restrict__ out) {
+  const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= total) return;
+  const long long xrow = gid / N;
+  const int n = static_cast<int>(gid - xrow * N);
+  const int bin = static_cast<int>(xrow % n_bin);
+  const int ant = static_cast<int>(xrow / n_bin);
+  const uint8_t* row = packed + static_cast<size_t>(ant) * row_bytes;
+  out[gid] = wipe_obs(row, fmt, n, means[ant], pc, fc[bin]);
+}
+
+__global__ void k_wipe_signal(const float* __restrict__ sig, const double* __restrict__ pp,
+                              const double* __restrict__ fc, int N, long long total,
+                              float2* __restrict__ out) {
+  const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= total) return;
+  const long long bin = gid / N;
+  const int n = static_cast<int>(gid - bin * N);
+  out[gid] = wipe_value(sig[n], pp[n], fc[bin]);
+}
+
+__global__ void k_mul_rows(const float2* __restrict__ X, const float2* __restrict__ rep, int M,
+                           int n_prn, int n_bin, int folded, long long r0, long long total,
+                           float2* __restrict__ Z) {
+  const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= total) return;
+  const long long rr = gid / M;
+  const int i = static_cast<int>(gid - rr * M);
+  const long long r = r0 + rr;
+  const int h = static_cast<int>(r & 1);
+  const long long q = r >> 1;  // (ant*n_prn + prn)*n_bin + bin
+  const int bin = static_cast<int>(q % n_bin);
+  const long long ap = q / n_bin;
+  const int prn = static_cast<int>(ap % n_prn);
+  const int ant = static_cast<int>(ap / n_prn);
+  const long long xrow = folded ? (static_cast<long long>(ant) * n_bin + bin) * 2 + h
+                                : static_cast<long long>(ant) * n_bin + bin;
+  Z[gid] = cconjmul(X[xrow * M + i], rep[(static_cast<long long>(prn) * 2 + h) * M + i]);
+}
+
+constexpr int kRedT = 256;
+
+__global__ __launch_bounds__(kRedT) void k_row_reduce(const float2* __restrict__ Z, int M, int P,
+                                                      int cw, RowOut* __restrict__ out) {
+  __shared__ float sp[kRedT / 64];
+  __shared__ int si[kRedT / 64];
+  const float2* z = Z + static_cast<size_t>(blockIdx.x) * M;
+  float bp = -1.0f;
+  int bi = -1;
+  int nan = 0;
+  for (int i = threadIdx.x; i < M; i += kRedT) {
+    const float2 v = z[i];
+    const float p = v.x * v.x + v.y * v.y;
+    nan |= (p != p);
+    arg_combine(bp, bi, p, i);
+  }
+  block_argmax<kRedT>(bp, bi, sp, si);
+  const int main_cp = bi % P;
+  float m2 = -1.0f;
+  for (int i = threadIdx.x; i < M; i += kRedT) {
+    if (ring_dist(i % P, main_cp, P) > cw) {
+      const float2 v = z[i];
+      m2 = fmaxf(m2, v.x * v.x + v.y * v.y);
+    }
+  }
+  m2 = block_max<kRedT>(m2, sp);
+  nan = __syncthreads_or(nan);
+  if (threadIdx.x == 0) {
+    RowOut o;
+    o.p1 = bp;
+    o.p2 = m2;
+    o.i1 = bi;
+    o.nan = nan;
+    out[blockIdx.x] = o;
+  }
+}
+
+// gain = 2 for folded rows: an unnormalised P-point transform pair carries half the gain of
+// the reference's 2P-point pair (FFT_2P([c,c]) at bin 2k is 2 FFT_P(c) at bin k).  The factor
+// is a power of two, so corr = (sqrt(p) * gain) * scale rounds exactly like sqrt(4p) * scale.
+__global__ void k_select(const RowOut* __restrict__ rows, const PeakSlot* __restrict__ slots,
+                         int n_ant, int n_prn, int n_bin, float gain, float scale,
+                         tga_peak* __restrict__ out, int* nan_flag) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= n_ant * n_prn) return;
+  const int ant = gid / n_prn;
+  const int prn = gid - ant * n_prn;
+  const RowOut* r = rows + static_cast<size_t>(gid) * n_bin * 2;
+  float best = 0.f;
+  int bbin = 0, bh = 0;
+  int nan = 0;
+  for (int bin = 0; bin < n_bin; ++bin) {
+    const RowOut r0 = r[2 * bin], r1 = r[2 * bin + 1];
+    nan |= r0.nan | r1.nan;
+    const float c0 = (__fsqrt_rn(r0.p1) * gain) * scale;
+    const float c1 = (__fsqrt_rn(r1.p1) * gain) * scale;
+    const int hh = c1 > c0 ? 1 : 0;  // [c,-c] only when strictly larger (correlate.rs:140)
+    const float cb = hh ? c1 : c0;
+    if (bin == 0 || cb >= best) {  // last maximum across bins (correlate.rs:151-154)
+      best = cb;
+      bbin = bin;
+      bh = hh;
+    }
+  }
+  const RowOut w = r[2 * bbin + bh];
+  float second = w.p2 >= 0.f ? (__fsqrt_rn(w.p2) * gain) * scale : -1.0f;
+  if (!(second > 0.0f)) second = 1e-6f;  // correlate.rs:172-174
+  const PeakSlot s = slots[prn];
+  tga_peak pk;
+  pk.best_peak = best;
+  pk.second_peak = second;
+  pk.best_freq_idx = static_cast<uint64_t>(bbin);
+  pk.best_codephase = static_cast<uint64_t>(w.i1);
+  out[s.base + static_cast<long long>(ant) * s.stride + s.local] = pk;
+  if (nan) atomicOr(nan_flag, 1);
+}
+
+inline unsigned grid_for(long long total, int block) {
+  return static_cast<unsigned>((total + block - 1) / block);
+}
+
+}  // namespace
+
+// ---- launchers -------------------------------------------------------------------------------
+void launch_ant_sums(const uint8_t* packed, size_t row_bytes, int n_ant, int64_t L, int fmt,
+                     double* means, hipStream_t st) {
+  hipLaunchKernelGGL(k_ant_sums, dim3(n_ant), dim3(256), 0, st, packed, row_bytes,
+                     static_cast<long long>(L), fmt, means);
+  TGA_HIP(hipGetLastError());
+}
+
+bool fused_supported(size_t P, size_t N) { return P == static_cast<size_t>(kFusedP) && N == 2 * P; }
+
+std::vector<float2> fused_twiddles() {
+  std::vector<float2> t(256);
+  for (int i = 0; i < 128; ++i) {
+    for (int part = 0; part < 2; ++part) {
+      const long long e = part == 0 ? i : 128LL * i;
+      const double a = -2.0 * kPiD * static_cast<double>(e % kFusedP) / kFusedP;
+      t[part * 128 + i] = make_float2(static_cast<float>(std::cos(a)),
+                                      static_cast<float>(std::sin(a)));
+    }
+  }
+  return t;
+}
+
+void launch_fused_forward(const uint8_t* packed, size_t row_bytes, int fmt, const double* means,
+                          const double* fc, int n_ant, int n_bin, double phase_const,
+                          const float2* tau, const float2* tw, float2* X, hipStream_t st) {
+  fused::FwdArgs a{packed, row_bytes, fmt, means, fc, n_bin, phase_const, tau, tw, X};
+  hipLaunchKernelGGL(fused::k_forward, dim3(n_ant * n_bin * 2), dim3(fused::T), 0, st, a);
+  TGA_HIP(hipGetLastError());
+}
+
+void launch_fused_rows(const float2* X, const float2* rep, const float2* tw, int n_ant,
+                       int n_bin, int n_prn, int cw, RowOut* rows, hipStream_t st) {
+  const int groups = n_prn < 8 ? n_prn : 8;
+  const int per = (n_prn + groups - 1) / groups;
+  fused::RowArgs a{X, rep, tw, n_bin, n_prn, groups, per, cw, rows};
+  const long long blocks = static_cast<long long>(n_ant) * n_bin * 2 * groups;
+  hipLaunchKernelGGL(fused::k_rows, dim3(static_cast<unsigned>(blocks)), dim3(fused::T), 0, st,
+                     a);
+  TGA_HIP(hipGetLastError());
+}
+
+void launch_fold_obs(const uint8_t* packed, size_t row_bytes, int fmt, const double* means,
+                     const double* fc, int n_ant, int n_bin, int P, double phase_const,
+                     const float2* tau, float2* out, hipStream_t st) {
+  const long long total = static_cast<long long>(n_ant) * n_bin * 2 * P;
+  hipLaunchKernelGGL(k_fold_obs, dim3(grid_for(total, 256)), dim3(256), 0, st, packed,
+                     row_bytes, fmt, means, fc, n_bin, P, total, phase_const, tau, out);
+  TGA_HIP(hipGetLastError());
+}
+
+void launch_wipe_obs(const uint8_t* packed, size_t row_bytes, int fmt, const double* means,
+                     const double* fc, int n_ant, int n_bin, int N, double phase_const,
+                     float2* out, hipStream_t st) {
+  const long long total = static_cast<long long>(n_ant) * n_bin * N;
+  hipLaunchKernelGGL(k_wipe_obs, dim3(grid_for(total, 256)), dim3(256), 0, st, packed,
+                     row_bytes, fmt, means, fc, n_bin, N, total, phase_const, out);
+  TGA_HIP(hipGetLastError());
+}
+
+void launch_wipe_signal(const float* sig, const double* pp, const double* fc, int n_bin, int N,
+                        float2* out, hipStream_t st) {
+  const long long total = static_cast<long long>(n_bin) * N;
+  hipLaunchKernelGGL(k_wipe_signal, dim3(grid_for(total, 256)), dim3(256), 0, st, sig, pp, fc,
+                     N, total, out);
+  TGA_HIP(hipGetLastError());
+}
+
+void launch_mul_rows(const float2* X, const float2* rep, int M, int n_prn, int n_bin,
+                     bool folded, long long r0, long long r1, float2* Z, hipStream_t st) {
+  const long long total = (r1 - r0) * M;
+  hipLaunchKernelGGL(k_mul_rows, dim3(grid_for(total, 256)), dim3(256), 0, st, X, rep, M,
+                     n_prn, n_bin, folded ? 1 : 0, r0, total, Z);
+  TGA_HIP(hipGetLastError());
+}
+
+void launch_row_reduce(const float2* Z, int M, int P, int cw, long long n_rows, RowOut* out,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_row_reduce, dim3(static_cast<unsigned>(n_rows)), dim3(kRedT), 0, st, Z,
+                     M, P, cw, out);
+  TGA_HIP(hipGetLastError());
+}
+
+void launch_select(const RowOut* rows, const PeakSlot* slots, int n_ant, int n_prn, int n_bin,
+                   float gain, float scale, tga_peak* out, int* nan_flag, hipStream_t st) {
+  const int total = n_ant * n_prn;
+  hipLaunchKernelGGL(k_select, dim3(grid_for(total, 128)), dim3(128), 0, st, rows, slots, n_ant,
+                     n_prn, n_bin, gain, scale, out, nan_flag);
+  TGA_HIP(hipGetLastError());
+}
+
+}  // namespace tga

@@ -1,0 +1,311 @@
+"""High-level wrappers over the C ABI (see the package docstring for the reference mapping)."""
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check, lib
+
+
+@dataclass
+class CorrelationPeak:
+    """correlate.rs:27-38."""
+    best_peak: float
+    second_peak: float
+    best_freq_idx: int
+    best_codephase: int
+
+
+@dataclass
+class AcquisitionResult:
+    """acquisition.rs:19-30 (and the per-system twins)."""
+    prn: int
+    signal_strength: float
+    second_peak: float
+    codephase_frac: float
+    frequency: float
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def device_count():
+    n = C.c_int(0)
+    check(lib().tga_device_count(C.byref(n)))
+    return n.value
+
+
+def _sys(system):
+    return L.SYSTEM_IDS[system] if isinstance(system, str) else int(system)
+
+
+def code_length(system):
+    n = C.c_size_t(0)
+    check(lib().tga_code_length(_sys(system), C.byref(n)))
+    return n.value
+
+
+def code_chips(system, prn):
+    n = code_length(system)
+    out = np.empty(n, dtype=np.int8)
+    check(lib().tga_code_chips(_sys(system), int(prn), _ptr(out, C.c_int8), n))
+    return out
+
+
+def code_resampled(system, prn, samples_per_code, num_samples):
+    out = np.empty(int(num_samples), dtype=np.float64)
+    check(lib().tga_code_resampled(_sys(system), int(prn), float(samples_per_code),
+                                   int(num_samples), _ptr(out, C.c_double)))
+    return out
+
+
+def gold_code(samples_per_code, prn, num_samples):
+    """acquisition.rs:162-165."""
+    return code_resampled("gps", prn, samples_per_code, num_samples)
+
+
+def sbas_gold_code(samples_per_code, prn, num_samples):
+    return code_resampled("sbas", prn, samples_per_code, num_samples)
+
+
+def qzss_gold_code(samples_per_code, prn, num_samples):
+    return code_resampled("qzss", prn, samples_per_code, num_samples)
+
+
+def e1c_code_resampled(samples_per_code, prn, num_samples):
+    return code_resampled("galileo", prn, samples_per_code, num_samples)
+
+
+def b1c_code_resampled(samples_per_code, prn, num_samples):
+    return code_resampled("beidou", prn, samples_per_code, num_samples)
+
+
+def l1c_code_resampled(samples_per_code, prn, num_samples):
+    return code_resampled("l1c", prn, samples_per_code, num_samples)
+
+
+def code_spectra(code, period):
+    """correlate.rs:47-58 -> (conj FFT([c,c]), conj FFT([c,-c])) as complex64 arrays."""
+    code = _f64(code)
+    n = len(code)
+    cc = np.empty(n, dtype=np.complex64)
+    ca = np.empty(n, dtype=np.complex64)
+    check(lib().tga_code_spectra(_ptr(code, C.c_double), n, int(period), _ptr(cc, C.c_float),
+                                 _ptr(ca, C.c_float)))
+    return cc, ca
+
+
+def correlate_code(signal_f32, phasepoints, code_complex, code_complex_alt, fc, num_samples,
+                   code_period_samples, sampling_freq):
+    """correlate.rs:91-183 on the GPU."""
+    s = _f32(signal_f32)
+    pp = _f64(phasepoints)
+    cc = np.ascontiguousarray(code_complex, dtype=np.complex64)
+    ca = np.ascontiguousarray(code_complex_alt, dtype=np.complex64)
+    fcs = _f64(fc)
+    n = int(num_samples)
+    if len(s) < n or len(pp) < n or len(cc) < n or len(ca) < n:
+        raise ValueError("signal, phasepoints and spectra need num_samples entries")
+    pk = L.Peak()
+    check(lib().tga_correlate_code(_ptr(s, C.c_float), _ptr(pp, C.c_double),
+                                   _ptr(cc, C.c_float), _ptr(ca, C.c_float),
+                                   _ptr(fcs, C.c_double) if len(fcs) else None, len(fcs), n,
+                                   int(code_period_samples), float(sampling_freq),
+                                   C.byref(pk)))
+    return CorrelationPeak(pk.best_peak, pk.second_peak, pk.best_freq_idx, pk.best_codephase)
+
+
+def acquire_single(system, signal_f32, phasepoints, sampling_freq, center_freq, search_band,
+                   prn, samples_per_period):
+    s = _f32(signal_f32)
+    pp = _f64(phasepoints)
+    if len(pp) < len(s):
+        raise ValueError("phasepoints shorter than the signal")
+    r = L.AcqResult()
+    check(lib().tga_acquire_single(_sys(system), _ptr(s, C.c_float), _ptr(pp, C.c_double),
+                                   len(s), float(sampling_freq), float(center_freq),
+                                   float(search_band), int(prn), int(samples_per_period),
+                                   C.byref(r)))
+    return AcquisitionResult(r.prn, r.signal_strength, r.second_peak, r.codephase_frac,
+                             r.frequency)
+
+
+def acquire_full(signal_f32, phasepoints, sampling_freq, center_freq, search_band, prn,
+                 samples_per_chunk):
+    """acquisition.rs:176-248."""
+    return acquire_single("gps", signal_f32, phasepoints, sampling_freq, center_freq,
+                          search_band, prn, samples_per_chunk)
+
+
+def acquire_sbas(signal_f32, phasepoints, sampling_freq, center_freq, search_band, prn,
+                 samples_per_chunk):
+    return acquire_single("sbas", signal_f32, phasepoints, sampling_freq, center_freq,
+                          search_band, prn, samples_per_chunk)
+
+
+def acquire_qzss(signal_f32, phasepoints, sampling_freq, center_freq, search_band, prn,
+                 samples_per_chunk):
+    return acquire_single("qzss", signal_f32, phasepoints, sampling_freq, center_freq,
+                          search_band, prn, samples_per_chunk)
+
+
+def acquire_galileo_single(signal_f32, phasepoints, sampling_freq, center_freq, search_band,
+                           prn, samples_per_code_period):
+    return acquire_single("galileo", signal_f32, phasepoints, sampling_freq, center_freq,
+                          search_band, prn, samples_per_code_period)
+
+
+def acquire_beidou_single(signal_f32, phasepoints, sampling_freq, center_freq, search_band,
+                          prn, samples_per_code_period):
+    return acquire_single("beidou", signal_f32, phasepoints, sampling_freq, center_freq,
+                          search_band, prn, samples_per_code_period)
+
+
+def acquire_l1c_single(signal_f32, phasepoints, sampling_freq, center_freq, search_band, prn,
+                       samples_per_code_period):
+    return acquire_single("l1c", signal_f32, phasepoints, sampling_freq, center_freq,
+                          search_band, prn, samples_per_code_period)
+
+
+def estimate_cn0(r_a, table):
+    """acr.rs:49-84; ``table`` is a name from ACR_TABLES or its index.  None for no estimate."""
+    t = L.ACR_TABLES[table] if isinstance(table, str) else int(table)
+    c = C.c_double(0)
+    v = C.c_int32(0)
+    check(lib().tga_estimate_cn0(float(r_a), t, C.byref(c), C.byref(v)))
+    return c.value if v.value else None
+
+
+@dataclass
+class SystemSearch:
+    """One system of a batched plan: PRNs (None = full range), absolute bin frequencies,
+    coherent length (0 = two code periods, as the reference)."""
+    system: str
+    fc: list
+    prns: list = None
+    num_samples: int = 0
+
+
+PEAK_DTYPE = np.dtype([("best_peak", np.float32), ("second_peak", np.float32),
+                       ("best_freq_idx", np.uint64), ("best_codephase", np.uint64)])
+
+
+class Plan:
+    """Batched search plan (tga_plan_*): one observation of n_ant packed rows per search."""
+
+    def __init__(self, sampling_freq, n_ant, record_samples, systems, input_format=1):
+        self._keep = []
+        descs = (L.SystemDesc * len(systems))()
+        for i, s in enumerate(systems):
+            fc = _f64(s.fc)
+            self._keep.append(fc)
+            descs[i].system = _sys(s.system)
+            descs[i].n_fc = len(fc)
+            descs[i].fc = _ptr(fc, C.c_double)
+            descs[i].num_samples = int(s.num_samples)
+            if s.prns is None:
+                descs[i].n_prn = 0
+                descs[i].prns = None
+            else:
+                pr = np.ascontiguousarray(s.prns, dtype=np.int32)
+                self._keep.append(pr)
+                descs[i].n_prn = len(pr)
+                descs[i].prns = _ptr(pr, C.c_int32)
+        d = L.PlanDesc()
+        d.sampling_freq = float(sampling_freq)
+        d.n_ant = int(n_ant)
+        d.input_format = int(input_format)
+        d.record_samples = int(record_samples)
+        d.n_systems = len(systems)
+        d.systems = descs
+        h = C.c_void_p()
+        check(lib().tga_plan_create(C.byref(d), C.byref(h)))
+        self._h = h
+        self.systems = systems
+        self.n_ant = int(n_ant)
+        rb = C.c_size_t(0)
+        check(lib().tga_plan_row_bytes(self._h, C.byref(rb)))
+        self.row_bytes = rb.value
+        n = C.c_size_t(0)
+        check(lib().tga_plan_peak_count(self._h, C.byref(n)))
+        self.n_peaks = n.value
+        self.prns = []
+        for i in range(len(systems)):
+            k = C.c_size_t(0)
+            check(lib().tga_plan_prns(self._h, i, None, 0, C.byref(k)))
+            arr = np.empty(k.value, dtype=np.int32)
+            check(lib().tga_plan_prns(self._h, i, _ptr(arr, C.c_int32), k.value, C.byref(k)))
+            self.prns.append(arr)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            check(lib().tga_plan_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _split(self, flat):
+        out, off = [], 0
+        for pr in self.prns:
+            n = self.n_ant * len(pr)
+            out.append(flat[off:off + n].reshape(self.n_ant, len(pr)))
+            off += n
+        return out
+
+    def search(self, packed):
+        """packed: uint8 [n_ant, row_bytes] host array -> list (per system) of structured
+        arrays [n_ant, n_prn] of PEAK_DTYPE."""
+        pk = np.ascontiguousarray(packed, dtype=np.uint8)
+        if pk.ndim != 2:
+            raise ValueError("packed must be [n_ant, row_bytes]")
+        out = np.zeros(self.n_peaks, dtype=PEAK_DTYPE)
+        check(lib().tga_search(self._h, _ptr(pk, C.c_uint8), pk.shape[0], pk.shape[1],
+                               out.ctypes.data_as(C.POINTER(L.Peak))))
+        return self._split(out)
+
+    def search_device(self, dev_ptr, stream=None):
+        """Search an observation already in device memory (an integer device address)."""
+        out = np.zeros(self.n_peaks, dtype=PEAK_DTYPE)
+        check(lib().tga_search_device(self._h, C.c_void_p(dev_ptr),
+                                      C.c_void_p(stream) if stream else None,
+                                      out.ctypes.data_as(C.POINTER(L.Peak))))
+        return self._split(out)
+
+    def set_timing(self, enable=True):
+        check(lib().tga_plan_set_timing(self._h, 1 if enable else 0))
+
+    def timing(self, kernel=0):
+        ms = C.c_double(0)
+        n = C.c_int64(0)
+        b = C.c_double(0)
+        check(lib().tga_plan_get_timing(self._h, int(kernel), C.byref(ms), C.byref(n),
+                                        C.byref(b)))
+        return ms.value, n.value, b.value
+
+    def kernel_name(self, kernel=0):
+        return lib().tga_plan_kernel_name(self._h, int(kernel)).decode()
+
+    def describe(self):
+        buf = C.create_string_buffer(4096)
+        check(lib().tga_plan_describe(self._h, buf, 4096))
+        return buf.value.decode()

@@ -1,0 +1,138 @@
+"""GPU parity of the batched plan (tga_plan_* / tga_search, the level-3 replacement of
+acquire_all_*) against the oracle run per (antenna, PRN) with the reference's semantics:
+de-mean over the whole record, truncate to N = min(2P, record), code_spectra of the N-sample
+replica, correlate_code (acquisition.rs:281-317, correlate.rs:91-183).  Covers the fused
+P = 16368 kernels, the generic folded path, the generic unfolded path (record < 2P), 1- and
+2-bit input, several systems sharing one group, and the plan's error paths."""
+import numpy as np
+import pytest
+
+import tga
+from tga import synth
+from oracle import codes as ocodes
+from parity import assert_peak_matches, near_ties, oracle_search
+
+pytestmark = pytest.mark.gpu
+
+
+def run_case(fs, L, n_ant, fmt, systems, signals, seed):
+    """systems: list of (name, prns, fc list). Returns (plan, peaks, packed)."""
+    packed = synth.synth_observation(n_ant, L, fs, signals, fmt=fmt, seed=seed)
+    plan = tga.Plan(fs, n_ant, L, [tga.SystemSearch(n, fc, prns) for n, prns, fc in systems],
+                    input_format=fmt)
+    peaks = plan.search(packed)
+    return plan, peaks, packed
+
+
+def check_against_oracle(plan, peaks, packed, fs, L, fmt, systems, ants=None, prn_idx=None):
+    levels = synth.unpack_levels(packed, L, fmt)
+    n_checked = 0
+    for si, (name, prns, fc) in enumerate(systems):
+        P = ocodes.samples_per_period(name, fs)
+        N = min(2 * P, L)
+        for a in (ants if ants is not None else range(plan.n_ant)):
+            for pi, prn in enumerate(plan.prns[si]):
+                if prn_idx is not None and pi not in prn_idx:
+                    continue
+                code = ocodes.code_resampled(name, prn, float(P), N)
+                ref, sig, pp, cc, ca = oracle_search(levels[a], fs, P, N, fc, code)
+                ties = near_ties(sig, pp, cc, ca, fc, N, P)
+                assert_peak_matches(peaks[si][a, pi], ref, P, ties,
+                                    what=f"{name} ant {a} prn {prn}")
+                n_checked += 1
+    return n_checked
+
+
+def grid(center, band, step):
+    n = int(round(2 * band / step)) + 1
+    return [center - band + i * step for i in range(n)]
+
+
+FS = 16.368e6
+IF = 4.092e6
+
+
+@pytest.mark.parametrize("fmt", [1, 2])
+def test_fused_ca_plan_matches_oracle(gpu, fmt):
+    L = 163680
+    signals = [dict(system="gps", prn=5, cn0=48.0, doppler=1000.0, delay=1234, phase=0.3),
+               dict(system="gps", prn=12, cn0=45.0, doppler=-2500.0, delay=9000, phase=1.1),
+               dict(system="sbas", prn=131, cn0=47.0, doppler=500.0, delay=15000, phase=2.0)]
+    fc = grid(IF, 3000.0, 500.0)
+    systems = [("gps", [1, 5, 12, 30], fc), ("sbas", [120, 131], fc)]
+    plan, peaks, packed = run_case(FS, L, 2, fmt, systems, signals, seed=11 + fmt)
+    assert "fused16368" in plan.describe()
+    check_against_oracle(plan, peaks, packed, FS, L, fmt, systems)
+    # injected signals are found at the expected code phase and nearest Doppler bin
+    for si, sv in ((0, 1), (0, 2), (1, 1)):
+        name = systems[si][0]
+        s = [x for x in signals if x["system"] == name and x["prn"] == plan.prns[si][sv]][0]
+        for a in range(2):
+            pk = peaks[si][a, sv]
+            assert int(pk["best_codephase"]) == (16368 - s["delay"]) % 16368
+            assert abs(fc[int(pk["best_freq_idx"])] - IF - s["doppler"]) <= 250.0
+
+
+def test_fused_galileo_at_4092khz(gpu):
+    # E1-C at fs = 4.092 MHz has P = 16368 samples per 4 ms period: the fused kernels serve it.
+    fs, L = 4.092e6, 2 * 16368 + 1000
+    signals = [dict(system="galileo", prn=11, cn0=50.0, doppler=-700.0, delay=4321)]
+    fc = grid(IF, 1000.0, 250.0)
+    systems = [("galileo", [3, 11], fc)]
+    plan, peaks, packed = run_case(fs, L, 1, 1, systems, signals, seed=5)
+    assert "fused16368" in plan.describe()
+    check_against_oracle(plan, peaks, packed, fs, L, 1, systems)
+    assert int(peaks[0][0, 1]["best_codephase"]) == (16368 - 4321) % 16368
+
+
+def test_generic_folded_plan_matches_oracle(gpu):
+    fs, L = 2.046e6, 3 * 2046
+    signals = [dict(system="gps", prn=3, cn0=55.0, doppler=600.0, delay=700),
+               dict(system="qzss", prn=193, cn0=55.0, doppler=-300.0, delay=100)]
+    fc = grid(IF, 1500.0, 300.0)
+    systems = [("gps", [3, 4], fc), ("qzss", [193, 200], fc)]
+    plan, peaks, packed = run_case(fs, L, 3, 2, systems, signals, seed=7)
+    assert "generic-folded" in plan.describe()
+    check_against_oracle(plan, peaks, packed, fs, L, 2, systems)
+
+
+def test_generic_unfolded_short_record(gpu):
+    # record shorter than two periods: N = L (acquisition.rs:287), no folding
+    fs, L = 2.046e6, 3000
+    signals = [dict(system="gps", prn=9, cn0=58.0, doppler=0.0, delay=222)]
+    fc = grid(IF, 900.0, 300.0)
+    systems = [("gps", [9, 10], fc)]
+    plan, peaks, packed = run_case(fs, L, 2, 1, systems, signals, seed=3)
+    assert "generic-unfolded" in plan.describe()
+    check_against_oracle(plan, peaks, packed, fs, L, 1, systems)
+
+
+def test_plan_default_prn_ranges_and_layout(gpu):
+    fs, L = 2.046e6, 2 * 2046
+    fc = [IF]
+    plan = tga.Plan(fs, 1, L, [tga.SystemSearch("sbas", fc), tga.SystemSearch("qzss", fc)])
+    assert list(plan.prns[0]) == list(range(120, 159))
+    assert list(plan.prns[1]) == list(range(184, 207))
+    assert plan.n_peaks == 39 + 23
+    packed = synth.synth_observation(1, L, fs, [], fmt=1, seed=1)
+    a = plan.search(packed)
+    b = plan.search(packed)
+    for x, y in zip(a, b):  # deterministic
+        assert x.tobytes() == y.tobytes()
+
+
+def test_plan_errors(gpu):
+    fc = [IF]
+    with pytest.raises(tga.TgaError) as e:
+        tga.Plan(FS, 1, 163680, [tga.SystemSearch("gps", [])])
+    assert e.value.name == "TGA_ERR_EMPTY_BINS"
+    with pytest.raises(tga.TgaError) as e:
+        tga.Plan(FS, 1, 163680, [tga.SystemSearch("gps", fc, [0, 1])])
+    assert e.value.name == "TGA_ERR_PRN_RANGE"
+    with pytest.raises(tga.TgaError) as e:
+        tga.Plan(FS, 1, 163680, [tga.SystemSearch("gps", fc)], input_format=3)
+    assert e.value.name == "TGA_ERR_INVALID_ARGUMENT"
+    plan = tga.Plan(FS, 2, 163680, [tga.SystemSearch("gps", fc, [1])])
+    with pytest.raises(tga.TgaError) as e:
+        plan.search(np.zeros((3, plan.row_bytes), np.uint8))
+    assert e.value.name == "TGA_ERR_ANTENNA_RANGE"
