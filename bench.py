@@ -63,6 +63,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU sample")
     ap.add_argument("--seed", type=int, default=0x5EED_2026 + 2)
     ap.add_argument("--traffic", default=os.path.join(REPO, "profiles", "pmc_traffic.json"))
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="first run one elementwise copy of MIB MiB (reads and writes MIB MiB) "
+                         "so hardware byte counters can be scaled against a known size")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the search "
@@ -96,6 +99,13 @@ def main():
     dev = torch.from_numpy(packed).cuda()
     torch.cuda.synchronize()
     ptr = dev.data_ptr()
+    if args.calib_copy:
+        src = torch.ones(args.calib_copy * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.mul(src, 2.0, out=dst)  # one elementwise kernel: reads and writes MIB MiB
+        torch.cuda.synchronize()
+        del src, dst
 
     def step():
         peaks = plan.search_device(ptr)
@@ -143,13 +153,25 @@ def main():
     if not args.no_cpu_baseline:
         from oracle import cpu_port
         workers = min(16, os.cpu_count() or 1)
-        levels = synth.unpack_levels(packed[:1], L, 2)[0]
-        t_cells, t_s, n_done = cpu_port.time_sample(levels, FS, "gps", list(range(1, 39)), fc,
-                                                    workers, budget_s=args.cpu_budget)
+        levels = synth.unpack_levels(packed, L, 2)
+        t_cells = t_s = 0.0
+        n_done = n_ants = 0
+        prn_sets = [("gps", list(range(1, 39))), ("sbas", list(range(120, 159))),
+                    ("qzss", list(range(184, 207)))]
+        for a in range(n_ant):  # whole antennas, then whole systems, until the budget is spent
+            for sysname, prns in prn_sets:
+                c, t, d = cpu_port.time_sample(levels[a], FS, sysname, prns, fc, workers,
+                                               budget_s=max(args.cpu_budget - t_s, 0.0))
+                t_cells += c
+                t_s += t
+                n_done += d
+            n_ants += 1
+            if t_s >= args.cpu_budget:
+                break
         cpu = {"value": t_cells / t_s, "unit": "cells/s", "cores": workers, "kind": "port",
-               "sample": f"antenna 0, {n_done} GPS PRNs x 41 bins, N=32736, complex64 "
-                         f"pocketfft over {workers} threads, reference work structure "
-                         f"(2 replica + 41x3 FFTs per PRN), {t_s:.1f} s"}
+               "sample": f"{n_done} (antenna, PRN) searches over {n_ants} antenna(s) x 41 bins, "
+                         f"N=32736, complex64 pocketfft over {workers} threads, reference work "
+                         f"structure (2 replica + 41x3 FFTs per PRN), {t_s:.1f} s"}
 
     out = {
         "metric": METRIC, "value": value, "unit": "cells/s", "n_gpus": 1, "steps": args.steps,

@@ -226,21 +226,35 @@ typedef struct tga_rfi_channel {
 
 /* analyze_channel for every antenna of a packed observation (1- or 2-bit), on the GPU:
  * f64 FFT of the whole de-meaned record, |X|^2 below n/2, median floor, local maxima
- * > 15 dB, 10 kHz clustering, at most 3 lines, ACF at lags 1, 3 and 16. */
+ * > 15 dB, 10 kHz clustering, at most 3 lines, ACF at lags 1, 3 and 16.  record_samples
+ * must exceed 16 (the largest ACF lag). */
 int tga_rfi(const uint8_t* packed, size_t n_ant, size_t row_bytes, int64_t record_samples,
             int32_t input_format, double sampling_freq, tga_rfi_channel* out);
 
 /* ---- host orchestration and JSON (acquire_all_* + lib.rs CombinedOutput) ---------------- */
 
-/* Runs the batched search for the systems of `plan` on one packed observation and writes the
- * reference's JSON (serde_json pretty, CombinedOutput shape) into a library-owned string
- * freed with tga_free_string.  center_freq[s] is the IF subtracted from fc to give the
- * reported Doppler (4.092e6 for every system of the reference); antenna_numbers[n_ant] are the
- * labels written to "antenna_numbers"; cn0 != 0 adds cn0_acr. */
+/* Runs the batched search of `plan` on one packed observation and writes the reference's JSON
+ * (serde_json::to_string_pretty of CombinedOutput, main.rs:893, lib.rs:31-45) into a
+ * library-owned string released with tga_free_string.
+ *   center_freq[s]     IF subtracted from the bin frequency to report the Doppler, per plan
+ *                      system (4.092e6 for every system of the reference)
+ *   antenna_numbers    n_ant labels written to "antenna_numbers" (the antenna filter)
+ *   cn0                non-zero adds "cn0_acr" (acquisition.rs:341-357)
+ *   filter_phase_mad, filter_freq_mad
+ *                      MAD filters of the CLI (main.rs:242-255); negative or NaN = off
+ * Per SV: strengths, phases ((best_codephase mod P)/P), freqs, optional cn0_acr, and the
+ * median/MAD fields when n_ant > 1 (acquisition.rs:365-377); results sorted by sv label. */
 int tga_acquire_all_json(tga_plan* plan, const uint8_t* packed, size_t n_ant,
                          size_t row_bytes, const double* center_freq,
-                         const int64_t* antenna_numbers, int32_t cn0, char** json);
+                         const int64_t* antenna_numbers, int32_t cn0, double filter_phase_mad,
+                         double filter_freq_mad, char** json);
+/* The same JSON from peaks a previous tga_search / tga_search_device returned for `plan`. */
+int tga_peaks_json(tga_plan* plan, const tga_peak* peaks, const double* center_freq,
+                   const int64_t* antenna_numbers, int32_t cn0, double filter_phase_mad,
+                   double filter_freq_mad, char** json);
 void tga_free_string(char* s);
+/* The JSON text of one f64 (ryu shortest round-trip digits in serde_json's layout). */
+int tga_format_f64(double x, char* buf, size_t cap);
 
 #ifdef __cplusplus
 }

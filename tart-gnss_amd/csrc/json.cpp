@@ -1,0 +1,281 @@
+// Host orchestration of acquire_all_* and the reference's JSON output.
+//
+// One batched search of the plan replaces the rayon loops of acquire_all_{gps,galileo,beidou,
+// sbas,l1c,qzss} (acquisition.rs:261-396 and siblings); this file maps its peaks to the
+// reference's per-SV records and writes CombinedOutput (lib.rs:31-45) the way
+// serde_json::to_string_pretty does (main.rs:893): two-space indent, keys in struct order,
+// Option::None fields skipped, None inside vectors as null, f64 through shortest round-trip
+// digits laid out like ryu (integral values with ".0", plain decimals for 1e-5 <= |x| < 1e16,
+// otherwise d.ddde<exp>).
+#include <algorithm>
+#include <charconv>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "internal.h"
+#include "plan.h"
+
+namespace tga {
+
+// f64 -> the text serde_json (ryu) prints.
+std::string json_f64(double x) {
+  if (!std::isfinite(x)) return "null";
+  if (x == 0.0) return std::signbit(x) ? "-0.0" : "0.0";
+  char buf[64];
+  auto r = std::to_chars(buf, buf + sizeof buf, std::fabs(x), std::chars_format::scientific);
+  std::string s(buf, r.ptr);
+  const size_t e = s.find('e');
+  std::string digits;
+  for (size_t i = 0; i < e; ++i)
+    if (s[i] != '.') digits += s[i];
+  while (digits.size() > 1 && digits.back() == '0') digits.pop_back();
+  const int exp10 = std::atoi(s.c_str() + e + 1);  // value = d.ddd * 10^exp10
+  const int length = static_cast<int>(digits.size());
+  const int kk = exp10 + 1;  // 10^(kk-1) <= v < 10^kk
+  const int k = kk - length;  // value = digits * 10^k
+  std::string out = std::signbit(x) ? "-" : "";
+  if (0 <= k && kk <= 16) {
+    out += digits + std::string(k, '0') + ".0";
+  } else if (0 < kk && kk <= 16) {
+    out += digits.substr(0, kk) + "." + digits.substr(kk);
+  } else if (-5 < kk && kk <= 0) {
+    out += "0." + std::string(-kk, '0') + digits;
+  } else if (length == 1) {
+    out += digits + "e" + std::to_string(kk - 1);
+  } else {
+    out += digits.substr(0, 1) + "." + digits.substr(1) + "e" + std::to_string(kk - 1);
+  }
+  return out;
+}
+
+namespace {
+
+// stats.rs:10-30
+double median(std::vector<double> v) {
+  std::sort(v.begin(), v.end());
+  const size_t mid = v.size() / 2;
+  return v.size() % 2 == 0 ? (v[mid - 1] + v[mid]) / 2.0 : v[mid];
+}
+
+double mad(const std::vector<double>& v, double med) {
+  std::vector<double> d(v.size());
+  for (size_t i = 0; i < v.size(); ++i) d[i] = std::fabs(v[i] - med);
+  return median(d);
+}
+
+struct SvRecord {
+  std::string sv;
+  std::vector<double> strengths, phases, freqs;
+  bool has_cn0 = false;
+  std::vector<double> cn0;
+  std::vector<bool> cn0_ok;
+  bool has_stats = false;
+  double phase_median = 0, phase_mad = 0, freq_median = 0, freq_mad = 0;
+};
+
+class Writer {
+ public:
+  std::string out;
+  void raw(const std::string& s) { out += s; }
+  void nl(int depth) {
+    out += '\n';
+    out.append(static_cast<size_t>(2 * depth), ' ');
+  }
+  void key(const char* k, int depth, bool first) {
+    if (!first) out += ',';
+    nl(depth);
+    out += '"';
+    out += k;
+    out += "\": ";
+  }
+  void str(const std::string& s) {
+    out += '"';
+    for (char c : s) {
+      if (c == '"' || c == '\\') out += '\\';
+      out += c;
+    }
+    out += '"';
+  }
+  template <class F>
+  void array(size_t n, int depth, F&& elem) {
+    if (n == 0) {
+      out += "[]";
+      return;
+    }
+    out += '[';
+    for (size_t i = 0; i < n; ++i) {
+      if (i) out += ',';
+      nl(depth + 1);
+      elem(i);
+    }
+    nl(depth);
+    out += ']';
+  }
+};
+
+}  // namespace
+
+std::string acquire_all_json(Plan& p, const tga_peak* peaks, const double* center,
+                             const int64_t* antenna_numbers, bool cn0, double filt_phase,
+                             double filt_freq) {
+  const int order[TGA_NUM_SYSTEMS] = {TGA_SYS_GPS, TGA_SYS_GALILEO, TGA_SYS_BEIDOU,
+                                      TGA_SYS_SBAS, TGA_SYS_L1C, TGA_SYS_QZSS};
+  Writer w;
+  w.raw("{");
+  bool first_sys = true;
+  for (int sysid : order) {
+    int si = -1;
+    for (size_t i = 0; i < p.systems.size(); ++i)
+      if (p.systems[i].system == sysid) {
+        if (si >= 0) fail(TGA_ERR_UNSUPPORTED, "a system appears twice in the plan");
+        si = static_cast<int>(i);
+      }
+    if (si < 0) continue;
+    const SysEntry& e = p.systems[si];
+    const Group& g = p.groups[e.group];
+    const SystemInfo& info = system_info(sysid);
+    const double P = static_cast<double>(g.P);
+    std::vector<SvRecord> recs;
+    for (size_t j = 0; j < e.prns.size(); ++j) {
+      SvRecord r;
+      char lab[32];
+      std::snprintf(lab, sizeof lab, info.label_fmt, e.prns[j]);
+      r.sv = lab;
+      std::vector<double> second;
+      for (int a = 0; a < p.n_ant; ++a) {
+        const tga_peak& pk = peaks[e.peak_offset + static_cast<size_t>(a) * e.prns.size() + j];
+        r.strengths.push_back(static_cast<double>(pk.best_peak));
+        second.push_back(static_cast<double>(pk.second_peak));
+        // acquisition.rs:237-239
+        r.phases.push_back(static_cast<double>(pk.best_codephase % g.P) / P);
+        r.freqs.push_back(g.fc[pk.best_freq_idx] - center[si]);
+      }
+      if (cn0) {  // acquisition.rs:341-357
+        for (int a = 0; a < p.n_ant; ++a) {
+          double c = 0;
+          bool ok = false;
+          const double vm = r.strengths[a], vs = second[a];
+          if (vs > 0.0 && vm > vs) ok = estimate_cn0((vm / vs) * (vm / vs), info.acr_table, &c);
+          r.cn0.push_back(c);
+          r.cn0_ok.push_back(ok);
+          r.has_cn0 = r.has_cn0 || ok;
+        }
+      }
+      if (p.n_ant > 1) {  // acquisition.rs:365-377
+        r.has_stats = true;
+        r.phase_median = median(r.phases);
+        r.phase_mad = mad(r.phases, r.phase_median);
+        r.freq_median = median(r.freqs);
+        r.freq_mad = mad(r.freqs, r.freq_median);
+      }
+      // MAD filters (main.rs:242-255): a missing MAD always passes
+      if (r.has_stats && filt_phase >= 0 && !(r.phase_mad <= filt_phase)) continue;
+      if (r.has_stats && filt_freq >= 0 && !(r.freq_mad <= filt_freq)) continue;
+      recs.push_back(std::move(r));
+    }
+    std::stable_sort(recs.begin(), recs.end(),
+                     [](const SvRecord& a, const SvRecord& b) { return a.sv < b.sv; });
+    w.key(info.key, 1, first_sys);
+    first_sys = false;
+    w.raw("{");
+    w.key("antenna_numbers", 2, true);
+    w.array(p.n_ant, 2, [&](size_t i) { w.raw(std::to_string(antenna_numbers[i])); });
+    w.key("results", 2, false);
+    w.array(recs.size(), 2, [&](size_t i) {
+      const SvRecord& r = recs[i];
+      const int d = 4;
+      w.raw("{");
+      w.key("sv", d, true);
+      w.str(r.sv);
+      auto vec = [&](const char* k, const std::vector<double>& v) {
+        w.key(k, d, false);
+        w.array(v.size(), d, [&](size_t q) { w.raw(json_f64(v[q])); });
+      };
+      vec("strengths", r.strengths);
+      vec("phases", r.phases);
+      vec("freqs", r.freqs);
+      if (r.has_cn0) {
+        w.key("cn0_acr", d, false);
+        w.array(r.cn0.size(), d, [&](size_t q) {
+          w.raw(r.cn0_ok[q] ? json_f64(r.cn0[q]) : std::string("null"));
+        });
+      }
+      if (r.has_stats) {
+        w.key("phase_median", d, false);
+        w.raw(json_f64(r.phase_median));
+        w.key("phase_mad", d, false);
+        w.raw(json_f64(r.phase_mad));
+        w.key("freq_median", d, false);
+        w.raw(json_f64(r.freq_median));
+        w.key("freq_mad", d, false);
+        w.raw(json_f64(r.freq_mad));
+      }
+      w.nl(d - 1);
+      w.raw("}");
+    });
+    w.nl(1);
+    w.raw("}");
+  }
+  if (first_sys) {
+    w.raw("}");
+  } else {
+    w.nl(0);
+    w.raw("}");
+  }
+  return w.out;
+}
+
+}  // namespace tga
+
+using namespace tga;
+
+extern "C" {
+
+int tga_format_f64(double x, char* buf, size_t cap) {
+  return plan_guard([&] {
+    if (!buf || cap == 0) fail(TGA_ERR_INVALID_ARGUMENT, "null buffer");
+    const std::string s = json_f64(x);
+    if (s.size() + 1 > cap) fail(TGA_ERR_INVALID_ARGUMENT, "buffer too small");
+    std::memcpy(buf, s.c_str(), s.size() + 1);
+  });
+}
+
+int tga_acquire_all_json(tga_plan* plan, const uint8_t* packed, size_t n_ant, size_t row_bytes,
+                         const double* center_freq, const int64_t* antenna_numbers, int32_t cn0,
+                         double filter_phase_mad, double filter_freq_mad, char** json) {
+  return plan_guard([&] {
+    if (!plan || !packed || !center_freq || !antenna_numbers || !json)
+      fail(TGA_ERR_INVALID_ARGUMENT, "null argument");
+    *json = nullptr;
+    Plan* p = reinterpret_cast<Plan*>(plan);
+    std::vector<tga_peak> peaks(p->n_peaks);
+    if (tga_search(plan, packed, n_ant, row_bytes, peaks.data()) != TGA_OK)
+      fail(TGA_ERR_DEVICE, tga_last_error());
+    const std::string s = acquire_all_json(*p, peaks.data(), center_freq, antenna_numbers,
+                                           cn0 != 0, filter_phase_mad, filter_freq_mad);
+    char* out = new char[s.size() + 1];
+    std::memcpy(out, s.c_str(), s.size() + 1);
+    *json = out;
+  });
+}
+
+int tga_peaks_json(tga_plan* plan, const tga_peak* peaks, const double* center_freq,
+                   const int64_t* antenna_numbers, int32_t cn0, double filter_phase_mad,
+                   double filter_freq_mad, char** json) {
+  return plan_guard([&] {
+    if (!plan || !peaks || !center_freq || !antenna_numbers || !json)
+      fail(TGA_ERR_INVALID_ARGUMENT, "null argument");
+    *json = nullptr;
+    Plan* p = reinterpret_cast<Plan*>(plan);
+    const std::string s = acquire_all_json(*p, peaks, center_freq, antenna_numbers, cn0 != 0,
+                                           filter_phase_mad, filter_freq_mad);
+    char* out = new char[s.size() + 1];
+    std::memcpy(out, s.c_str(), s.size() + 1);
+    *json = out;
+  });
+}
+
+void tga_free_string(char* s) { delete[] s; }
+
+}  // extern "C"

@@ -281,11 +281,22 @@ struct RowArgs {
   RowOut* rows;
 };
 
+// Correlation rows.  Per (antenna, bin, hypothesis) block and per PRN of its group:
+//   pass A  radix 33 (NS = 1), 496 butterflies, operands conj(X) . R loaded straight from
+//           global memory into registers
+//   pass B  radix 16 (NS = 33) through LDS
+//   pass C  radix 31 (NS = 528) from LDS; outputs stay in registers as |z|^2.  The 16
+//           butterflies beyond one per thread are computed as 496 single outputs.
+// Each thread keeps its top two powers: its outputs of pass C lie 528 samples apart, so at
+// most one of them falls inside the +-cw exclusion window around the block's argmax and the
+// masked second maximum needs no second pass over the row.
 __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
   __shared__ float2 buf[P];
   __shared__ float2 stw[256];
   __shared__ float red_p[NW];
   __shared__ int red_i[NW];
+  constexpr int NBA = P / 33;  // 496
+  constexpr int NBC = P / 31;  // 528
   const int tid = threadIdx.x;
   for (int i = tid; i < 256; i += T) stw[i] = a.tw[i];
   // Blocks are dealt round-robin to the 8 XCDs: the PRN group follows blockIdx % groups so the
@@ -298,53 +309,74 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
   const int p0 = g * a.per_group;
   const int p1 = min(p0 + a.per_group, a.n_prn);
   const float2* X = a.X + static_cast<size_t>(xrow) * P;
+  __syncthreads();
   for (int prn = p0; prn < p1; ++prn) {
     const float2* R = a.rep + (static_cast<size_t>(prn) * 2 + h) * P;
-    int t0 = tid;
-    asm volatile("" : "+v"(t0));
-    // spectrum product conj(X) . R staged through LDS in natural order (coalesced loads)
-    for (int i = t0; i < P; i += T) buf[i] = cconjmul(X[i], R[i]);
-    __syncthreads();
-    passes_to_last(buf, stw, tid);
-    // last pass: radix 16 after 1023; |z|^2 goes back to LDS as a float row
-    int t4 = tid;
-    asm volatile("" : "+v"(t4));
-    float2 v[2][16];
+    int ta = tid;
+    asm volatile("" : "+v"(ta));
+    if (ta < NBA) {
+      float2 v[33];
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const int j = t4 + b * T;
-      if (j < NB4) {
+      for (int r = 0; r < 33; ++r) v[r] = cconjmul(X[ta + r * NBA], R[ta + r * NBA]);
+      Dft<33>::run(v);
 #pragma unroll
-        for (int r = 0; r < 16; ++r) v[b][r] = buf[j + r * NB4];
-      }
+      for (int r = 0; r < 33; ++r) buf[ta * 33 + r] = v[r];
     }
     __syncthreads();
-    float* pw = reinterpret_cast<float*>(buf);
-    float bp = -1.0f;
-    int bi = -1;
+    lds_pass<16, 33>(buf, stw, tid);
+    // pass C
+    int tc = tid;
+    asm volatile("" : "+v"(tc));
+    float2 v[31];
+#pragma unroll
+    for (int r = 0; r < 31; ++r) v[r] = buf[tc + r * NBC];
+    const bool left = tc < LEFT * 31;
+    const int q = T + tc / 31;
+    const int kk = tc - (tc / 31) * 31;
+    float pl = -1.0f;
+    int il = -1;
     int nan = 0;
+    if (left) {
+      float2 w[31];
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const int j = t4 + b * T;
-      if (j < NB4) {
+      for (int r = 0; r < 31; ++r) w[r] = buf[q + r * NBC];
+      float2 acc = w[0];
+      int e = 0;  // r * kk mod 31
 #pragma unroll
-        for (int r = 1; r < 16; ++r) v[b][r] = cmul(v[b][r], tw_lds(stw, r * j));
-        Dft<16>::run(v[b]);
+      for (int r = 1; r < 31; ++r) {
+        e += kk;
+        if (e >= 31) e -= 31;
+        const float2 x = cmul(w[r], tw_lds(stw, r * q));       // pass twiddle W_P^{r q}
+        acc = cadd(acc, cmul(x, tw_lds(stw, e * NBC)));         // DFT-31 root W_31^{r kk}
+      }
+      pl = acc.x * acc.x + acc.y * acc.y;
+      il = q + kk * NBC;
+      nan |= (pl != pl);
+    }
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const float p = v[b][r].x * v[b][r].x + v[b][r].y * v[b][r].y;
-          pw[j + r * NB4] = p;
-          nan |= (p != p);
-          arg_combine(bp, bi, p, j + r * NB4);
-        }
+    for (int r = 1; r < 31; ++r) v[r] = cmul(v[r], tw_lds(stw, r * tc));
+    Dft<31>::run(v);
+    float t1 = -1.0f, t2 = -1.0f;
+    int i1 = -1;
+#pragma unroll
+    for (int r = 0; r < 31; ++r) {
+      const float pv = v[r].x * v[r].x + v[r].y * v[r].y;
+      const int idx = tc + r * NBC;
+      nan |= (pv != pv);
+      if (pv > t1 || (pv == t1 && idx > i1)) {
+        t2 = t1;
+        t1 = pv;
+        i1 = idx;
+      } else {
+        t2 = fmaxf(t2, pv);
       }
     }
-    block_argmax<T>(bp, bi, red_p, red_i);  // its barriers also publish pw
-    float m2 = -1.0f;
-    int t5 = tid;
-    asm volatile("" : "+v"(t5));
-    for (int i = t5; i < P; i += T)
-      if (ring_dist(i, bi, P) > a.cw) m2 = fmaxf(m2, pw[i]);
+    float bp = t1;
+    int bi = i1;
+    if (left) arg_combine(bp, bi, pl, il);
+    block_argmax<T>(bp, bi, red_p, red_i);
+    float m2 = ring_dist(i1, bi, P) > a.cw ? t1 : t2;
+    if (left && ring_dist(il, bi, P) > a.cw) m2 = fmaxf(m2, pl);
     m2 = block_max<T>(m2, red_p);
     nan = __syncthreads_or(nan);
     if (tid == 0) {

@@ -97,9 +97,12 @@ PROTOTYPES = {
     "tga_rfi": (C.c_int, [_u8p, C.c_size_t, C.c_size_t, C.c_int64, C.c_int32, C.c_double,
                           C.POINTER(RfiChannel)]),
     "tga_acquire_all_json": (C.c_int, [C.c_void_p, _u8p, C.c_size_t, C.c_size_t, _dp,
-                                       C.POINTER(C.c_int64), C.c_int32,
-                                       C.POINTER(C.c_void_p)]),
+                                       C.POINTER(C.c_int64), C.c_int32, C.c_double,
+                                       C.c_double, C.POINTER(C.c_void_p)]),
+    "tga_peaks_json": (C.c_int, [C.c_void_p, C.POINTER(Peak), _dp, C.POINTER(C.c_int64),
+                                 C.c_int32, C.c_double, C.c_double, C.POINTER(C.c_void_p)]),
     "tga_free_string": (None, [C.c_void_p]),
+    "tga_format_f64": (C.c_int, [C.c_double, C.c_char_p, C.c_size_t]),
 }
 
 _lib = None

@@ -18,7 +18,8 @@ pytestmark = pytest.mark.gpu
 
 def make_case(system, prn, P, N, fs, delay, f0, noise, seed):
     code = ocodes.code_resampled(system, prn, float(P), N)
-    sig = synth.synth_signal(code, P, delay, f0, fs, noise, seed)
+    long = ocodes.code_resampled(system, prn, float(P), max(N, P))
+    sig = synth.synth_signal(long, P, delay, f0, fs, noise, seed)[:N]
     return code, sig
 
 
