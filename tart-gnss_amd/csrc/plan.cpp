@@ -114,7 +114,7 @@ void Plan::build(const tga_plan_desc& d) {
       g->N = N;
       g->fc = fc;
       g->folded = fold;
-      g->fused = fold && fused_supported(P, N);
+      g->fused = fold && fused_supported(P, N, static_cast<int>(std::ceil(fs / 1.023e6)));
       g->M = fold ? P : N;
     }
     e.group = static_cast<int>(g - groups.data());

@@ -30,7 +30,7 @@ void launch_ant_sums(const uint8_t* packed, size_t row_bytes, int n_ant, int64_t
 
 // ---- fused path, code period P = 16368 (16.368 MHz C/A) --------------------------------
 constexpr int kFusedP = 16368;
-bool fused_supported(size_t P, size_t N);
+bool fused_supported(size_t P, size_t N, int cw);
 // X[(ant*n_bin + bin)*2 + h][P]: FFT of the folded sum (h=0) / twisted difference (h=1).
 void launch_fused_forward(const uint8_t* packed, size_t row_bytes, int fmt, const double* means,
                           const double* fc, int n_ant, int n_bin, double phase_const,

@@ -281,24 +281,95 @@ struct RowArgs {
   RowOut* rows;
 };
 
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// (power, index) as one 64-bit key: for p >= 0 the float bits order like the value, and the
+// index in the low word breaks ties towards the larger index (last maximum).
+__device__ __forceinline__ unsigned long long pkey(float p, int idx) {
+  return (static_cast<unsigned long long>(__float_as_uint(p)) << 32) |
+         static_cast<unsigned int>(idx);
+}
+
+// Running top two of a set of powers whose indices are pairwise more than 2 cw apart: `key`
+// holds the best (power, index), `second` the best power other than it.
+struct Top2 {
+  unsigned long long key = 0;  // power 0 at index 0 is below every real key but equal-safe
+  float second = -1.0f;
+  bool any = false;
+  __device__ __forceinline__ void add(float p, int idx) {
+    const unsigned long long k = pkey(p, idx);
+    const float best = any ? __uint_as_float(static_cast<unsigned int>(key >> 32)) : -1.0f;
+    const bool up = !any || k > key;
+    second = up ? best : fmaxf(second, p);
+    key = up ? k : key;
+    any = true;
+  }
+  __device__ __forceinline__ float best() const {
+    return any ? __uint_as_float(static_cast<unsigned int>(key >> 32)) : -1.0f;
+  }
+  __device__ __forceinline__ int index() const {
+    return any ? static_cast<int>(key & 0xffffffffu) : -1;
+  }
+  // max over the set without the element inside the exclusion window around m (at most one)
+  __device__ __forceinline__ float masked(int m, int cw) const {
+    if (!any) return -1.0f;
+    return ring_dist(index(), m, P) > cw ? best() : second;
+  }
+};
+
 // Correlation rows.  Per (antenna, bin, hypothesis) block and per PRN of its group:
 //   pass A  radix 33 (NS = 1), 496 butterflies, operands conj(X) . R loaded straight from
 //           global memory into registers
-//   pass B  radix 16 (NS = 33) through LDS
-//   pass C  radix 31 (NS = 528) from LDS; outputs stay in registers as |z|^2.  The 16
-//           butterflies beyond one per thread are computed as 496 single outputs.
-// Each thread keeps its top two powers: its outputs of pass C lie 528 samples apart, so at
-// most one of them falls inside the +-cw exclusion window around the block's argmax and the
-// masked second maximum needs no second pass over the row.
+//   pass B  radix 16 (NS = 33) through LDS, twiddles W_528^{rk} from a [r][k] LDS table
+//           (lanes read consecutive entries: no bank conflicts)
+//   pass C  radix 31 (NS = 528) on the matrix cores.  With a_j = v_j + v_{31-j},
+//           b_j = v_j - v_{31-j} (j = 1..15, a_0 = v_0), Y_k = A_k - i B_k and
+//           Y_{31-k} = A_k + i B_k where A = C a, B = S b for the 16x16 real matrices
+//           C[k][j] = cos(2 pi jk/31) (C[k][0] = 1) and S[k][j] = sin(2 pi jk/31) (S[k][0] = 0).
+//           A tile of 16 butterflies is four 16x16x16 real GEMMs = 16 v_mfma_f32_16x16x4f32;
+//           528 butterflies = 33 tiles, dealt to the 8 waves (wave 0 takes tile 32 too).
+//           The inter-pass twiddles W_P^{r q} of a lane's inputs are kept in registers.
+// Per lane, the powers of tiles w, w+8, w+16, w+24 lie more than 32 samples apart pairwise
+// (indices q + k 528 with q = 16 t + lane), so each lane keeps the top two of that set and
+// the masked second maximum needs no second pass; tile 32 is tracked as a second set.
 __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
+  constexpr int NBA = P / 33;  // 496
+  constexpr int NBB = P / 16;  // 1023
+  constexpr int NBC = P / 31;  // 528
+  constexpr int TILES = NBC / 16;  // 33
   __shared__ float2 buf[P];
   __shared__ float2 stw[256];
+  __shared__ float2 tb[16 * 33];  // W_528^{r k}, [r][k]
+  __shared__ float2 t1023[1023];  // W_1023^m = W_P^{16 m}
   __shared__ float red_p[NW];
   __shared__ int red_i[NW];
-  constexpr int NBA = P / 33;  // 496
-  constexpr int NBC = P / 31;  // 528
   const int tid = threadIdx.x;
   for (int i = tid; i < 256; i += T) stw[i] = a.tw[i];
+  __syncthreads();
+  for (int i = tid; i < 16 * 33; i += T) tb[i] = tw_lds(stw, (i / 33) * (i % 33) * 31);
+  for (int i = tid; i < 1023; i += T) t1023[i] = tw_lds(stw, 16 * i);
+  const int wave = tid >> 6, lane = tid & 63, ln16 = lane & 15, grp = lane >> 4;
+  // GEMM constants: lane holds C[ln16][4s + grp] and S[ln16][4s + grp] for step s
+  float cA[4], sA[4];
+#pragma unroll
+  for (int st = 0; st < 4; ++st) {
+    const int j = 4 * st + grp;
+    const float2 w = tw_lds(stw, ((j * ln16) % 31) * NBC);  // exp(-2 pi i jk/31)
+    cA[st] = j == 0 ? 1.0f : w.x;
+    sA[st] = j == 0 ? 0.0f : -w.y;
+  }
+  // pass-C input twiddles W_P^{r q}, q = 16 t + ln16, factored as W_P^{r ln16} (per lane, in
+  // registers) times W_1023^{r t} (LDS table t1023): slot 2s -> r = j, slot 2s+1 -> r = 31 - j
+  float2 twl[8];
+  int rr[8];
+#pragma unroll
+  for (int st = 0; st < 4; ++st) {
+    const int j = 4 * st + grp;
+    rr[2 * st] = j;
+    rr[2 * st + 1] = j == 0 ? 0 : 31 - j;
+    twl[2 * st] = tw_lds(stw, j * ln16);
+    twl[2 * st + 1] = j == 0 ? make_float2(0.f, 0.f) : tw_lds(stw, (31 - j) * ln16);
+  }
   // Blocks are dealt round-robin to the 8 XCDs: the PRN group follows blockIdx % groups so the
   // replica spectra a group touches stay in one XCD's L2.
   const int g = blockIdx.x % a.groups;
@@ -312,6 +383,7 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
   __syncthreads();
   for (int prn = p0; prn < p1; ++prn) {
     const float2* R = a.rep + (static_cast<size_t>(prn) * 2 + h) * P;
+    // ---- pass A
     int ta = tid;
     asm volatile("" : "+v"(ta));
     if (ta < NBA) {
@@ -323,62 +395,120 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
       for (int r = 0; r < 33; ++r) buf[ta * 33 + r] = v[r];
     }
     __syncthreads();
-    lds_pass<16, 33>(buf, stw, tid);
-    // pass C
-    int tc = tid;
-    asm volatile("" : "+v"(tc));
-    float2 v[31];
+    // ---- pass B
+    {
+      int tb2 = tid;
+      asm volatile("" : "+v"(tb2));
+      float2 v[2][16];
 #pragma unroll
-    for (int r = 0; r < 31; ++r) v[r] = buf[tc + r * NBC];
-    const bool left = tc < LEFT * 31;
-    const int q = T + tc / 31;
-    const int kk = tc - (tc / 31) * 31;
-    float pl = -1.0f;
-    int il = -1;
-    int nan = 0;
-    if (left) {
-      float2 w[31];
+      for (int b = 0; b < 2; ++b) {
+        const int j = tb2 + b * T;
+        if (b == 0 || j < NBB) {
 #pragma unroll
-      for (int r = 0; r < 31; ++r) w[r] = buf[q + r * NBC];
-      float2 acc = w[0];
-      int e = 0;  // r * kk mod 31
-#pragma unroll
-      for (int r = 1; r < 31; ++r) {
-        e += kk;
-        if (e >= 31) e -= 31;
-        const float2 x = cmul(w[r], tw_lds(stw, r * q));       // pass twiddle W_P^{r q}
-        acc = cadd(acc, cmul(x, tw_lds(stw, e * NBC)));         // DFT-31 root W_31^{r kk}
+          for (int r = 0; r < 16; ++r) v[b][r] = buf[j + r * NBB];
+        }
       }
-      pl = acc.x * acc.x + acc.y * acc.y;
-      il = q + kk * NBC;
-      nan |= (pl != pl);
+      __syncthreads();
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int j = tb2 + b * T;
+        if (b == 0 || j < NBB) {
+          const int q33 = j / 33;
+          const int k = j - q33 * 33;
+#pragma unroll
+          for (int r = 1; r < 16; ++r) v[b][r] = cmul(v[b][r], tb[r * 33 + k]);
+          Dft<16>::run(v[b]);
+          const int base = q33 * NBC + k;
+#pragma unroll
+          for (int r = 0; r < 16; ++r) buf[base + r * 33] = v[b][r];
+        }
+      }
+      __syncthreads();
     }
+    // ---- pass C on the matrix cores: gather every tile's GEMM operands, then issue the
+    // MFMAs of all tiles interleaved (independent accumulator chains), then read out
+    int gc = grp;
+    asm volatile("" : "+v"(gc));
+    const int ntile = wave == 0 ? 5 : 4;  // tiles wave, wave+8, ..., 32
+    Top2 main_set, extra_set;
+    float accn = 0.0f;  // sum of powers: NaN-propagating detector
+    // two batches of tile slots (0-2, 3-4) bound the live GEMM operands and accumulators
 #pragma unroll
-    for (int r = 1; r < 31; ++r) v[r] = cmul(v[r], tw_lds(stw, r * tc));
-    Dft<31>::run(v);
-    float t1 = -1.0f, t2 = -1.0f;
-    int i1 = -1;
+    for (int b0 = 0; b0 < 5; b0 += 3) {
+      constexpr int NB = 3;
+      float fr[NB][4][4];  // [tile][step][a.re, a.im, b.re, b.im]
 #pragma unroll
-    for (int r = 0; r < 31; ++r) {
-      const float pv = v[r].x * v[r].x + v[r].y * v[r].y;
-      const int idx = tc + r * NBC;
-      nan |= (pv != pv);
-      if (pv > t1 || (pv == t1 && idx > i1)) {
-        t2 = t1;
-        t1 = pv;
-        i1 = idx;
-      } else {
-        t2 = fmaxf(t2, pv);
+      for (int u = 0; u < NB; ++u) {
+        const int ts = b0 + u;
+        if (ts < ntile) {
+          const int t = wave + 8 * ts;
+          const int q = 16 * t + ln16;
+#pragma unroll
+          for (int st = 0; st < 4; ++st) {
+            const int r1 = rr[2 * st], r2 = rr[2 * st + 1];
+            int m1 = r1 * t, m2 = r2 * t;
+            m1 -= (m1 / 1023) * 1023;
+            m2 -= (m2 / 1023) * 1023;
+            const float2 w1 = cmul(twl[2 * st], t1023[m1]);
+            const float2 w2 = cmul(twl[2 * st + 1], t1023[m2]);
+            const float2 v1 = cmul(buf[q + r1 * NBC], w1);
+            const float2 v2 = cmul(buf[q + r2 * NBC], w2);
+            fr[u][st][0] = v1.x + v2.x;
+            fr[u][st][1] = v1.y + v2.y;
+            fr[u][st][2] = v1.x - v2.x;
+            fr[u][st][3] = v1.y - v2.y;
+          }
+        }
+      }
+      floatx4 acc[NB][4];
+#pragma unroll
+      for (int u = 0; u < NB; ++u)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[u][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int st = 0; st < 4; ++st) {
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+          if (b0 + u < ntile) {
+            acc[u][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(cA[st], fr[u][st][0], acc[u][0], 0, 0, 0);
+            acc[u][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(cA[st], fr[u][st][1], acc[u][1], 0, 0, 0);
+            acc[u][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(sA[st], fr[u][st][2], acc[u][2], 0, 0, 0);
+            acc[u][3] = __builtin_amdgcn_mfma_f32_16x16x4f32(sA[st], fr[u][st][3], acc[u][3], 0, 0, 0);
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        const int ts = b0 + u;
+        if (ts < ntile) {
+          const int q = 16 * (wave + 8 * ts) + ln16;
+          Top2& set = ts < 4 ? main_set : extra_set;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int kk = 4 * gc + e;  // output row of the GEMM tile
+            const float ar = acc[u][0][e], ai = acc[u][1][e];
+            const float br = acc[u][2][e], bi = acc[u][3][e];
+            const float y1r = ar + bi, y1i = ai - br;  // Y_kk
+            const float p1 = y1r * y1r + y1i * y1i;
+            accn += p1;
+            set.add(p1, q + kk * NBC);
+            if (kk > 0) {
+              const float y2r = ar - bi, y2i = ai + br;  // Y_{31-kk}
+              const float p2 = y2r * y2r + y2i * y2i;
+              accn += p2;
+              set.add(p2, q + (31 - kk) * NBC);
+            }
+          }
+        }
       }
     }
-    float bp = t1;
-    int bi = i1;
-    if (left) arg_combine(bp, bi, pl, il);
+    float bp = main_set.best();
+    int bi = main_set.index();
+    arg_combine(bp, bi, extra_set.best(), extra_set.index());
     block_argmax<T>(bp, bi, red_p, red_i);
-    float m2 = ring_dist(i1, bi, P) > a.cw ? t1 : t2;
-    if (left && ring_dist(il, bi, P) > a.cw) m2 = fmaxf(m2, pl);
-    m2 = block_max<T>(m2, red_p);
-    nan = __syncthreads_or(nan);
+    const float m2 = block_max<T>(fmaxf(main_set.masked(bi, a.cw), extra_set.masked(bi, a.cw)),
+                                  red_p);
+    const int nan = __syncthreads_or(accn != accn);
     if (tid == 0) {
       RowOut o;
       o.p1 = bp;
@@ -544,7 +674,10 @@ void launch_ant_sums(const uint8_t* packed, size_t row_bytes, int n_ant, int64_t
   TGA_HIP(hipGetLastError());
 }
 
-bool fused_supported(size_t P, size_t N) { return P == static_cast<size_t>(kFusedP) && N == 2 * P; }
+bool fused_supported(size_t P, size_t N, int cw) {
+  // the rows kernel's masked second peak needs outputs of one lane more than 2 cw apart
+  return P == static_cast<size_t>(kFusedP) && N == 2 * P && cw >= 0 && cw <= 16;
+}
 
 std::vector<float2> fused_twiddles() {
   std::vector<float2> t(256);
