@@ -180,7 +180,8 @@ def main():
         "config": {"workload": "GPS+SBAS+QZSS L1 C/A, 24 antennas, 2-bit, 16.368 MHz, 10 ms; "
                                "100 PRNs x 41 bins (+-10 kHz, 500 Hz), N=2P=32736, FP32 + ACR",
                    "n_ant": n_ant, "n_prn": n_prn, "n_bins": len(fc), "code_period": P,
-                   "cells": cells, "form": "reference (dual hypothesis, folded P-point rows)"},
+                   "cells": cells, "form": "reference (dual hypothesis, folded P-point rows)",
+                   "plan": plan.describe().splitlines()[0]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": plan.kernel_name(0), "kernel_ms": avg_rows_s * 1000.0,

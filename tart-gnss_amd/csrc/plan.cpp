@@ -7,6 +7,7 @@
 // reference's common grid form one group of 100 PRNs).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 
@@ -185,6 +186,7 @@ void Plan::build(const tga_plan_desc& d) {
     g.n_bin = n_bin;
     if (g.fused) {
       upload(g.d_tw, fused_twiddles(), stream);
+      autotune(g);
     } else {
       g.d_Xs.alloc(g.n_xrow * M * sizeof(float2));
       // rows are processed in chunks of at most ~32 Mi complex values (256 MiB) per buffer
@@ -203,6 +205,48 @@ void Plan::build(const tga_plan_desc& d) {
     TGA_HIP(hipEventCreate(&t.start));
     TGA_HIP(hipEventCreate(&t.stop));
   }
+}
+
+// Rows-kernel autotuner: times every candidate shape once (after one warm-up launch) on the
+// group's real sizes, with zeroed spectra, and keeps the fastest.  TGA_FUSED_VARIANT=0|1 pins
+// the radix-31 implementation (tests use it to cover both).
+void Plan::autotune(Group& g) {
+  const char* pin = std::getenv("TGA_FUSED_VARIANT");
+  std::vector<FusedRowsConfig> cands;
+  for (int v = 0; v < 2; ++v) {
+    if (pin && std::atoi(pin) != v) continue;
+    FusedRowsConfig c;
+    c.variant = v;
+    c.groups = 8;
+    cands.push_back(c);
+  }
+  if (cands.size() == 1) {
+    g.rows_cfg = cands[0];
+    return;
+  }
+  TGA_HIP(hipMemsetAsync(g.d_X.p, 0, g.d_X.bytes, stream));
+  hipEvent_t e0, e1;
+  TGA_HIP(hipEventCreate(&e0));
+  TGA_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  for (size_t i = 0; i < cands.size(); ++i) {
+    launch_fused_rows(g.d_X.as<float2>(), g.d_rep.as<float2>(), g.d_tw.as<float2>(), n_ant,
+                      g.n_bin, g.n_prn, g.cw, g.d_rows.as<RowOut>(), stream, cands[i]);
+    TGA_HIP(hipEventRecord(e0, stream));
+    launch_fused_rows(g.d_X.as<float2>(), g.d_rep.as<float2>(), g.d_tw.as<float2>(), n_ant,
+                      g.n_bin, g.n_prn, g.cw, g.d_rows.as<RowOut>(), stream, cands[i]);
+    TGA_HIP(hipEventRecord(e1, stream));
+    TGA_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    TGA_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (i < 4) g.tune_ms[i] = ms;
+    if (ms < best) {
+      best = ms;
+      g.rows_cfg = cands[i];
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
 }
 
 void Plan::time_begin(int slot, hipStream_t st) {
@@ -263,7 +307,7 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
     const int M = static_cast<int>(g.M);
     if (g.fused) {
       launch_fused_rows(g.X, g.d_rep.as<float2>(), g.d_tw.as<float2>(), n_ant, g.n_bin, g.n_prn,
-                        g.cw, g.d_rows.as<RowOut>(), st);
+                        g.cw, g.d_rows.as<RowOut>(), st, g.rows_cfg);
     } else {
       const FftPlan* fp = fft_plan(g.M);
       for (long long r0 = 0; r0 < g.n_rows; r0 += g.chunk_rows) {
@@ -301,9 +345,12 @@ std::string Plan::describe() const {
   for (const auto& e : systems) {
     const Group& g = groups[e.group];
     char buf[256];
-    std::snprintf(buf, sizeof buf, "%s: P=%zu N=%zu bins=%zu prns=%zu group=%d path=%s\n",
+    std::snprintf(buf, sizeof buf,
+                  "%s: P=%zu N=%zu bins=%zu prns=%zu group=%d path=%s rows_variant=%d "
+                  "groups=%d tune_ms=%.3f/%.3f\n",
                   system_info(e.system).key, g.P, g.N, g.fc.size(), e.prns.size(), e.group,
-                  g.fused ? "fused16368" : (g.folded ? "generic-folded" : "generic-unfolded"));
+                  g.fused ? "fused16368" : (g.folded ? "generic-folded" : "generic-unfolded"),
+                  g.rows_cfg.variant, g.rows_cfg.groups, g.tune_ms[0], g.tune_ms[1]);
     s += buf;
   }
   return s;

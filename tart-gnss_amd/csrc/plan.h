@@ -25,6 +25,8 @@ struct Group {
   int cw = 1;
   DevBuf d_fc, d_slots, d_tau, d_rep, d_X, d_Xs, d_rows, d_Z, d_Zs, d_tw;
   float2* X = nullptr;  // where the forward spectra of the last search landed
+  FusedRowsConfig rows_cfg;  // rows-kernel shape picked by the autotuner
+  double tune_ms[4] = {0, 0, 0, 0};
   double row_bytes_alg = 0, fwd_bytes_alg = 0;
 };
 
@@ -64,6 +66,7 @@ struct Plan {
   void time_begin(int slot, hipStream_t st);
   void time_end(int slot, hipStream_t st, double bytes);
   void time_collect();
+  void autotune(Group& g);
 };
 
 // Error-to-status wrapper shared by the C ABI translation units.

@@ -35,9 +35,17 @@ bool fused_supported(size_t P, size_t N, int cw);
 void launch_fused_forward(const uint8_t* packed, size_t row_bytes, int fmt, const double* means,
                           const double* fc, int n_ant, int n_bin, double phase_const,
                           const float2* tau, const float2* tw, float2* X, hipStream_t st);
+// Tunable shape of the rows kernel: `variant` 0 runs the radix-31 pass on the VALU, 1 on the
+// matrix cores; `groups` PRN groups per (antenna, bin, hypothesis) (a multiple of the 8 XCDs
+// keeps each group's replica spectra within one XCD's L2).
+struct FusedRowsConfig {
+  int variant = 0;
+  int groups = 8;
+};
 // rows[((ant*n_prn + prn)*n_bin + bin)*2 + h]
 void launch_fused_rows(const float2* X, const float2* rep, const float2* tw, int n_ant,
-                       int n_bin, int n_prn, int cw, RowOut* rows, hipStream_t st);
+                       int n_bin, int n_prn, int cw, RowOut* rows, hipStream_t st,
+                       const FusedRowsConfig& cfg);
 // Twiddle table of the fused kernels: W_P^i (i < 128) then W_P^{128 i} (i < 128).
 std::vector<float2> fused_twiddles();
 

@@ -60,6 +60,21 @@ __device__ __forceinline__ void arg_combine(float& p, int& i, float q, int j) {
   }
 }
 
+// Workgroup barrier that orders LDS traffic only (__syncthreads() also waits for outstanding
+// global loads); the block reductions and the fused kernels exchange data through LDS alone.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// Raw buffer view of `bytes` bytes at p: loads take a 32-bit lane offset plus a scalar offset,
+// so a strided gather needs no 64-bit address arithmetic, and reads past the end return 0.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_view(const void* p, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, bytes, 0x00020000);
+}
+__device__ __forceinline__ float2 buffer_load_f2(__amdgpu_buffer_rsrc_t rs, int voff, int soff) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0));
+}
+
 template <int NT>
 __device__ __forceinline__ void block_argmax(float& p, int& i, float* sp, int* si) {
   constexpr int NW = (NT + 63) / 64;
@@ -74,12 +89,12 @@ __device__ __forceinline__ void block_argmax(float& p, int& i, float* sp, int* s
     sp[wave] = p;
     si[wave] = i;
   }
-  __syncthreads();
+  lds_barrier();
   p = sp[0];
   i = si[0];
 #pragma unroll
   for (int w = 1; w < NW; ++w) arg_combine(p, i, sp[w], si[w]);
-  __syncthreads();
+  lds_barrier();
 }
 
 template <int NT>
@@ -89,11 +104,11 @@ __device__ __forceinline__ float block_max(float v, float* sp) {
   for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0) sp[wave] = v;
-  __syncthreads();
+  lds_barrier();
   v = sp[0];
 #pragma unroll
   for (int w = 1; w < NW; ++w) v = fmaxf(v, sp[w]);
-  __syncthreads();
+  lds_barrier();
   return v;
 }
 
@@ -157,7 +172,7 @@ __device__ __forceinline__ void lds_pass(float2* buf, const float2* stw, int tid
       for (int r = 0; r < R; ++r) v[b][r] = buf[j + r * NB];
     }
   }
-  __syncthreads();
+  lds_barrier();
 #pragma unroll
   for (int b = 0; b < BPT; ++b) {
     const int j = tid + b * T;
@@ -174,7 +189,7 @@ __device__ __forceinline__ void lds_pass(float2* buf, const float2* stw, int tid
       for (int r = 0; r < R; ++r) buf[base + r * NS] = v[b][r];
     }
   }
-  __syncthreads();
+  lds_barrier();
 }
 
 // First pass, radix 31 (NS = 1, no twiddles): butterfly j = tid by the symmetric codelet, and
@@ -318,20 +333,23 @@ struct Top2 {
 };
 
 // Correlation rows.  Per (antenna, bin, hypothesis) block and per PRN of its group:
-//   pass A  radix 33 (NS = 1), 496 butterflies, operands conj(X) . R loaded straight from
-//           global memory into registers
+//   pass A  radix 33 (NS = 1), 496 butterflies, operands conj(X) . R gathered straight from
+//           global memory (buffer loads) into registers
 //   pass B  radix 16 (NS = 33) through LDS, twiddles W_528^{rk} from a [r][k] LDS table
 //           (lanes read consecutive entries: no bank conflicts)
-//   pass C  radix 31 (NS = 528) on the matrix cores.  With a_j = v_j + v_{31-j},
-//           b_j = v_j - v_{31-j} (j = 1..15, a_0 = v_0), Y_k = A_k - i B_k and
-//           Y_{31-k} = A_k + i B_k where A = C a, B = S b for the 16x16 real matrices
-//           C[k][j] = cos(2 pi jk/31) (C[k][0] = 1) and S[k][j] = sin(2 pi jk/31) (S[k][0] = 0).
-//           A tile of 16 butterflies is four 16x16x16 real GEMMs = 16 v_mfma_f32_16x16x4f32;
-//           528 butterflies = 33 tiles, dealt to the 8 waves (wave 0 takes tile 32 too).
-//           The inter-pass twiddles W_P^{r q} of a lane's inputs are kept in registers.
-// Per lane, the powers of tiles w, w+8, w+16, w+24 lie more than 32 samples apart pairwise
-// (indices q + k 528 with q = 16 t + lane), so each lane keeps the top two of that set and
-// the masked second maximum needs no second pass; tile 32 is tracked as a second set.
+//   pass C  radix 31 (NS = 528); outputs stay in registers as |z|^2.  Two implementations:
+//     VARIANT 0 (VALU): one butterfly per thread by the symmetric codelet with its twiddles
+//           W_P^{r j} in registers; the 16 butterflies beyond one per thread are split into
+//           256 (butterfly, output pair) units, one per thread of waves 0-3.
+//     VARIANT 1 (matrix cores): with a_j = v_j + v_{31-j}, b_j = v_j - v_{31-j} (a_0 = v_0),
+//           Y_k = A_k - i B_k and Y_{31-k} = A_k + i B_k where A = C a, B = S b for the 16x16
+//           real matrices C[k][j] = cos(2 pi jk/31) (C[k][0] = 1), S[k][j] = sin(2 pi jk/31)
+//           (S[k][0] = 0): a tile of 16 butterflies is 16 v_mfma_f32_16x16x4f32, and the 33
+//           tiles are dealt to the 8 waves (wave 0 takes tile 32 too).
+// Each thread keeps the top two powers of sets whose indices lie more than 32 samples apart
+// pairwise, so at most one member of a set falls inside the +-cw exclusion window around the
+// block's argmax and the masked second maximum needs no second pass over the row.
+template <int VARIANT>
 __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
   constexpr int NBA = P / 33;  // 496
   constexpr int NBB = P / 16;  // 1023
@@ -340,35 +358,45 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
   __shared__ float2 buf[P];
   __shared__ float2 stw[256];
   __shared__ float2 tb[16 * 33];  // W_528^{r k}, [r][k]
-  __shared__ float2 t1023[1023];  // W_1023^m = W_P^{16 m}
+  __shared__ float2 tx[VARIANT == 0 ? 31 * LEFT + 31 : 1023];
   __shared__ float red_p[NW];
   __shared__ int red_i[NW];
   const int tid = threadIdx.x;
   for (int i = tid; i < 256; i += T) stw[i] = a.tw[i];
   __syncthreads();
   for (int i = tid; i < 16 * 33; i += T) tb[i] = tw_lds(stw, (i / 33) * (i % 33) * 31);
-  for (int i = tid; i < 1023; i += T) t1023[i] = tw_lds(stw, 16 * i);
   const int wave = tid >> 6, lane = tid & 63, ln16 = lane & 15, grp = lane >> 4;
-  // GEMM constants: lane holds C[ln16][4s + grp] and S[ln16][4s + grp] for step s
+  // variant 0: tl = tx[0 .. 31*16): W_P^{r (512 + b)} as [r][b]; r31 = tx[496 ..): (cos, sin)
+  //            (2 pi m / 31); wc: W_P^{r tid} in registers
+  // variant 1: t1023 = tx: W_1023^m; GEMM constants cA/sA; twl: W_P^{r ln16}, rr: the lane's r
+  float2* tl = tx;
+  float2* r31 = tx + 31 * LEFT;
+  float2* t1023 = tx;
+  float2 wc[31];
   float cA[4], sA[4];
-#pragma unroll
-  for (int st = 0; st < 4; ++st) {
-    const int j = 4 * st + grp;
-    const float2 w = tw_lds(stw, ((j * ln16) % 31) * NBC);  // exp(-2 pi i jk/31)
-    cA[st] = j == 0 ? 1.0f : w.x;
-    sA[st] = j == 0 ? 0.0f : -w.y;
-  }
-  // pass-C input twiddles W_P^{r q}, q = 16 t + ln16, factored as W_P^{r ln16} (per lane, in
-  // registers) times W_1023^{r t} (LDS table t1023): slot 2s -> r = j, slot 2s+1 -> r = 31 - j
   float2 twl[8];
   int rr[8];
+  if constexpr (VARIANT == 0) {
+    for (int i = tid; i < 31 * LEFT; i += T) tl[i] = tw_lds(stw, (i / LEFT) * (T + i % LEFT));
+    if (tid < 31) {
+      const float2 w = tw_lds(stw, tid * NBC);  // exp(-2 pi i m / 31)
+      r31[tid] = make_float2(w.x, -w.y);
+    }
 #pragma unroll
-  for (int st = 0; st < 4; ++st) {
-    const int j = 4 * st + grp;
-    rr[2 * st] = j;
-    rr[2 * st + 1] = j == 0 ? 0 : 31 - j;
-    twl[2 * st] = tw_lds(stw, j * ln16);
-    twl[2 * st + 1] = j == 0 ? make_float2(0.f, 0.f) : tw_lds(stw, (31 - j) * ln16);
+    for (int r = 1; r < 31; ++r) wc[r] = tw_lds(stw, r * tid);
+  } else {
+    for (int i = tid; i < 1023; i += T) t1023[i] = tw_lds(stw, 16 * i);
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      const int j = 4 * st + grp;
+      const float2 w = tw_lds(stw, ((j * ln16) % 31) * NBC);  // exp(-2 pi i jk/31)
+      cA[st] = j == 0 ? 1.0f : w.x;
+      sA[st] = j == 0 ? 0.0f : -w.y;
+      rr[2 * st] = j;
+      rr[2 * st + 1] = j == 0 ? 0 : 31 - j;
+      twl[2 * st] = tw_lds(stw, j * ln16);
+      twl[2 * st + 1] = j == 0 ? make_float2(0.f, 0.f) : tw_lds(stw, (31 - j) * ln16);
+    }
   }
   // Blocks are dealt round-robin to the 8 XCDs: the PRN group follows blockIdx % groups so the
   // replica spectra a group touches stay in one XCD's L2.
@@ -379,22 +407,25 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
   const int ant = (xrow >> 1) / a.n_bin;
   const int p0 = g * a.per_group;
   const int p1 = min(p0 + a.per_group, a.n_prn);
-  const float2* X = a.X + static_cast<size_t>(xrow) * P;
+  const __amdgpu_buffer_rsrc_t xv = buffer_view(a.X + static_cast<size_t>(xrow) * P, P * 8);
   __syncthreads();
   for (int prn = p0; prn < p1; ++prn) {
-    const float2* R = a.rep + (static_cast<size_t>(prn) * 2 + h) * P;
+    const __amdgpu_buffer_rsrc_t rv =
+        buffer_view(a.rep + (static_cast<size_t>(prn) * 2 + h) * P, P * 8);
     // ---- pass A
     int ta = tid;
     asm volatile("" : "+v"(ta));
     if (ta < NBA) {
       float2 v[33];
 #pragma unroll
-      for (int r = 0; r < 33; ++r) v[r] = cconjmul(X[ta + r * NBA], R[ta + r * NBA]);
+      for (int r = 0; r < 33; ++r)
+        v[r] = cconjmul(buffer_load_f2(xv, ta * 8, r * NBA * 8),
+                        buffer_load_f2(rv, ta * 8, r * NBA * 8));
       Dft<33>::run(v);
 #pragma unroll
       for (int r = 0; r < 33; ++r) buf[ta * 33 + r] = v[r];
     }
-    __syncthreads();
+    lds_barrier();
     // ---- pass B
     {
       int tb2 = tid;
@@ -408,7 +439,7 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
           for (int r = 0; r < 16; ++r) v[b][r] = buf[j + r * NBB];
         }
       }
-      __syncthreads();
+      lds_barrier();
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
         const int j = tb2 + b * T;
@@ -423,80 +454,138 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
           for (int r = 0; r < 16; ++r) buf[base + r * 33] = v[b][r];
         }
       }
-      __syncthreads();
+      lds_barrier();
     }
-    // ---- pass C on the matrix cores: gather every tile's GEMM operands, then issue the
-    // MFMAs of all tiles interleaved (independent accumulator chains), then read out
-    int gc = grp;
-    asm volatile("" : "+v"(gc));
-    const int ntile = wave == 0 ? 5 : 4;  // tiles wave, wave+8, ..., 32
+    // ---- pass C
     Top2 main_set, extra_set;
     float accn = 0.0f;  // sum of powers: NaN-propagating detector
-    // two batches of tile slots (0-2, 3-4) bound the live GEMM operands and accumulators
+    if constexpr (VARIANT == 0) {
+      int tc = tid;
+      asm volatile("" : "+v"(tc));
+      float2 v[31];
 #pragma unroll
-    for (int b0 = 0; b0 < 5; b0 += 3) {
-      constexpr int NB = 3;
-      float fr[NB][4][4];  // [tile][step][a.re, a.im, b.re, b.im]
+      for (int r = 0; r < 31; ++r) v[r] = buf[tc + r * NBC];
+      if (tc < 16 * LEFT) {  // left-over unit: butterfly T + b, output pair kp
+        const int b = tc & (LEFT - 1);
+        const int kp = tc >> 4;
+        const int q = T + b;
+        float2 w[31];
 #pragma unroll
-      for (int u = 0; u < NB; ++u) {
-        const int ts = b0 + u;
-        if (ts < ntile) {
-          const int t = wave + 8 * ts;
-          const int q = 16 * t + ln16;
+        for (int r = 0; r < 31; ++r) w[r] = buf[q + r * NBC];
 #pragma unroll
-          for (int st = 0; st < 4; ++st) {
-            const int r1 = rr[2 * st], r2 = rr[2 * st + 1];
-            int m1 = r1 * t, m2 = r2 * t;
-            m1 -= (m1 / 1023) * 1023;
-            m2 -= (m2 / 1023) * 1023;
-            const float2 w1 = cmul(twl[2 * st], t1023[m1]);
-            const float2 w2 = cmul(twl[2 * st + 1], t1023[m2]);
-            const float2 v1 = cmul(buf[q + r1 * NBC], w1);
-            const float2 v2 = cmul(buf[q + r2 * NBC], w2);
-            fr[u][st][0] = v1.x + v2.x;
-            fr[u][st][1] = v1.y + v2.y;
-            fr[u][st][2] = v1.x - v2.x;
-            fr[u][st][3] = v1.y - v2.y;
-          }
+        for (int r = 1; r < 31; ++r) w[r] = cmul(w[r], tl[r * LEFT + b]);
+        float ar = w[0].x, ai = w[0].y, br = 0.f, bi = 0.f;
+        int m = 0;
+#pragma unroll
+        for (int jj = 1; jj <= 15; ++jj) {
+          const float2 s = cadd(w[jj], w[31 - jj]);
+          const float2 d = csub(w[jj], w[31 - jj]);
+          m += kp;
+          if (m >= 31) m -= 31;
+          const float2 cs = r31[m];
+          ar = fmaf(s.x, cs.x, ar);
+          ai = fmaf(s.y, cs.x, ai);
+          br = fmaf(d.x, cs.y, br);
+          bi = fmaf(d.y, cs.y, bi);
+        }
+        // kp = 0: every root is 1 and the sine sums vanish, so Y_0 = ar + i ai
+        const float y1r = ar + bi, y1i = ai - br;  // Y_kp
+        const float p1 = y1r * y1r + y1i * y1i;
+        accn += p1;
+        extra_set.add(p1, q + kp * NBC);
+        if (kp > 0) {
+          const float y2r = ar - bi, y2i = ai + br;  // Y_{31-kp}
+          const float p2 = y2r * y2r + y2i * y2i;
+          accn += p2;
+          // the two outputs of a unit lie (31 - 2 kp) 528 > 32 samples apart
+          extra_set.add(p2, q + (31 - kp) * NBC);
         }
       }
-      floatx4 acc[NB][4];
 #pragma unroll
-      for (int u = 0; u < NB; ++u)
+      for (int r = 1; r < 31; ++r) v[r] = cmul(v[r], wc[r]);
+      Dft<31>::run(v);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[u][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+      for (int r = 0; r < 31; ++r) {
+        const float pv = v[r].x * v[r].x + v[r].y * v[r].y;
+        accn += pv;
+        main_set.add(pv, tc + r * NBC);  // outputs of one butterfly lie 528 apart
+      }
+    } else {
+      int gc = grp;
+      asm volatile("" : "+v"(gc));
+      const int ntile = wave == 0 ? 5 : 4;  // tiles wave, wave+8, ..., 32
+      // two batches of tile slots (0-2, 3-4) bound the live GEMM operands and accumulators
 #pragma unroll
-      for (int st = 0; st < 4; ++st) {
+      for (int b0 = 0; b0 < 5; b0 += 3) {
+        constexpr int NB = 3;
+        float fr[NB][4][4];  // [tile][step][a.re, a.im, b.re, b.im]
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
-          if (b0 + u < ntile) {
-            acc[u][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(cA[st], fr[u][st][0], acc[u][0], 0, 0, 0);
-            acc[u][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(cA[st], fr[u][st][1], acc[u][1], 0, 0, 0);
-            acc[u][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(sA[st], fr[u][st][2], acc[u][2], 0, 0, 0);
-            acc[u][3] = __builtin_amdgcn_mfma_f32_16x16x4f32(sA[st], fr[u][st][3], acc[u][3], 0, 0, 0);
+          const int ts = b0 + u;
+          if (ts < ntile) {
+            const int t = wave + 8 * ts;
+            const int q = 16 * t + ln16;
+            // m1 = (r1 t) mod 1023 stepping r1 = gc, gc+4, ...; m2 = ((31 - r1) t) mod 1023
+            const int d4 = (4 * t) % 1023, c31 = (31 * t) % 1023;
+            int m1 = (gc * t) % 1023;
+#pragma unroll
+            for (int st = 0; st < 4; ++st) {
+              const int r1 = rr[2 * st], r2 = rr[2 * st + 1];
+              if (st > 0) {
+                m1 += d4;
+                if (m1 >= 1023) m1 -= 1023;
+              }
+              int m2 = c31 - m1;
+              if (m2 < 0) m2 += 1023;
+              const float2 w1 = cmul(twl[2 * st], t1023[m1]);
+              const float2 w2 = cmul(twl[2 * st + 1], t1023[m2]);
+              const float2 v1 = cmul(buf[q + r1 * NBC], w1);
+              const float2 v2 = cmul(buf[q + r2 * NBC], w2);
+              fr[u][st][0] = v1.x + v2.x;
+              fr[u][st][1] = v1.y + v2.y;
+              fr[u][st][2] = v1.x - v2.x;
+              fr[u][st][3] = v1.y - v2.y;
+            }
           }
         }
-      }
+        floatx4 acc[NB][4];
 #pragma unroll
-      for (int u = 0; u < NB; ++u) {
-        const int ts = b0 + u;
-        if (ts < ntile) {
-          const int q = 16 * (wave + 8 * ts) + ln16;
-          Top2& set = ts < 4 ? main_set : extra_set;
+        for (int u = 0; u < NB; ++u)
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int kk = 4 * gc + e;  // output row of the GEMM tile
-            const float ar = acc[u][0][e], ai = acc[u][1][e];
-            const float br = acc[u][2][e], bi = acc[u][3][e];
-            const float y1r = ar + bi, y1i = ai - br;  // Y_kk
-            const float p1 = y1r * y1r + y1i * y1i;
-            accn += p1;
-            set.add(p1, q + kk * NBC);
-            if (kk > 0) {
-              const float y2r = ar - bi, y2i = ai + br;  // Y_{31-kk}
-              const float p2 = y2r * y2r + y2i * y2i;
-              accn += p2;
-              set.add(p2, q + (31 - kk) * NBC);
+          for (int c = 0; c < 4; ++c) acc[u][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+#pragma unroll
+          for (int u = 0; u < NB; ++u) {
+            if (b0 + u < ntile) {
+              acc[u][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(cA[st], fr[u][st][0], acc[u][0], 0, 0, 0);
+              acc[u][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(cA[st], fr[u][st][1], acc[u][1], 0, 0, 0);
+              acc[u][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(sA[st], fr[u][st][2], acc[u][2], 0, 0, 0);
+              acc[u][3] = __builtin_amdgcn_mfma_f32_16x16x4f32(sA[st], fr[u][st][3], acc[u][3], 0, 0, 0);
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+          const int ts = b0 + u;
+          if (ts < ntile) {
+            const int q = 16 * (wave + 8 * ts) + ln16;
+            Top2& set = ts < 4 ? main_set : extra_set;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const int kk = 4 * gc + e;  // output row of the GEMM tile
+              const float ar = acc[u][0][e], ai = acc[u][1][e];
+              const float br = acc[u][2][e], bi = acc[u][3][e];
+              const float y1r = ar + bi, y1i = ai - br;  // Y_kk
+              const float p1 = y1r * y1r + y1i * y1i;
+              accn += p1;
+              set.add(p1, q + kk * NBC);
+              if (kk > 0) {
+                const float y2r = ar - bi, y2i = ai + br;  // Y_{31-kk}
+                const float p2 = y2r * y2r + y2i * y2i;
+                accn += p2;
+                set.add(p2, q + (31 - kk) * NBC);
+              }
             }
           }
         }
@@ -701,13 +790,18 @@ void launch_fused_forward(const uint8_t* packed, size_t row_bytes, int fmt, cons
 }
 
 void launch_fused_rows(const float2* X, const float2* rep, const float2* tw, int n_ant,
-                       int n_bin, int n_prn, int cw, RowOut* rows, hipStream_t st) {
-  const int groups = n_prn < 8 ? n_prn : 8;
+                       int n_bin, int n_prn, int cw, RowOut* rows, hipStream_t st,
+                       const FusedRowsConfig& cfg) {
+  const int groups = n_prn < cfg.groups ? n_prn : cfg.groups;
   const int per = (n_prn + groups - 1) / groups;
   fused::RowArgs a{X, rep, tw, n_bin, n_prn, groups, per, cw, rows};
   const long long blocks = static_cast<long long>(n_ant) * n_bin * 2 * groups;
-  hipLaunchKernelGGL(fused::k_rows, dim3(static_cast<unsigned>(blocks)), dim3(fused::T), 0, st,
-                     a);
+  if (cfg.variant == 0)
+    hipLaunchKernelGGL(fused::k_rows<0>, dim3(static_cast<unsigned>(blocks)), dim3(fused::T), 0,
+                       st, a);
+  else
+    hipLaunchKernelGGL(fused::k_rows<1>, dim3(static_cast<unsigned>(blocks)), dim3(fused::T), 0,
+                       st, a);
   TGA_HIP(hipGetLastError());
 }
 

@@ -52,8 +52,15 @@ FS = 16.368e6
 IF = 4.092e6
 
 
+@pytest.fixture(params=[0, 1], ids=["valu31", "mfma31"])
+def fused_variant(request, monkeypatch):
+    """Both radix-31 implementations of the fused rows kernel (the autotuner picks one)."""
+    monkeypatch.setenv("TGA_FUSED_VARIANT", str(request.param))
+    return request.param
+
+
 @pytest.mark.parametrize("fmt", [1, 2])
-def test_fused_ca_plan_matches_oracle(gpu, fmt):
+def test_fused_ca_plan_matches_oracle(gpu, fmt, fused_variant):
     L = 163680
     signals = [dict(system="gps", prn=5, cn0=52.0, doppler=1000.0, delay=1234, phase=0.3),
                dict(system="gps", prn=12, cn0=45.0, doppler=-2500.0, delay=9000, phase=1.1),
@@ -62,6 +69,7 @@ def test_fused_ca_plan_matches_oracle(gpu, fmt):
     systems = [("gps", [1, 5, 12, 30], fc), ("sbas", [120, 131], fc)]
     plan, peaks, packed = run_case(FS, L, 2, fmt, systems, signals, seed=11 + fmt)
     assert "fused16368" in plan.describe()
+    assert f"rows_variant={fused_variant}" in plan.describe()
     check_against_oracle(plan, peaks, packed, FS, L, fmt, systems)
     # injected signals are found at the expected code phase and nearest Doppler bin
     for si, sv in ((0, 1), (0, 2), (1, 1)):
@@ -77,7 +85,7 @@ def test_fused_ca_plan_matches_oracle(gpu, fmt):
             assert abs(fc[int(pk["best_freq_idx"])] - IF - s["doppler"]) <= 250.0
 
 
-def test_fused_galileo_at_4092khz(gpu):
+def test_fused_galileo_at_4092khz(gpu, fused_variant):
     # E1-C at fs = 4.092 MHz has P = 16368 samples per 4 ms period: the fused kernels serve it.
     fs, L = 4.092e6, 2 * 16368 + 1000
     signals = [dict(system="galileo", prn=11, cn0=50.0, doppler=-700.0, delay=4321)]
