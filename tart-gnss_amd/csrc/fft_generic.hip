@@ -6,9 +6,13 @@
 //   v    = DFT_R(v)
 //   dst[(j / ns) ns R + k + r ns] = v[r]
 // Reads of consecutive j are consecutive addresses; so are the writes once ns > 1.  Radices
-// 16, 8, 4, 2, 3, 5, 7, 11, 13, 31 have register codelets; any other prime factor p runs
-// through a direct O(p^2) butterfly.  This engine serves every length that has no fused LDS
-// kernel: level-1 calls of any N, replica spectra, RFI records, long code periods.
+// 16, 8, 4, 2, 3, 5, 7, 11, 13, 31 have register codelets; any other prime factor p up to
+// kMaxDirectPrime runs through a direct O(p^2) butterfly.  A length with a larger prime factor
+// (a ragged record, an odd sampling rate) is computed with Bluestein's chirp-z identity
+//   X_k = w_k sum_j (x_j w_j) conj(w_{k-j}),   w_k = exp(-i pi k^2 / n),
+// as a cyclic convolution of power-of-two length m >= 2n - 1 on this same engine, so every
+// length costs O(n log n).  This engine serves every length that has no fused LDS kernel:
+// level-1 calls of any N, replica spectra, RFI records, long code periods.
 #include <algorithm>
 #include <cmath>
 #include <map>
@@ -26,6 +30,10 @@ struct FftPlan {
   int split = 1;
   DevBuf tw;  // tw1[split] then tw2[ceil(n/split)]
   std::vector<DevBuf> prime_roots;  // per pass with a generic radix: W_R^m, m < R
+  // Bluestein form (m > 0): m-point sub-plan, chirp w_k (k < n), FFT_m(b) / m
+  size_t m = 0;
+  const FftPlan* sub = nullptr;
+  DevBuf chirp, bspec;
 };
 
 struct FftPlanD {
@@ -34,11 +42,15 @@ struct FftPlanD {
   int split = 1;
   DevBuf tw;
   std::vector<DevBuf> prime_roots;
+  size_t m = 0;
+  const FftPlanD* sub = nullptr;
+  DevBuf chirp, bspec;
 };
 
 namespace {
 
 constexpr int kBlock = 256;
+constexpr size_t kMaxDirectPrime = 127;  // larger prime factors switch to Bluestein
 
 bool has_codelet(int r) {
   switch (r) {
@@ -57,6 +69,67 @@ void push_odd_primes(size_t m, std::vector<int>& out) {
 size_t strip_pow2(size_t m) {
   while (m % 2 == 0) m /= 2;
   return m;
+}
+
+size_t largest_prime_factor(size_t m) {
+  size_t best = 1;
+  for (size_t p = 2; p * p <= m; ++p)
+    while (m % p == 0) { best = p; m /= p; }
+  return m > 1 ? m : best;
+}
+
+// ---- Bluestein (chirp-z) stages; T is float2 or double2 --------------------------------------
+template <class T>
+__device__ __forceinline__ T bmul(T a, T b) {
+  T r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+
+// a[row][k] = x[row][k] w_k for k < n, 0 up to m
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_blu_pre(const T* __restrict__ x,
+                                                    const T* __restrict__ chirp,
+                                                    T* __restrict__ a, long long n, long long m,
+                                                    long long total) {
+  const long long gid = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (gid >= total) return;
+  const long long row = gid / m, k = gid - row * m;
+  T v;
+  v.x = 0;
+  v.y = 0;
+  if (k < n) v = bmul(x[row * n + k], chirp[k]);
+  a[gid] = v;
+}
+
+// A <- conj(A * B / m) (B already carries 1/m): the inverse transform as a forward one
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_blu_mid(T* __restrict__ A, const T* __restrict__ B,
+                                                    long long m, long long total) {
+  const long long gid = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (gid >= total) return;
+  T v = bmul(A[gid], B[gid % m]);
+  v.y = -v.y;
+  A[gid] = v;
+}
+
+// y[row][k] = w_k conj(R[row][k]) for k < n
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_blu_post(const T* __restrict__ R,
+                                                     const T* __restrict__ chirp,
+                                                     T* __restrict__ y, long long n, long long m,
+                                                     long long total) {
+  const long long gid = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (gid >= total) return;
+  const long long row = gid / n, k = gid - row * n;
+  T r = R[row * m + k];
+  r.y = -r.y;
+  y[gid] = bmul(chirp[k], r);
+}
+
+unsigned grid_for(long long total) {
+  return static_cast<unsigned>((total + kBlock - 1) / kBlock);
 }
 
 // Single precision pass order: odd radices first, then 16s and one 8/4/2 remainder.
@@ -201,9 +274,64 @@ std::vector<T> host_roots(size_t n, size_t count, size_t step) {
   return out;
 }
 
+inline float2* forward_any(const FftPlan* p, float2* d, float2* s, size_t b, hipStream_t st) {
+  return fft_forward(p, d, s, b, st);
+}
+inline double2* forward_any(const FftPlanD* p, double2* d, double2* s, size_t b,
+                            hipStream_t st) {
+  return fft_forward_d(p, d, s, b, st);
+}
+inline const FftPlan* plan_any(size_t n, const FftPlan*) { return fft_plan(n); }
+inline const FftPlanD* plan_any(size_t n, const FftPlanD*) { return fft_plan_d(n); }
+
+// Bluestein tables: w_k = exp(-i pi k^2 / n) with k^2 reduced mod 2n exactly, and the
+// spectrum of b (b_l = conj(w_l) for |l| < n, cyclic over m, scaled by 1/m, a power of two).
+template <class Plan, class T>
+void init_bluestein(Plan& p, size_t n) {
+  size_t m = 1;
+  while (m < 2 * n - 1) m <<= 1;
+  p.m = m;
+  p.sub = plan_any(m, static_cast<const Plan*>(nullptr));
+  std::vector<T> w(n), b(m);
+  for (size_t k = 0; k < n; ++k) {
+    const unsigned long long e =
+        (static_cast<unsigned long long>(k) * k) % (2ull * static_cast<unsigned long long>(n));
+    const double a = -kPiD * static_cast<double>(e) / static_cast<double>(n);
+    w[k].x = static_cast<decltype(w[k].x)>(std::cos(a));
+    w[k].y = static_cast<decltype(w[k].y)>(std::sin(a));
+  }
+  const double inv_m = 1.0 / static_cast<double>(m);
+  for (size_t l = 0; l < m; ++l) {
+    b[l].x = 0;
+    b[l].y = 0;
+  }
+  for (size_t l = 0; l < n; ++l) {
+    const unsigned long long e =
+        (static_cast<unsigned long long>(l) * l) % (2ull * static_cast<unsigned long long>(n));
+    const double a = kPiD * static_cast<double>(e) / static_cast<double>(n);
+    T v;
+    v.x = static_cast<decltype(v.x)>(std::cos(a) * inv_m);
+    v.y = static_cast<decltype(v.y)>(std::sin(a) * inv_m);
+    b[l] = v;
+    if (l > 0) b[m - l] = v;
+  }
+  p.chirp.alloc(n * sizeof(T));
+  TGA_HIP(hipMemcpy(p.chirp.p, w.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  DevBuf d(m * sizeof(T)), sc(m * sizeof(T));
+  TGA_HIP(hipMemcpy(d.p, b.data(), m * sizeof(T), hipMemcpyHostToDevice));
+  T* r = forward_any(p.sub, d.as<T>(), sc.as<T>(), 1, nullptr);
+  p.bspec.alloc(m * sizeof(T));
+  TGA_HIP(hipMemcpyAsync(p.bspec.p, r, m * sizeof(T), hipMemcpyDeviceToDevice, nullptr));
+  TGA_HIP(hipStreamSynchronize(nullptr));
+}
+
 template <class Plan, class T>
 void init_plan(Plan& p, size_t n, bool codelets) {
   p.n = n;
+  if (n > 1 && largest_prime_factor(n) > kMaxDirectPrime) {
+    init_bluestein<Plan, T>(p, n);
+    return;
+  }
   p.radix = codelets ? factorize(n) : factorize_d(n);
   p.split = static_cast<int>(std::ceil(std::sqrt(static_cast<double>(n))));
   if (p.split < 1) p.split = 1;
@@ -226,7 +354,7 @@ void init_plan(Plan& p, size_t n, bool codelets) {
   }
 }
 
-std::mutex g_plan_mu;
+std::recursive_mutex g_plan_mu;  // Bluestein plans build their sub-plan inside
 std::map<std::pair<int, size_t>, std::unique_ptr<FftPlan>> g_plans;
 std::map<std::pair<int, size_t>, std::unique_ptr<FftPlanD>> g_plans_d;
 
@@ -244,7 +372,7 @@ const FftPlan* fft_plan(size_t n) {
   if (n == 0) fail(TGA_ERR_INVALID_ARGUMENT, "FFT length must be positive");
   if (n > (1u << 30)) fail(TGA_ERR_UNSUPPORTED, "FFT length too large");
   const int dev = current_device_checked();
-  std::lock_guard<std::mutex> lk(g_plan_mu);
+  std::lock_guard<std::recursive_mutex> lk(g_plan_mu);
   auto key = std::make_pair(dev, n);
   auto it = g_plans.find(key);
   if (it != g_plans.end()) return it->second.get();
@@ -265,10 +393,37 @@ TwiddleView fft_twiddles(const FftPlan* plan) {
   return v;
 }
 
+namespace {
+
+// The m-point work buffers are allocated per call and released after the stream drains: the
+// Bluestein form only serves lengths with a large prime factor, never the fused search.
+template <class Plan, class T>
+T* bluestein_forward(const Plan* plan, T* data, size_t batch, hipStream_t st) {
+  const long long n = static_cast<long long>(plan->n), m = static_cast<long long>(plan->m);
+  const long long tm = m * static_cast<long long>(batch), tn = n * static_cast<long long>(batch);
+  DevBuf a(static_cast<size_t>(tm) * sizeof(T)), s(static_cast<size_t>(tm) * sizeof(T));
+  hipLaunchKernelGGL(k_blu_pre<T>, dim3(grid_for(tm)), dim3(kBlock), 0, st, data,
+                     plan->chirp.template as<T>(), a.as<T>(), n, m, tm);
+  TGA_HIP(hipGetLastError());
+  T* A = forward_any(plan->sub, a.as<T>(), s.as<T>(), batch, st);
+  hipLaunchKernelGGL(k_blu_mid<T>, dim3(grid_for(tm)), dim3(kBlock), 0, st, A,
+                     plan->bspec.template as<T>(), m, tm);
+  TGA_HIP(hipGetLastError());
+  T* R = forward_any(plan->sub, A, A == a.as<T>() ? s.as<T>() : a.as<T>(), batch, st);
+  hipLaunchKernelGGL(k_blu_post<T>, dim3(grid_for(tn)), dim3(kBlock), 0, st, R,
+                     plan->chirp.template as<T>(), data, n, m, tn);
+  TGA_HIP(hipGetLastError());
+  TGA_HIP(hipStreamSynchronize(st));
+  return data;
+}
+
+}  // namespace
+
 float2* fft_forward(const FftPlan* plan, float2* data, float2* scratch, size_t batch,
                     hipStream_t st) {
   const int n = static_cast<int>(plan->n);
   if (n == 1 || batch == 0) return data;
+  if (plan->m) return bluestein_forward<FftPlan, float2>(plan, data, batch, st);
   const float2* tw1 = plan->tw.as<float2>();
   const float2* tw2 = tw1 + plan->split;
   float2* src = data;
@@ -305,7 +460,7 @@ const FftPlanD* fft_plan_d(size_t n) {
   if (n == 0) fail(TGA_ERR_INVALID_ARGUMENT, "FFT length must be positive");
   if (n > (1u << 30)) fail(TGA_ERR_UNSUPPORTED, "FFT length too large");
   const int dev = current_device_checked();
-  std::lock_guard<std::mutex> lk(g_plan_mu);
+  std::lock_guard<std::recursive_mutex> lk(g_plan_mu);
   auto key = std::make_pair(dev, n);
   auto it = g_plans_d.find(key);
   if (it != g_plans_d.end()) return it->second.get();
@@ -320,6 +475,7 @@ double2* fft_forward_d(const FftPlanD* plan, double2* data, double2* scratch, si
                        hipStream_t st) {
   const int n = static_cast<int>(plan->n);
   if (n == 1 || batch == 0) return data;
+  if (plan->m) return bluestein_forward<FftPlanD, double2>(plan, data, batch, st);
   const double2* tw1 = plan->tw.as<double2>();
   const double2* tw2 = tw1 + plan->split;
   double2* src = data;

@@ -309,3 +309,62 @@ class Plan:
         buf = C.create_string_buffer(4096)
         check(lib().tga_plan_describe(self._h, buf, 4096))
         return buf.value.decode()
+
+    def _json(self, fn, first, center_freq, antenna_numbers, cn0, filter_phase_mad,
+              filter_freq_mad):
+        cf = _f64(center_freq if np.ndim(center_freq) else [center_freq] * len(self.systems))
+        an = np.ascontiguousarray(antenna_numbers if antenna_numbers is not None
+                                  else range(self.n_ant), dtype=np.int64)
+        if len(cf) != len(self.systems) or len(an) != self.n_ant:
+            raise ValueError("center_freq per system and antenna_numbers per antenna")
+        out = C.c_void_p()
+        fpm = -1.0 if filter_phase_mad is None else float(filter_phase_mad)
+        ffm = -1.0 if filter_freq_mad is None else float(filter_freq_mad)
+        check(fn(self._h, *first, _ptr(cf, C.c_double), _ptr(an, C.c_int64),
+                 1 if cn0 else 0, fpm, ffm, C.byref(out)))
+        try:
+            return C.string_at(out.value).decode()
+        finally:
+            lib().tga_free_string(out)
+
+    def acquire_all_json(self, packed, center_freq=4.092e6, antenna_numbers=None, cn0=False,
+                         filter_phase_mad=None, filter_freq_mad=None):
+        """Search one observation and return the reference's CombinedOutput JSON text."""
+        pk = np.ascontiguousarray(packed, dtype=np.uint8)
+        return self._json(lib().tga_acquire_all_json,
+                          (_ptr(pk, C.c_uint8), pk.shape[0], pk.shape[1]), center_freq,
+                          antenna_numbers, cn0, filter_phase_mad, filter_freq_mad)
+
+    def peaks_json(self, peaks, center_freq=4.092e6, antenna_numbers=None, cn0=False,
+                   filter_phase_mad=None, filter_freq_mad=None):
+        """JSON of peaks returned by search() (list per system of [n_ant, n_prn] arrays)."""
+        flat = np.ascontiguousarray(np.concatenate([p.reshape(-1) for p in peaks]))
+        return self._json(lib().tga_peaks_json, (flat.ctypes.data_as(C.POINTER(L.Peak)),),
+                          center_freq, antenna_numbers, cn0, filter_phase_mad, filter_freq_mad)
+
+
+def format_f64(x):
+    """The JSON text serde_json writes for an f64 (tga_format_f64)."""
+    buf = C.create_string_buffer(64)
+    check(lib().tga_format_f64(float(x), buf, 64))
+    return buf.value.decode()
+
+
+def rfi(packed, record_samples, input_format, sampling_freq):
+    """Narrowband RFI report per antenna (rfi.rs:142-184) on the GPU; a list of dicts shaped
+    like the reference's RfiChannel (None where the reference has None)."""
+    pk = np.ascontiguousarray(packed, dtype=np.uint8)
+    if pk.ndim != 2:
+        raise ValueError("packed must be [n_ant, row_bytes]")
+    out = (L.RfiChannel * pk.shape[0])()
+    check(lib().tga_rfi(_ptr(pk, C.c_uint8), pk.shape[0], pk.shape[1], int(record_samples),
+                        int(input_format), float(sampling_freq), out))
+
+    def opt(x):
+        return None if np.isnan(x) else float(x)
+
+    return [{"antenna": ch.antenna, "dead": bool(ch.dead), "acf_lag1": opt(ch.acf_lag1),
+             "acf_lag3": opt(ch.acf_lag3), "acf_lag16": opt(ch.acf_lag16),
+             "period_samples": opt(ch.period_samples),
+             "lines": [{"frequency": ch.lines[i].frequency, "excess_db": ch.lines[i].excess_db}
+                       for i in range(ch.n_lines)]} for ch in out]

@@ -26,7 +26,9 @@ def make_case(system, prn, P, N, fs, delay, f0, noise, seed):
 @pytest.mark.parametrize("N,P,fs,nbins", [(256, 256, 1.023e6, 3), (512, 256, 1.023e6, 5),
                                           (2046, 1023, 1.023e6, 7), (4092, 2046, 2.046e6, 9),
                                           (2043, 2046, 2.046e6, 4), (4099, 2046, 2.046e6, 3),
-                                          (32736, 16368, 16.368e6, 5)])
+                                          (32736, 16368, 16.368e6, 5),
+                                          # large prime factors (Bluestein form): 3 * 11171
+                                          (33513, 16368, 16.368e6, 3)])
 def test_correlate_code_matches_oracle(gpu, N, P, fs, nbins):
     code, sig = make_case("gps", 7, P, N, fs, 301 % P, 1500.0, 0.8, N + 3)
     pp = oc.phasepoints(fs, N)
@@ -41,7 +43,8 @@ def test_correlate_code_matches_oracle(gpu, N, P, fs, nbins):
 
 
 def test_code_spectra_matches_oracle(gpu):
-    for N, P in [(2046, 1023), (32736, 16368), (1500, 1023)]:
+    # 4099 is prime and 33513 = 3 * 11171: both run the Bluestein (chirp-z) form
+    for N, P in [(2046, 1023), (32736, 16368), (1500, 1023), (4099, 2046), (33513, 16368)]:
         code = ocodes.code_resampled("gps", 3, float(P), N)
         cc, ca = tga.code_spectra(code, P)
         rc, ra = oc.code_spectra(code, P, mode="fp64")
