@@ -4,7 +4,9 @@ Workload (default): GPS + SBAS + QZSS L1 C/A, 24 antennas, 2-bit real samples at
 16.368 MHz, 10 ms records (163680 samples); seeded signals: 8 GPS, 2 SBAS and 1 QZSS at
 C/N0 uniform in [38, 50] dB-Hz with per-SV Doppler and per-antenna carrier offsets from a
 3 m disk; search 38 + 39 + 23 PRNs x 41 Doppler bins (+-10 kHz, 500 Hz step), N = 2P
-(reference form, two hypotheses), FP32, plus ACR C/N0 on the host.
+(reference form, two hypotheses), FP32, plus ACR C/N0 on the host.  The FP16 matrix-core
+form (f16 spectra, radix passes as f16 GEMMs, f64 carrier phase) then runs the same
+observation and is reported under extra.fp16_mfma with its difference to the FP32 peaks.
 
 A step = one tga_search_device call on an observation already resident in HBM: decode +
 de-mean, forward spectra, all correlation rows, selection, copy of the 2400 peaks to the
@@ -63,6 +65,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU sample")
     ap.add_argument("--seed", type=int, default=0x5EED_2026 + 2)
     ap.add_argument("--traffic", default=os.path.join(REPO, "profiles", "pmc_traffic.json"))
+    ap.add_argument("--no-fp16", action="store_true",
+                    help="skip the FP16 matrix-core form (reported beside the FP32 value)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="first run one elementwise copy of MIB MiB (reads and writes MIB MiB) "
                          "so hardware byte counters can be scaled against a known size")
@@ -107,8 +111,8 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    def step():
-        peaks = plan.search_device(ptr)
+    def step(pl):
+        peaks = pl.search_device(ptr)
         # ACR C/N0 per (antenna, PRN) on the host in f64 (acquisition.rs:341-357)
         ok = 0
         for arr in peaks:
@@ -121,15 +125,20 @@ def main():
                         ok += 1
         return peaks, ok
 
-    for _ in range(args.warmup):
-        peaks, _ = step()
-    plan.set_timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        peaks, n_cn0 = step()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    def timed(pl):
+        """Warm-up, then exactly args.steps timed steps; returns (seconds, last peaks, cn0)."""
+        for _ in range(args.warmup):
+            step(pl)
+        pl.set_timing(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            pk, n_ok = step(pl)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        return dt, pk, n_ok
+
+    dt, peaks, n_cn0 = timed(plan)
     ms_step = dt * 1000.0 / args.steps
     value = cells * args.steps / dt
 
@@ -139,6 +148,31 @@ def main():
     avg_rows_s = rows_ms / 1000.0 / max(rows_n, 1)
     achieved = rows_bytes / avg_rows_s / 1e9
     traffic = load_traffic(args.traffic, plan.kernel_name(0))
+
+    fp16 = None
+    if not args.no_fp16:
+        # the FP16 matrix-core form on the same observation, compared with the FP32 peaks
+        plan16 = tga.Plan(FS, n_ant, L, systems, input_format=2, precision="fp16")
+        dt16, peaks16, _ = timed(plan16)
+        r_ms, r_n, r_bytes = plan16.timing(0)
+        avg16 = r_ms / 1000.0 / max(r_n, 1)
+        rel, same = [], 0
+        for a32, a16 in zip(peaks, peaks16):
+            b32 = a32["best_peak"].astype(np.float64)
+            rel.append(np.abs(a16["best_peak"].astype(np.float64) - b32) / b32)
+            same += int(np.sum((a32["best_codephase"] == a16["best_codephase"]) &
+                               (a32["best_freq_idx"] == a16["best_freq_idx"])))
+        rel = np.concatenate([r.ravel() for r in rel])
+        fp16 = {"value": cells * args.steps / dt16, "unit": "cells/s",
+                "ms_per_step": dt16 * 1000.0 / args.steps, "kernel": plan16.kernel_name(0),
+                "kernel_ms": avg16 * 1000.0, "alg_bytes_per_launch": r_bytes,
+                "achieved_GBs": r_bytes / avg16 / 1e9,
+                "frac": r_bytes / avg16 / 1e9 / HBM_PEAK_GBS,
+                "speedup_vs_fp32": (cells * args.steps / dt16) / value,
+                "best_peak_rel_diff_vs_fp32": {"max": float(rel.max()),
+                                               "median": float(np.median(rel))},
+                "same_bin_and_codephase_as_fp32": f"{same}/{rel.size}",
+                "plan": plan16.describe().splitlines()[0]}
 
     # detections: every injected SV found at its code phase and nearest bin on most antennas
     found = 0
@@ -189,7 +223,7 @@ def main():
         "cpu_baseline": cpu,
         "extra": {"forward_ms": fwd_ms / max(fwd_n, 1), "select_ms": sel_ms / max(sel_n, 1),
                   "plan_create_s": t_plan, "synth_s": t_synth, "cn0_valid": n_cn0,
-                  "injected_found": f"{found}/{len(signals)}"},
+                  "injected_found": f"{found}/{len(signals)}", "fp16_mfma": fp16},
     }
     print(json.dumps(out))
 

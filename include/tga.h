@@ -154,12 +154,26 @@ typedef struct tga_system_desc {
   int64_t num_samples; /* coherent length N; 0 selects the reference's min(2P, record) */
 } tga_system_desc;
 
+/* Arithmetic form of the correlation rows.
+ *   TGA_PRECISION_FP32  the reference form: f32 spectra and f32 transforms, results within
+ *                       1e-5 of the reference's peaks (the parity path).
+ *   TGA_PRECISION_FP16  an extension with no counterpart in the reference: groups served by
+ *                       the fused P = 16368 kernels store the spectra as f16 and run the
+ *                       radix-33/16/31 passes as f16 GEMMs on the matrix cores with f32
+ *                       accumulation; the carrier phase stays in f64 (the FP32 form rounds
+ *                       it to f32 like the reference);
+ *                       other groups run the FP32 form.  A row whose values leave the f16
+ *                       range reports TGA_ERR_NAN. */
+#define TGA_PRECISION_FP32 0
+#define TGA_PRECISION_FP16 1
+
 typedef struct tga_plan_desc {
   double sampling_freq;   /* Hz; 16.368e6 for TART */
   int32_t n_ant;          /* antenna rows per observation */
   int32_t input_format;   /* TGA_INPUT_1BIT or TGA_INPUT_2BIT */
   int64_t record_samples; /* samples per antenna row (the whole record: de-meaning uses all) */
   int32_t n_systems;
+  int32_t precision;      /* TGA_PRECISION_FP32 (0) or TGA_PRECISION_FP16 */
   const tga_system_desc* systems;
 } tga_plan_desc;
 

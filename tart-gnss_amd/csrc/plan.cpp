@@ -54,6 +54,9 @@ void Plan::build(const tga_plan_desc& d) {
     fail(TGA_ERR_INVALID_ARGUMENT, "input_format must be TGA_INPUT_1BIT or TGA_INPUT_2BIT");
   if (d.n_systems < 1 || d.systems == nullptr)
     fail(TGA_ERR_INVALID_ARGUMENT, "a plan needs at least one system");
+  if (d.precision != TGA_PRECISION_FP32 && d.precision != TGA_PRECISION_FP16)
+    fail(TGA_ERR_INVALID_ARGUMENT, "precision must be TGA_PRECISION_FP32 or TGA_PRECISION_FP16");
+  precision = d.precision;
   current_device_checked();
   fs = d.sampling_freq;
   n_ant = d.n_ant;
@@ -177,17 +180,27 @@ void Plan::build(const tga_plan_desc& d) {
     g.d_rep = std::move(r == a.as<float2>() ? a : sc);
     g.codes.clear();
     g.codes.shrink_to_fit();
+    g.fp16 = g.fused && precision == TGA_PRECISION_FP16;
     // forward spectra and per-row results
     g.n_xrow = static_cast<size_t>(n_ant) * n_bin * (g.folded ? 2 : 1);
-    g.d_X.alloc(g.n_xrow * M * sizeof(float2));
     g.n_rows = static_cast<long long>(n_ant) * n_prn * n_bin * 2;
     g.d_rows.alloc(static_cast<size_t>(g.n_rows) * sizeof(RowOut));
     g.n_prn = n_prn;
     g.n_bin = n_bin;
-    if (g.fused) {
+    if (g.fp16) {
+      const long long nr = static_cast<long long>(n_prn) * 2 * static_cast<long long>(M);
+      g.d_rep16.alloc(static_cast<size_t>(nr) * sizeof(uint32_t));
+      launch_to_f16(g.d_rep.as<float2>(), g.d_rep16.as<uint32_t>(), nr, kF16Scale, stream);
+      TGA_HIP(hipStreamSynchronize(stream));
+      g.d_rep.release();
+      g.d_X16.alloc(g.n_xrow * M * sizeof(uint32_t));
+      upload(g.d_tw, fused_twiddles(), stream);
+    } else if (g.fused) {
+      g.d_X.alloc(g.n_xrow * M * sizeof(float2));
       upload(g.d_tw, fused_twiddles(), stream);
       autotune(g);
     } else {
+      g.d_X.alloc(g.n_xrow * M * sizeof(float2));
       g.d_Xs.alloc(g.n_xrow * M * sizeof(float2));
       // rows are processed in chunks of at most ~32 Mi complex values (256 MiB) per buffer
       const long long cap = std::max<long long>(1, (32LL << 20) / static_cast<long long>(M));
@@ -197,8 +210,9 @@ void Plan::build(const tga_plan_desc& d) {
     }
     // algorithmic bytes of the rows stage: each row reads its forward spectrum and replica
     // spectrum once (M complex f32 each)
-    g.row_bytes_alg = static_cast<double>(g.n_rows) * 2.0 * static_cast<double>(M) * 8.0;
-    g.fwd_bytes_alg = static_cast<double>(g.n_xrow) * static_cast<double>(M) * 8.0;
+    const double elem = g.fp16 ? 4.0 : 8.0;  // bytes per complex value
+    g.row_bytes_alg = static_cast<double>(g.n_rows) * 2.0 * static_cast<double>(M) * elem;
+    g.fwd_bytes_alg = static_cast<double>(g.n_xrow) * static_cast<double>(M) * elem;
   }
   TGA_HIP(hipStreamSynchronize(stream));
   for (auto& t : timers) {
@@ -285,7 +299,11 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
   time_begin(1, st);
   for (auto& g : groups) {
     const int M = static_cast<int>(g.M);
-    if (g.fused) {
+    if (g.fp16) {
+      launch_fused_forward16(d_in, row_bytes, fmt, d_means.as<double>(), g.d_fc.as<double>(),
+                             n_ant, g.n_bin, phase_const, g.d_tau.as<float2>(),
+                             g.d_tw.as<float2>(), g.d_X16.as<uint32_t>(), st);
+    } else if (g.fused) {
       launch_fused_forward(d_in, row_bytes, fmt, d_means.as<double>(), g.d_fc.as<double>(),
                            n_ant, g.n_bin, phase_const, g.d_tau.as<float2>(),
                            g.d_tw.as<float2>(), g.d_X.as<float2>(), st);
@@ -305,7 +323,10 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
   time_begin(0, st);
   for (auto& g : groups) {
     const int M = static_cast<int>(g.M);
-    if (g.fused) {
+    if (g.fp16) {
+      launch_fused_rows16(g.d_X16.as<uint32_t>(), g.d_rep16.as<uint32_t>(), g.d_tw.as<float2>(),
+                          n_ant, g.n_bin, g.n_prn, g.cw, g.d_rows.as<RowOut>(), 8, st);
+    } else if (g.fused) {
       launch_fused_rows(g.X, g.d_rep.as<float2>(), g.d_tw.as<float2>(), n_ant, g.n_bin, g.n_prn,
                         g.cw, g.d_rows.as<RowOut>(), st, g.rows_cfg);
     } else {
@@ -323,9 +344,12 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
   }
   time_end(0, st, rows_bytes);
   time_begin(2, st);
-  for (auto& g : groups)
+  for (auto& g : groups) {
+    // folded rows carry half the 2P-point gain; the FP16 form's powers carry kF16Scale^4
+    const float gain = (g.folded ? 2.0f : 1.0f) / (g.fp16 ? kF16Scale * kF16Scale : 1.0f);
     launch_select(g.d_rows.as<RowOut>(), g.d_slots.as<PeakSlot>(), n_ant, g.n_prn, g.n_bin,
-                  g.folded ? 2.0f : 1.0f, g.scale, d_peaks.as<tga_peak>(), d_nan.as<int>(), st);
+                  gain, g.scale, d_peaks.as<tga_peak>(), d_nan.as<int>(), st);
+  }
   time_end(2, st, 0.0);
 }
 
@@ -337,7 +361,9 @@ void Plan::finish(tga_peak* out, hipStream_t st) {
   TGA_HIP(hipStreamSynchronize(st));
   time_collect();
   if (nan)
-    fail(TGA_ERR_NAN, "called `Option::unwrap()` on a `None` value (NaN in the correlation)");
+    fail(TGA_ERR_NAN, precision == TGA_PRECISION_FP16
+                          ? "non-finite correlation value (NaN input or f16 range exceeded)"
+                          : "called `Option::unwrap()` on a `None` value (NaN in the correlation)");
 }
 
 std::string Plan::describe() const {
@@ -349,7 +375,9 @@ std::string Plan::describe() const {
                   "%s: P=%zu N=%zu bins=%zu prns=%zu group=%d path=%s rows_variant=%d "
                   "groups=%d tune_ms=%.3f/%.3f\n",
                   system_info(e.system).key, g.P, g.N, g.fc.size(), e.prns.size(), e.group,
-                  g.fused ? "fused16368" : (g.folded ? "generic-folded" : "generic-unfolded"),
+                  g.fp16 ? "fused16368-fp16"
+                         : g.fused ? "fused16368"
+                                   : (g.folded ? "generic-folded" : "generic-unfolded"),
                   g.rows_cfg.variant, g.rows_cfg.groups, g.tune_ms[0], g.tune_ms[1]);
     s += buf;
   }
@@ -464,8 +492,9 @@ int tga_plan_get_timing(tga_plan* plan, int32_t kernel, double* total_ms, int64_
 const char* tga_plan_kernel_name(const tga_plan* plan, int32_t kernel) {
   const Plan* p = reinterpret_cast<const Plan*>(plan);
   const bool fused = p && !p->groups.empty() && p->groups[0].fused;
+  const bool fp16 = fused && p->groups[0].fp16;
   switch (kernel) {
-    case 0: return fused ? "k_rows" : "k_mul_rows+k_pass+k_row_reduce";
+    case 0: return fp16 ? "k_rows16" : fused ? "k_rows" : "k_mul_rows+k_pass+k_row_reduce";
     case 1: return fused ? "k_forward" : "k_wipe_obs+k_pass";
     case 2: return "k_select";
   }

@@ -15,7 +15,7 @@ constexpr double kPi = 3.14159265358979323846;
 struct Group {
   size_t P = 0, N = 0, M = 0;  // code period, coherent length, transform length
   std::vector<double> fc;
-  bool folded = false, fused = false;
+  bool folded = false, fused = false, fp16 = false;
   std::vector<std::vector<double>> codes;  // replicas, plan creation only
   std::vector<PeakSlot> slots;
   int n_prn = 0, n_bin = 0;
@@ -24,6 +24,7 @@ struct Group {
   float scale = 1.f;
   int cw = 1;
   DevBuf d_fc, d_slots, d_tau, d_rep, d_X, d_Xs, d_rows, d_Z, d_Zs, d_tw;
+  DevBuf d_rep16, d_X16;  // FP16 form: f16 (re, im) pairs scaled by kF16Scale
   float2* X = nullptr;  // where the forward spectra of the last search landed
   FusedRowsConfig rows_cfg;  // rows-kernel shape picked by the autotuner
   double tune_ms[4] = {0, 0, 0, 0};
@@ -48,7 +49,7 @@ struct Timer {
 
 struct Plan {
   double fs = 0, phase_const = 0;
-  int n_ant = 0, fmt = 1;
+  int n_ant = 0, fmt = 1, precision = TGA_PRECISION_FP32;
   long long L = 0;
   size_t row_bytes = 0, n_peaks = 0;
   std::vector<SysEntry> systems;

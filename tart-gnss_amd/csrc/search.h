@@ -46,6 +46,19 @@ struct FusedRowsConfig {
 void launch_fused_rows(const float2* X, const float2* rep, const float2* tw, int n_ant,
                        int n_bin, int n_prn, int cw, RowOut* rows, hipStream_t st,
                        const FusedRowsConfig& cfg);
+// FP16 form of the fused path: X and the replica spectra are stored as f16 (re, im) pairs of
+// the value times kF16Scale (so |conj(X) R| stays far inside the f16 range for 1- and 2-bit
+// input), the radix passes run as f16 GEMMs on the matrix cores with f32 accumulation, and the
+// row powers are those of the transform times kF16Scale^2.
+constexpr float kF16Scale = 1.0f / 256.0f;
+void launch_fused_forward16(const uint8_t* packed, size_t row_bytes, int fmt,
+                            const double* means, const double* fc, int n_ant, int n_bin,
+                            double phase_const, const float2* tau, const float2* tw,
+                            uint32_t* X16, hipStream_t st);
+void launch_fused_rows16(const uint32_t* X16, const uint32_t* rep16, const float2* tw,
+                         int n_ant, int n_bin, int n_prn, int cw, RowOut* rows, int groups,
+                         hipStream_t st);
+void launch_to_f16(const float2* in, uint32_t* out, long long n, float scale, hipStream_t st);
 // Twiddle table of the fused kernels: W_P^i (i < 128) then W_P^{128 i} (i < 128).
 std::vector<float2> fused_twiddles();
 

@@ -52,6 +52,16 @@ __device__ __forceinline__ float2 wipe_obs(const uint8_t* row, int fmt, long lon
   return wipe_value(s, __dmul_rn(pc, static_cast<double>(n)), f);
 }
 
+// The same sample with the carrier phase kept in f64 (no f32 rounding of phi): the precision
+// form's NCO, whose phase error stays ~1e-16 rad over any record length.
+__device__ __forceinline__ float2 wipe_obs_f64(const uint8_t* row, int fmt, long long n,
+                                               double mean, double pc, double f) {
+  const float s = __double2float_rn(static_cast<double>(decode_level(row, fmt, n)) - mean);
+  double sd, cd;
+  sincos(__dmul_rn(__dmul_rn(pc, static_cast<double>(n)), f), &sd, &cd);
+  return make_float2(__double2float_rn(cd) * s, __double2float_rn(sd) * s);
+}
+
 // ---- block reductions (wave64 shuffles, then LDS across waves) --------------------------------
 __device__ __forceinline__ void arg_combine(float& p, int& i, float q, int j) {
   if (q > p || (q == p && j > i)) {
@@ -244,9 +254,21 @@ struct FwdArgs {
   double pc;
   const float2* tau;
   const float2* tw;
-  float2* X;
+  float2* X;     // FP32 form
+  uint32_t* X16;  // FP16 form: f16 (re, im) pairs of X * kF16Scale
 };
 
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// f16 (re, im) pair of v * s, rounded to nearest
+__device__ __forceinline__ uint32_t pack_f16(float2 v, float s) {
+  return __builtin_bit_cast(uint32_t, half2_t{static_cast<_Float16>(v.x * s),
+                                              static_cast<_Float16>(v.y * s)});
+}
+
+template <bool F16>
 __global__ __launch_bounds__(T) void k_forward(FwdArgs a) {
   __shared__ float2 buf[P];
   __shared__ float2 stw[256];
@@ -260,14 +282,15 @@ __global__ __launch_bounds__(T) void k_forward(FwdArgs a) {
   const double mean = a.means[ant];
   const double f = a.fc[bin];
   for (int n = tid; n < P; n += T) {
-    const float2 x0 = wipe_obs(row, a.fmt, n, mean, a.pc, f);
-    const float2 x1 = wipe_obs(row, a.fmt, n + P, mean, a.pc, f);
+    const float2 x0 = F16 ? wipe_obs_f64(row, a.fmt, n, mean, a.pc, f)
+                          : wipe_obs(row, a.fmt, n, mean, a.pc, f);
+    const float2 x1 = F16 ? wipe_obs_f64(row, a.fmt, n + P, mean, a.pc, f)
+                          : wipe_obs(row, a.fmt, n + P, mean, a.pc, f);
     buf[n] = h == 0 ? cadd(x0, x1) : cmul(csub(x0, x1), a.tau[n]);
   }
   __syncthreads();
   passes_to_last(buf, stw, tid);
   // last pass: radix 16 after 1023, output in natural order
-  float2* out = a.X + static_cast<size_t>(xrow) * P;
 #pragma unroll
   for (int b = 0; b < 2; ++b) {
     const int j = tid + b * T;
@@ -278,8 +301,15 @@ __global__ __launch_bounds__(T) void k_forward(FwdArgs a) {
 #pragma unroll
       for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], tw_lds(stw, r * j));
       Dft<16>::run(v);
+      if constexpr (F16) {
+        uint32_t* out = a.X16 + static_cast<size_t>(xrow) * P;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) out[j + r * NB4] = v[r];
+        for (int r = 0; r < 16; ++r) out[j + r * NB4] = pack_f16(v[r], kF16Scale);
+      } else {
+        float2* out = a.X + static_cast<size_t>(xrow) * P;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) out[j + r * NB4] = v[r];
+      }
     }
   }
 }
@@ -609,6 +639,238 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
   }
 }
 
+
+// ---- FP16 form of the rows kernel ------------------------------------------------------------
+// Operands X and R are stored as f16 (re, im) pairs scaled by kF16Scale, conj(X) . R is formed
+// with packed f16 arithmetic, and every radix-R pass is a real GEMM on the matrix cores
+// (v_mfma_f32_16x16x32_f16, f32 accumulation):
+//   [Re Y_q; Im Y_q] = sum_r [[wr, -wi], [wi, wr]]_{qr} [Re v_r; Im v_r],  wr + i wi = W_R^{qr}
+// rows 2q / 2q+1, columns 2r / 2r+1, zero-padded to 16-row tiles and 32-column k-blocks:
+// radix 33 -> 5 x 3 fragments, radix 16 -> 2 x 1, radix 31 -> 4 x 2.  A B tile is 16
+// butterflies: lane l supplies butterfly l%16 and k-rows 8(l/16)..+7 of a k-block (inputs
+// r = 4(l/16)..+3), and receives D rows 4(l/16)..+3 of a tile (outputs q = 2(l/16), 2(l/16)+1
+// of each 8-output tile).  The Stockham twiddles are applied in f32 to the pass outputs before
+// they are rounded to f16 in LDS:
+//   pass A (33, NS 1): output q of butterfly j carries pass B's input twiddle W_528^{(j/31) q}
+//     and, moved through pass B's DFT by linearity, the part W_P^{a k} of pass C's twiddle
+//     that depends only on pass B's butterfly (a = j_B / 33, k = j_B mod 33): W_P^{q j} in all;
+//   pass B (16, NS 33): output q of butterfly j then needs W_496^{(j/33) q}.
+// Pass C (31, NS 528) keeps its outputs in registers: output q of butterfly j is sample
+// j + 528 q, and |.|^2 feeds the same top-2 sets as the FP32 form.
+struct Row16Args {
+  const uint32_t* X;    // [xrow][P] f16 pairs
+  const uint32_t* rep;  // [prn][2][P] f16 pairs
+  const float2* tw;
+  int n_bin;
+  int n_prn;
+  int groups;
+  int per_group;
+  int cw;
+  RowOut* rows;
+};
+
+// conj(x) * r on f16 pairs (two packed instructions)
+__device__ __forceinline__ uint32_t conjmul16(uint32_t xw, uint32_t rw) {
+  const half2_t x = __builtin_bit_cast(half2_t, xw);
+  const half2_t r = __builtin_bit_cast(half2_t, rw);
+  const half2_t a = half2_t{x.x, x.x} * r;
+  const half2_t b = half2_t{r.y, -r.x};
+  return __builtin_bit_cast(uint32_t, a + half2_t{x.y, x.y} * b);
+}
+
+// Fragment (tile t, k-block kb) of the real DFT matrix of radix R held by this lane.
+template <int R>
+__device__ __forceinline__ halfx8 dft_frag(const float2* stw, int t, int kb, int lane) {
+  const int rho = 16 * t + (lane & 15);
+  const int q = rho >> 1, po = rho & 1;
+  halfx8 f;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int kap = 32 * kb + 8 * (lane >> 4) + i;
+    const int r = kap >> 1, pi = kap & 1;
+    float v = 0.0f;
+    if (q < R && r < R) {
+      const float2 w = tw_lds(stw, ((q * r) % R) * (P / R));  // W_R^{qr}
+      v = po == 0 ? (pi == 0 ? w.x : -w.y) : (pi == 0 ? w.y : w.x);
+    }
+    f[i] = static_cast<_Float16>(v);
+  }
+  return f;
+}
+
+__device__ __forceinline__ float2 cmulf(float a, float b, float2 w) {
+  return make_float2(a * w.x - b * w.y, a * w.y + b * w.x);
+}
+
+__global__ __launch_bounds__(T) void k_rows16(Row16Args a) {
+  constexpr int NBA = P / 33;  // 496
+  constexpr int NBB = P / 16;  // 1023
+  constexpr int NBC = P / 31;  // 528
+  __shared__ uint32_t buf0[P];
+  __shared__ uint32_t buf1[P];
+  __shared__ float2 stw[256];
+  __shared__ float2 t16[31 * 16];  // W_496^{a q}, [a][q]
+  __shared__ float red_p[NW];
+  __shared__ int red_i[NW];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < 256; i += T) stw[i] = a.tw[i];
+  __syncthreads();
+  for (int i = tid; i < 31 * 16; i += T) t16[i] = tw_lds(stw, 33 * (i >> 4) * (i & 15));
+  const int wave = tid >> 6, lane = tid & 63, ln16 = lane & 15, grp = lane >> 4;
+  halfx8 fa[5][3], fb[2], fc[4][2];
+#pragma unroll
+  for (int t = 0; t < 5; ++t)
+#pragma unroll
+    for (int kb = 0; kb < 3; ++kb) fa[t][kb] = dft_frag<33>(stw, t, kb, lane);
+#pragma unroll
+  for (int t = 0; t < 2; ++t) fb[t] = dft_frag<16>(stw, t, 0, lane);
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) fc[t][kb] = dft_frag<31>(stw, t, kb, lane);
+  const int g = blockIdx.x % a.groups;
+  const int xrow = blockIdx.x / a.groups;
+  const int h = xrow & 1;
+  const int bin = (xrow >> 1) % a.n_bin;
+  const int ant = (xrow >> 1) / a.n_bin;
+  const int p0 = g * a.per_group;
+  const int p1 = min(p0 + a.per_group, a.n_prn);
+  const __amdgpu_buffer_rsrc_t xv = buffer_view(a.X + static_cast<size_t>(xrow) * P, P * 4);
+  __syncthreads();
+  for (int prn = p0; prn < p1; ++prn) {
+    const __amdgpu_buffer_rsrc_t rv =
+        buffer_view(a.rep + (static_cast<size_t>(prn) * 2 + h) * P, P * 4);
+    // ---- pass A: radix 33 from HBM, 31 tiles of 16 butterflies over the 8 waves
+#pragma unroll 1
+    for (int s = 0; s < 4; ++s) {
+      const int tile = wave + 8 * s;
+      if (tile < NBA / 16) {
+        const int j = 16 * tile + ln16;
+        const int vo = (j + NBA * 4 * grp) * 4;
+        halfx8 b[3];
+#pragma unroll
+        for (int kb = 0; kb < 3; ++kb) {
+          u32x4 w;
+#pragma unroll
+          for (int i2 = 0; i2 < 4; ++i2) {
+            // r = 16 kb + 4 grp + i2; r >= 33 lies past the row and loads 0
+            const int so = NBA * (16 * kb + i2) * 4;
+            if (kb < 2 || i2 == 0)
+              w[i2] = conjmul16(__builtin_amdgcn_raw_buffer_load_b32(xv, vo, so, 0),
+                                __builtin_amdgcn_raw_buffer_load_b32(rv, vo, so, 0));
+            else
+              w[i2] = 0u;
+          }
+          b[kb] = __builtin_bit_cast(halfx8, w);
+        }
+        floatx4 acc[5];
+#pragma unroll
+        for (int t = 0; t < 5; ++t) {
+          acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int kb = 0; kb < 3; ++kb)
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[t][kb], b[kb], acc[t], 0, 0, 0);
+        }
+        // twiddle W_P^{q j} for q = 8t + 2 grp (+1)
+        const float2 s1 = tw_lds(stw, j);
+        const float2 s8 = tw_lds(stw, 8 * j);
+        float2 w0 = tw_lds(stw, 2 * grp * j);
+        uint32_t* dst = buf0 + 33 * j + 2 * grp;
+#pragma unroll
+        for (int t = 0; t < 5; ++t) {
+          if (t > 0) w0 = cmul(w0, s8);
+          const float2 w1 = cmul(w0, s1);
+          if (t < 4 || grp == 0) dst[8 * t] = pack_f16(cmulf(acc[t][0], acc[t][1], w0), 1.0f);
+          if (t < 4) dst[8 * t + 1] = pack_f16(cmulf(acc[t][2], acc[t][3], w1), 1.0f);
+        }
+      }
+    }
+    lds_barrier();
+    // ---- pass B: radix 16, 64 tiles (the last holds 15 butterflies)
+#pragma unroll 2
+    for (int s = 0; s < 8; ++s) {
+      const int j = 16 * (wave + 8 * s) + ln16;
+      const bool ok = j < NBB;
+      u32x4 w;
+#pragma unroll
+      for (int i2 = 0; i2 < 4; ++i2) w[i2] = ok ? buf0[j + NBB * (4 * grp + i2)] : 0u;
+      const halfx8 b = __builtin_bit_cast(halfx8, w);
+      floatx4 acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(fb[0], b, floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+      floatx4 acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(fb[1], b, floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+      if (ok) {
+        const int aa = j / 33;
+        const int k = j - 33 * aa;
+        uint32_t* dst = buf1 + aa * NBC + k + 66 * grp;
+        const float2* tq = t16 + aa * 16 + 2 * grp;
+        dst[0] = pack_f16(cmulf(acc0[0], acc0[1], tq[0]), 1.0f);
+        dst[33] = pack_f16(cmulf(acc0[2], acc0[3], tq[1]), 1.0f);
+        dst[264] = pack_f16(cmulf(acc1[0], acc1[1], tq[8]), 1.0f);
+        dst[297] = pack_f16(cmulf(acc1[2], acc1[3], tq[9]), 1.0f);
+      }
+    }
+    lds_barrier();
+    // ---- pass C: radix 31, 33 tiles (wave 0 takes tile 32 too), outputs to the top-2 sets
+    Top2 main_set, extra_set;
+    float accn = 0.0f;  // sum of powers: NaN / overflow detector
+    const int ntile = wave == 0 ? 5 : 4;
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+      if (s < ntile) {
+        const int j = 16 * (wave + 8 * s) + ln16;
+        halfx8 b[2];
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb) {
+          u32x4 w;
+#pragma unroll
+          for (int i2 = 0; i2 < 4; ++i2) {
+            const int r = 16 * kb + 4 * grp + i2;
+            w[i2] = r < 31 ? buf1[j + NBC * r] : 0u;
+          }
+          b[kb] = __builtin_bit_cast(halfx8, w);
+        }
+        Top2& set = s < 4 ? main_set : extra_set;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(fc[t][0], b[0], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(fc[t][1], b[1], acc, 0, 0, 0);
+          const int q = 8 * t + 2 * grp;
+          const float pa = acc[0] * acc[0] + acc[1] * acc[1];
+          accn += pa;
+          set.add(pa, j + NBC * q);
+          if (t < 3 || grp < 3) {
+            const float pb = acc[2] * acc[2] + acc[3] * acc[3];
+            accn += pb;
+            set.add(pb, j + NBC * (q + 1));
+          }
+        }
+      }
+    }
+    float bp = main_set.best();
+    int bi = main_set.index();
+    arg_combine(bp, bi, extra_set.best(), extra_set.index());
+    block_argmax<T>(bp, bi, red_p, red_i);
+    const float m2 = block_max<T>(fmaxf(main_set.masked(bi, a.cw), extra_set.masked(bi, a.cw)),
+                                  red_p);
+    const int bad = __syncthreads_or(!(accn <= 3.0e38f));
+    if (tid == 0) {
+      RowOut o;
+      o.p1 = bp;
+      o.p2 = m2;
+      o.i1 = bi;
+      o.nan = bad;
+      a.rows[((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) * 2 + h] = o;
+    }
+  }
+}
+
+// f32 pairs -> f16 pairs scaled by s
+__global__ void k_to_f16(const float2* __restrict__ in, uint32_t* __restrict__ out, long long n,
+                         float s) {
+  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (i < n) out[i] = pack_f16(in[i], s);
+}
+
 }  // namespace fused
 
 // ================================ generic path ===========================================
@@ -784,8 +1046,8 @@ std::vector<float2> fused_twiddles() {
 void launch_fused_forward(const uint8_t* packed, size_t row_bytes, int fmt, const double* means,
                           const double* fc, int n_ant, int n_bin, double phase_const,
                           const float2* tau, const float2* tw, float2* X, hipStream_t st) {
-  fused::FwdArgs a{packed, row_bytes, fmt, means, fc, n_bin, phase_const, tau, tw, X};
-  hipLaunchKernelGGL(fused::k_forward, dim3(n_ant * n_bin * 2), dim3(fused::T), 0, st, a);
+  fused::FwdArgs a{packed, row_bytes, fmt, means, fc, n_bin, phase_const, tau, tw, X, nullptr};
+  hipLaunchKernelGGL(fused::k_forward<false>, dim3(n_ant * n_bin * 2), dim3(fused::T), 0, st, a);
   TGA_HIP(hipGetLastError());
 }
 
@@ -802,6 +1064,33 @@ void launch_fused_rows(const float2* X, const float2* rep, const float2* tw, int
   else
     hipLaunchKernelGGL(fused::k_rows<1>, dim3(static_cast<unsigned>(blocks)), dim3(fused::T), 0,
                        st, a);
+  TGA_HIP(hipGetLastError());
+}
+
+void launch_fused_forward16(const uint8_t* packed, size_t row_bytes, int fmt,
+                            const double* means, const double* fc, int n_ant, int n_bin,
+                            double phase_const, const float2* tau, const float2* tw,
+                            uint32_t* X16, hipStream_t st) {
+  fused::FwdArgs a{packed, row_bytes, fmt, means, fc, n_bin, phase_const, tau, tw, nullptr, X16};
+  hipLaunchKernelGGL(fused::k_forward<true>, dim3(n_ant * n_bin * 2), dim3(fused::T), 0, st, a);
+  TGA_HIP(hipGetLastError());
+}
+
+void launch_fused_rows16(const uint32_t* X16, const uint32_t* rep16, const float2* tw,
+                         int n_ant, int n_bin, int n_prn, int cw, RowOut* rows, int groups,
+                         hipStream_t st) {
+  groups = n_prn < groups ? n_prn : groups;
+  const int per = (n_prn + groups - 1) / groups;
+  fused::Row16Args a{X16, rep16, tw, n_bin, n_prn, groups, per, cw, rows};
+  const long long blocks = static_cast<long long>(n_ant) * n_bin * 2 * groups;
+  hipLaunchKernelGGL(fused::k_rows16, dim3(static_cast<unsigned>(blocks)), dim3(fused::T), 0, st,
+                     a);
+  TGA_HIP(hipGetLastError());
+}
+
+void launch_to_f16(const float2* in, uint32_t* out, long long n, float scale, hipStream_t st) {
+  hipLaunchKernelGGL(fused::k_to_f16, dim3(grid_for(n, 256)), dim3(256), 0, st, in, out, n,
+                     scale);
   TGA_HIP(hipGetLastError());
 }
 

@@ -16,6 +16,7 @@ SYS_GPS, SYS_GALILEO, SYS_BEIDOU, SYS_SBAS, SYS_L1C, SYS_QZSS = range(6)
 SYSTEM_IDS = {"gps": SYS_GPS, "galileo": SYS_GALILEO, "beidou": SYS_BEIDOU, "sbas": SYS_SBAS,
               "l1c": SYS_L1C, "qzss": SYS_QZSS}
 INPUT_1BIT, INPUT_2BIT = 1, 2
+PRECISIONS = {"fp32": 0, "fp16": 1}
 ACR_TABLES = {"GPS_L1_CA_ACR_TABLE": 0, "GALILEO_E1C_ACR_TABLE": 1,
               "BEIDOU_B1C_ACR_TABLE": 2, "GPS_L1C_ACR_TABLE": 3}
 
@@ -50,7 +51,8 @@ class SystemDesc(C.Structure):
 class PlanDesc(C.Structure):
     _fields_ = [("sampling_freq", C.c_double), ("n_ant", C.c_int32),
                 ("input_format", C.c_int32), ("record_samples", C.c_int64),
-                ("n_systems", C.c_int32), ("systems", C.POINTER(SystemDesc))]
+                ("n_systems", C.c_int32), ("precision", C.c_int32),
+                ("systems", C.POINTER(SystemDesc))]
 
 
 class RfiLine(C.Structure):

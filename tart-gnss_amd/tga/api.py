@@ -201,9 +201,12 @@ PEAK_DTYPE = np.dtype([("best_peak", np.float32), ("second_peak", np.float32),
 
 
 class Plan:
-    """Batched search plan (tga_plan_*): one observation of n_ant packed rows per search."""
+    """Batched search plan (tga_plan_*): one observation of n_ant packed rows per search.
+    ``precision`` "fp32" is the reference form; "fp16" the f16-storage matrix-core form of the
+    fused P = 16368 path (other groups stay fp32)."""
 
-    def __init__(self, sampling_freq, n_ant, record_samples, systems, input_format=1):
+    def __init__(self, sampling_freq, n_ant, record_samples, systems, input_format=1,
+                 precision="fp32"):
         self._keep = []
         descs = (L.SystemDesc * len(systems))()
         for i, s in enumerate(systems):
@@ -227,6 +230,7 @@ class Plan:
         d.input_format = int(input_format)
         d.record_samples = int(record_samples)
         d.n_systems = len(systems)
+        d.precision = L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         d.systems = descs
         h = C.c_void_p()
         check(lib().tga_plan_create(C.byref(d), C.byref(h)))
