@@ -362,6 +362,65 @@ struct Top2 {
   }
 };
 
+// LDS scratch of one row's reduction: per-wave argmax, masked maximum and bad-value flag.
+template <int NW_>
+struct RowRed {
+  float p[NW_];
+  int i[NW_];
+  float m[NW_];
+  int bad[NW_];
+};
+
+// End of one correlation row: block argmax (last maximum), the masked second maximum around
+// it, and the NaN / overflow flag, with two workgroup barriers.  Safe to call once per row in
+// a loop: every LDS slot is read before the barrier that precedes its next write.
+template <int NT>
+__device__ __forceinline__ void row_finish(const Top2& main_set, const Top2& extra_set,
+                                           bool bad, int cw, RowRed<NT / 64>& red, RowOut* out) {
+  constexpr int NWR = NT / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float bp = main_set.best();
+  int bi = main_set.index();
+  arg_combine(bp, bi, extra_set.best(), extra_set.index());
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const float q = __shfl_xor(bp, off, 64);
+    const int j = __shfl_xor(bi, off, 64);
+    arg_combine(bp, bi, q, j);
+  }
+  const bool wave_bad = __ballot(bad) != 0;
+  if (lane == 0) {
+    red.p[wave] = bp;
+    red.i[wave] = bi;
+    red.bad[wave] = wave_bad ? 1 : 0;
+  }
+  lds_barrier();
+  bp = red.p[0];
+  bi = red.i[0];
+  int any_bad = red.bad[0];
+#pragma unroll
+  for (int w = 1; w < NWR; ++w) {
+    arg_combine(bp, bi, red.p[w], red.i[w]);
+    any_bad |= red.bad[w];
+  }
+  float m = fmaxf(main_set.masked(bi, cw), extra_set.masked(bi, cw));
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+  if (lane == 0) red.m[wave] = m;
+  lds_barrier();
+  if (threadIdx.x == 0) {
+    float m2 = red.m[0];
+#pragma unroll
+    for (int w = 1; w < NWR; ++w) m2 = fmaxf(m2, red.m[w]);
+    RowOut o;
+    o.p1 = bp;
+    o.p2 = m2;
+    o.i1 = bi;
+    o.nan = any_bad;
+    *out = o;
+  }
+}
+
 // Correlation rows.  Per (antenna, bin, hypothesis) block and per PRN of its group:
 //   pass A  radix 33 (NS = 1), 496 butterflies, operands conj(X) . R gathered straight from
 //           global memory (buffer loads) into registers
@@ -389,8 +448,7 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
   __shared__ float2 stw[256];
   __shared__ float2 tb[16 * 33];  // W_528^{r k}, [r][k]
   __shared__ float2 tx[VARIANT == 0 ? 31 * LEFT + 31 : 1023];
-  __shared__ float red_p[NW];
-  __shared__ int red_i[NW];
+  __shared__ RowRed<NW> red;
   const int tid = threadIdx.x;
   for (int i = tid; i < 256; i += T) stw[i] = a.tw[i];
   __syncthreads();
@@ -621,21 +679,8 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
         }
       }
     }
-    float bp = main_set.best();
-    int bi = main_set.index();
-    arg_combine(bp, bi, extra_set.best(), extra_set.index());
-    block_argmax<T>(bp, bi, red_p, red_i);
-    const float m2 = block_max<T>(fmaxf(main_set.masked(bi, a.cw), extra_set.masked(bi, a.cw)),
-                                  red_p);
-    const int nan = __syncthreads_or(accn != accn);
-    if (tid == 0) {
-      RowOut o;
-      o.p1 = bp;
-      o.p2 = m2;
-      o.i1 = bi;
-      o.nan = nan;
-      a.rows[((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) * 2 + h] = o;
-    }
+    row_finish<T>(main_set, extra_set, accn != accn, a.cw, red,
+                  a.rows + ((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) * 2 + h);
   }
 }
 
@@ -702,32 +747,56 @@ __device__ __forceinline__ float2 cmulf(float a, float b, float2 w) {
   return make_float2(a * w.x - b * w.y, a * w.y + b * w.x);
 }
 
-__global__ __launch_bounds__(T) void k_rows16(Row16Args a) {
+// Raw operand words of one pass-A tile for this lane: X and R at j + 496 r for the lane's
+// inputs r = 16 kb + 4 grp + i2 (kb < 2) and r = 32 (kb = 2, i2 = 0); r >= 33 lies past the
+// row, where buffer loads return 0.
+struct TileA {
+  uint32_t x[9], r[9];
+};
+
+__device__ __forceinline__ void load_tile_a(TileA& d, __amdgpu_buffer_rsrc_t xv,
+                                            __amdgpu_buffer_rsrc_t rv, int vo) {
+  constexpr int NBA = P / 33;
+#pragma unroll
+  for (int u = 0; u < 9; ++u) {
+    const int so = NBA * (16 * (u >> 2) + (u & 3)) * 4;
+    d.x[u] = __builtin_amdgcn_raw_buffer_load_b32(xv, vo, so, 0);
+    d.r[u] = __builtin_amdgcn_raw_buffer_load_b32(rv, vo, so, 0);
+  }
+}
+
+// 1024 threads (16 waves, four per SIMD): the matrix-core form keeps few values per lane, so
+// more waves can hide the LDS and matrix-core latencies.
+constexpr int T16 = 1024;
+constexpr int NW16 = T16 / 64;
+
+__global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
   constexpr int NBA = P / 33;  // 496
   constexpr int NBB = P / 16;  // 1023
   constexpr int NBC = P / 31;  // 528
-  __shared__ uint32_t buf0[P];
-  __shared__ uint32_t buf1[P];
+  // LDS rows are padded so the four 16-lane groups of a wave, which read inputs r 4 apart,
+  // start 16 banks apart: pass B input r of butterfly j at j + SB r, pass C at j + SC r.
+  constexpr int SB = NBB + 5;  // 1028
+  constexpr int SC = NBC + 4;  // 532
+  __shared__ uint32_t buf0[16 * SB];
+  __shared__ uint32_t buf1[31 * SC];
+  __shared__ u32x4 fa_lds[15 * 64];  // radix-33 fragments [t*3 + kb][lane]
   __shared__ float2 stw[256];
   __shared__ float2 t16[31 * 16];  // W_496^{a q}, [a][q]
-  __shared__ float red_p[NW];
-  __shared__ int red_i[NW];
+  __shared__ RowRed<NW16> red;
   const int tid = threadIdx.x;
-  for (int i = tid; i < 256; i += T) stw[i] = a.tw[i];
+  for (int i = tid; i < 256; i += T16) stw[i] = a.tw[i];
   __syncthreads();
-  for (int i = tid; i < 31 * 16; i += T) t16[i] = tw_lds(stw, 33 * (i >> 4) * (i & 15));
-  const int wave = tid >> 6, lane = tid & 63, ln16 = lane & 15, grp = lane >> 4;
-  halfx8 fa[5][3], fb[2], fc[4][2];
+  for (int i = tid; i < 31 * 16; i += T16) t16[i] = tw_lds(stw, 33 * (i >> 4) * (i & 15));
+  for (int i = tid; i < 15 * 64; i += T16)
+    fa_lds[i] = __builtin_bit_cast(u32x4, dft_frag<33>(stw, (i >> 6) / 3, (i >> 6) % 3, i & 63));
+  halfx8 fb[2], fc[4][2];
 #pragma unroll
-  for (int t = 0; t < 5; ++t)
-#pragma unroll
-    for (int kb = 0; kb < 3; ++kb) fa[t][kb] = dft_frag<33>(stw, t, kb, lane);
-#pragma unroll
-  for (int t = 0; t < 2; ++t) fb[t] = dft_frag<16>(stw, t, 0, lane);
+  for (int t = 0; t < 2; ++t) fb[t] = dft_frag<16>(stw, t, 0, tid & 63);
 #pragma unroll
   for (int t = 0; t < 4; ++t)
 #pragma unroll
-    for (int kb = 0; kb < 2; ++kb) fc[t][kb] = dft_frag<31>(stw, t, kb, lane);
+    for (int kb = 0; kb < 2; ++kb) fc[t][kb] = dft_frag<31>(stw, t, kb, tid & 63);
   const int g = blockIdx.x % a.groups;
   const int xrow = blockIdx.x / a.groups;
   const int h = xrow & 1;
@@ -740,67 +809,70 @@ __global__ __launch_bounds__(T) void k_rows16(Row16Args a) {
   for (int prn = p0; prn < p1; ++prn) {
     const __amdgpu_buffer_rsrc_t rv =
         buffer_view(a.rep + (static_cast<size_t>(prn) * 2 + h) * P, P * 4);
-    // ---- pass A: radix 33 from HBM, 31 tiles of 16 butterflies over the 8 waves
-#pragma unroll 1
-    for (int s = 0; s < 4; ++s) {
-      const int tile = wave + 8 * s;
+    // Lane coordinates are re-derived per PRN through an opaque copy: otherwise the compiler
+    // hoists every lane-dependent twiddle and address out of the PRN loop and spills them.
+    int tid2 = tid;
+    asm volatile("" : "+v"(tid2));
+    const int wave = tid2 >> 6, lane = tid2 & 63, ln16 = lane & 15, grp = lane >> 4;
+    // ---- pass A: radix 33 from HBM, 31 tiles of 16 butterflies: tiles wave and wave + 16
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int tile = wave + 16 * s;
       if (tile < NBA / 16) {
         const int j = 16 * tile + ln16;
-        const int vo = (j + NBA * 4 * grp) * 4;
+        TileA cur;
+        load_tile_a(cur, xv, rv, (j + NBA * 4 * grp) * 4);
         halfx8 b[3];
 #pragma unroll
         for (int kb = 0; kb < 3; ++kb) {
           u32x4 w;
 #pragma unroll
           for (int i2 = 0; i2 < 4; ++i2) {
-            // r = 16 kb + 4 grp + i2; r >= 33 lies past the row and loads 0
-            const int so = NBA * (16 * kb + i2) * 4;
-            if (kb < 2 || i2 == 0)
-              w[i2] = conjmul16(__builtin_amdgcn_raw_buffer_load_b32(xv, vo, so, 0),
-                                __builtin_amdgcn_raw_buffer_load_b32(rv, vo, so, 0));
-            else
-              w[i2] = 0u;
+            const int u = 4 * kb + i2;
+            w[i2] = u < 9 ? conjmul16(cur.x[u], cur.r[u]) : 0u;
           }
           b[kb] = __builtin_bit_cast(halfx8, w);
         }
-        floatx4 acc[5];
-#pragma unroll
-        for (int t = 0; t < 5; ++t) {
-          acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-          for (int kb = 0; kb < 3; ++kb)
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[t][kb], b[kb], acc[t], 0, 0, 0);
-        }
-        // twiddle W_P^{q j} for q = 8t + 2 grp (+1)
+        // twiddle W_P^{q j} for q = 8t + 2 grp (+1); output q of butterfly j is pass-B
+        // input r = j / 31 of butterfly 33 (j mod 31) + q
         const float2 s1 = tw_lds(stw, j);
         const float2 s8 = tw_lds(stw, 8 * j);
         float2 w0 = tw_lds(stw, 2 * grp * j);
-        uint32_t* dst = buf0 + 33 * j + 2 * grp;
+        const int jd = j / 31;
+        uint32_t* dst = buf0 + 33 * (j - 31 * jd) + SB * jd + 2 * grp;
 #pragma unroll
         for (int t = 0; t < 5; ++t) {
+          floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int kb = 0; kb < 3; ++kb)
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                __builtin_bit_cast(halfx8, fa_lds[(t * 3 + kb) * 64 + lane]), b[kb], acc, 0, 0,
+                0);
           if (t > 0) w0 = cmul(w0, s8);
           const float2 w1 = cmul(w0, s1);
-          if (t < 4 || grp == 0) dst[8 * t] = pack_f16(cmulf(acc[t][0], acc[t][1], w0), 1.0f);
-          if (t < 4) dst[8 * t + 1] = pack_f16(cmulf(acc[t][2], acc[t][3], w1), 1.0f);
+          if (t < 4 || grp == 0) dst[8 * t] = pack_f16(cmulf(acc[0], acc[1], w0), 1.0f);
+          if (t < 4) dst[8 * t + 1] = pack_f16(cmulf(acc[2], acc[3], w1), 1.0f);
         }
       }
     }
     lds_barrier();
-    // ---- pass B: radix 16, 64 tiles (the last holds 15 butterflies)
-#pragma unroll 2
-    for (int s = 0; s < 8; ++s) {
-      const int j = 16 * (wave + 8 * s) + ln16;
+    // ---- pass B: radix 16, 64 tiles (the last holds 15 butterflies): wave + 16 s
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int j = 16 * (wave + 16 * s) + ln16;
       const bool ok = j < NBB;
       u32x4 w;
 #pragma unroll
-      for (int i2 = 0; i2 < 4; ++i2) w[i2] = ok ? buf0[j + NBB * (4 * grp + i2)] : 0u;
+      for (int i2 = 0; i2 < 4; ++i2) w[i2] = ok ? buf0[j + SB * (4 * grp + i2)] : 0u;
       const halfx8 b = __builtin_bit_cast(halfx8, w);
       floatx4 acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(fb[0], b, floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
       floatx4 acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(fb[1], b, floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
       if (ok) {
+        // output q of butterfly j (a = j / 33, k = j mod 33) is pass-C input r = a of
+        // butterfly k + 33 q
         const int aa = j / 33;
         const int k = j - 33 * aa;
-        uint32_t* dst = buf1 + aa * NBC + k + 66 * grp;
+        uint32_t* dst = buf1 + aa * SC + k + 66 * grp;
         const float2* tq = t16 + aa * 16 + 2 * grp;
         dst[0] = pack_f16(cmulf(acc0[0], acc0[1], tq[0]), 1.0f);
         dst[33] = pack_f16(cmulf(acc0[2], acc0[3], tq[1]), 1.0f);
@@ -809,14 +881,14 @@ __global__ __launch_bounds__(T) void k_rows16(Row16Args a) {
       }
     }
     lds_barrier();
-    // ---- pass C: radix 31, 33 tiles (wave 0 takes tile 32 too), outputs to the top-2 sets
+    // ---- pass C: radix 31, 33 tiles: tiles wave and wave + 16 on every wave, and the four
+    // output tiles t of tile 32 on waves 0-3 (wave t takes output tile t)
     Top2 main_set, extra_set;
     float accn = 0.0f;  // sum of powers: NaN / overflow detector
-    const int ntile = wave == 0 ? 5 : 4;
 #pragma unroll
-    for (int s = 0; s < 5; ++s) {
-      if (s < ntile) {
-        const int j = 16 * (wave + 8 * s) + ln16;
+    for (int s = 0; s < 3; ++s) {
+      if (s < 2 || wave < 4) {
+        const int j = 16 * (s < 2 ? wave + 16 * s : 32) + ln16;
         halfx8 b[2];
 #pragma unroll
         for (int kb = 0; kb < 2; ++kb) {
@@ -824,43 +896,32 @@ __global__ __launch_bounds__(T) void k_rows16(Row16Args a) {
 #pragma unroll
           for (int i2 = 0; i2 < 4; ++i2) {
             const int r = 16 * kb + 4 * grp + i2;
-            w[i2] = r < 31 ? buf1[j + NBC * r] : 0u;
+            w[i2] = r < 31 ? buf1[j + SC * r] : 0u;
           }
           b[kb] = __builtin_bit_cast(halfx8, w);
         }
-        Top2& set = s < 4 ? main_set : extra_set;
+        Top2& set = s < 2 ? main_set : extra_set;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-          floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(fc[t][0], b[0], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(fc[t][1], b[1], acc, 0, 0, 0);
-          const int q = 8 * t + 2 * grp;
-          const float pa = acc[0] * acc[0] + acc[1] * acc[1];
-          accn += pa;
-          set.add(pa, j + NBC * q);
-          if (t < 3 || grp < 3) {
-            const float pb = acc[2] * acc[2] + acc[3] * acc[3];
-            accn += pb;
-            set.add(pb, j + NBC * (q + 1));
+          if (s < 2 || t == wave) {
+            floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(fc[t][0], b[0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(fc[t][1], b[1], acc, 0, 0, 0);
+            const int q = 8 * t + 2 * grp;
+            const float pa = acc[0] * acc[0] + acc[1] * acc[1];
+            accn += pa;
+            set.add(pa, j + NBC * q);
+            if (t < 3 || grp < 3) {
+              const float pb = acc[2] * acc[2] + acc[3] * acc[3];
+              accn += pb;
+              set.add(pb, j + NBC * (q + 1));
+            }
           }
         }
       }
     }
-    float bp = main_set.best();
-    int bi = main_set.index();
-    arg_combine(bp, bi, extra_set.best(), extra_set.index());
-    block_argmax<T>(bp, bi, red_p, red_i);
-    const float m2 = block_max<T>(fmaxf(main_set.masked(bi, a.cw), extra_set.masked(bi, a.cw)),
-                                  red_p);
-    const int bad = __syncthreads_or(!(accn <= 3.0e38f));
-    if (tid == 0) {
-      RowOut o;
-      o.p1 = bp;
-      o.p2 = m2;
-      o.i1 = bi;
-      o.nan = bad;
-      a.rows[((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) * 2 + h] = o;
-    }
+    row_finish<T16>(main_set, extra_set, !(accn <= 3.0e38f), a.cw, red,
+                    a.rows + ((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) * 2 + h);
   }
 }
 
@@ -1083,8 +1144,8 @@ void launch_fused_rows16(const uint32_t* X16, const uint32_t* rep16, const float
   const int per = (n_prn + groups - 1) / groups;
   fused::Row16Args a{X16, rep16, tw, n_bin, n_prn, groups, per, cw, rows};
   const long long blocks = static_cast<long long>(n_ant) * n_bin * 2 * groups;
-  hipLaunchKernelGGL(fused::k_rows16, dim3(static_cast<unsigned>(blocks)), dim3(fused::T), 0, st,
-                     a);
+  hipLaunchKernelGGL(fused::k_rows16, dim3(static_cast<unsigned>(blocks)), dim3(fused::T16), 0,
+                     st, a);
   TGA_HIP(hipGetLastError());
 }
 
