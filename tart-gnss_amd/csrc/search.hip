@@ -352,6 +352,23 @@ struct Top2 {
   __device__ __forceinline__ float best() const {
     return any ? __uint_as_float(static_cast<unsigned int>(key >> 32)) : -1.0f;
   }
+  // a set holding one (power, index) pair
+  __device__ __forceinline__ static Top2 leaf(float p, int idx) {
+    Top2 t;
+    t.key = pkey(p, idx);
+    t.any = true;
+    return t;
+  }
+  // union of two sets whose members are pairwise more than 2 cw apart
+  __device__ __forceinline__ static Top2 merge(const Top2& a, const Top2& b) {
+    const bool up = b.key > a.key;
+    Top2 t;
+    t.key = up ? b.key : a.key;
+    const float lose = __uint_as_float(static_cast<unsigned int>((up ? a.key : b.key) >> 32));
+    t.second = fmaxf(fmaxf(a.second, b.second), lose);
+    t.any = true;
+    return t;
+  }
   __device__ __forceinline__ int index() const {
     return any ? static_cast<int>(key & 0xffffffffu) : -1;
   }
@@ -882,14 +899,13 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
     }
     lds_barrier();
     // ---- pass C: radix 31, 33 tiles: tiles wave and wave + 16 on every wave, and the four
-    // output tiles t of tile 32 on waves 0-3 (wave t takes output tile t)
+    // output tiles t of tile 32 on waves 0-3 (wave t takes output tile t).  All products are
+    // issued before the first power is used, and the 16 powers of the two main tiles are
+    // merged as a tree, so neither the matrix-core latency nor the top-2 chain serialises.
     Top2 main_set, extra_set;
     float accn = 0.0f;  // sum of powers: NaN / overflow detector
-#pragma unroll
-    for (int s = 0; s < 3; ++s) {
-      if (s < 2 || wave < 4) {
-        const int j = 16 * (s < 2 ? wave + 16 * s : 32) + ln16;
-        halfx8 b[2];
+    {
+      auto load_c = [&](int j, halfx8* b) {
 #pragma unroll
         for (int kb = 0; kb < 2; ++kb) {
           u32x4 w;
@@ -900,23 +916,64 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
           }
           b[kb] = __builtin_bit_cast(halfx8, w);
         }
-        Top2& set = s < 2 ? main_set : extra_set;
+      };
+      const int j0 = 16 * wave + ln16, j1 = j0 + 256;
+      halfx8 b0[2], b1[2];
+      load_c(j0, b0);
+      load_c(j1, b1);
+      floatx4 c0[4], c1[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          if (s < 2 || t == wave) {
-            floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(fc[t][0], b[0], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(fc[t][1], b[1], acc, 0, 0, 0);
-            const int q = 8 * t + 2 * grp;
-            const float pa = acc[0] * acc[0] + acc[1] * acc[1];
-            accn += pa;
-            set.add(pa, j + NBC * q);
-            if (t < 3 || grp < 3) {
-              const float pb = acc[2] * acc[2] + acc[3] * acc[3];
-              accn += pb;
-              set.add(pb, j + NBC * (q + 1));
-            }
+      for (int t = 0; t < 4; ++t) {
+        c0[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fc[t][0], b0[0], floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        c1[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fc[t][0], b1[0], floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        c0[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fc[t][1], b0[1], c0[t], 0, 0, 0);
+        c1[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fc[t][1], b1[1], c1[t], 0, 0, 0);
+      }
+      Top2 lv[16];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int q = 8 * t + 2 * grp;
+        const float pa0 = c0[t][0] * c0[t][0] + c0[t][1] * c0[t][1];
+        const float pb0 = c0[t][2] * c0[t][2] + c0[t][3] * c0[t][3];
+        const float pa1 = c1[t][0] * c1[t][0] + c1[t][1] * c1[t][1];
+        const float pb1 = c1[t][2] * c1[t][2] + c1[t][3] * c1[t][3];
+        const bool okb = t < 3 || grp < 3;  // output q + 1 = 31 does not exist
+        accn += (pa0 + pa1) + (okb ? pb0 + pb1 : 0.0f);
+        lv[4 * t + 0] = Top2::leaf(pa0, j0 + NBC * q);
+        lv[4 * t + 1] = Top2::leaf(pa1, j1 + NBC * q);
+        lv[4 * t + 2] = okb ? Top2::leaf(pb0, j0 + NBC * (q + 1)) : Top2();
+        lv[4 * t + 3] = okb ? Top2::leaf(pb1, j1 + NBC * (q + 1)) : Top2();
+      }
+#pragma unroll
+      for (int w = 8; w >= 1; w >>= 1)
+#pragma unroll
+        for (int i = 0; i < w; ++i) lv[i] = Top2::merge(lv[i], lv[i + w]);
+      main_set = lv[0];
+      if (wave < 4) {
+        const int j = 512 + ln16;
+        halfx8 b[2];
+        load_c(j, b);
+        // output tile t = wave: select its fragments without dynamic register indexing
+        halfx8 f0 = fc[0][0], f1 = fc[0][1];
+#pragma unroll
+        for (int t = 1; t < 4; ++t)
+          if (wave == t) {
+            f0 = fc[t][0];
+            f1 = fc[t][1];
           }
+        floatx4 acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(f0, b[0], floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(f1, b[1], acc, 0, 0, 0);
+        const int q = 8 * wave + 2 * grp;
+        const float pa = acc[0] * acc[0] + acc[1] * acc[1];
+        accn += pa;
+        extra_set.add(pa, j + NBC * q);
+        if (wave < 3 || grp < 3) {
+          const float pb = acc[2] * acc[2] + acc[3] * acc[3];
+          accn += pb;
+          extra_set.add(pb, j + NBC * (q + 1));
         }
       }
     }
