@@ -1,16 +1,23 @@
 """Benchmark: batched code-phase/Doppler search throughput on one MI355X.
 
-Workload (default): GPS + SBAS + QZSS L1 C/A, 24 antennas, 2-bit real samples at
-16.368 MHz, 10 ms records (163680 samples); seeded signals: 8 GPS, 2 SBAS and 1 QZSS at
-C/N0 uniform in [38, 50] dB-Hz with per-SV Doppler and per-antenna carrier offsets from a
-3 m disk; search 38 + 39 + 23 PRNs x 41 Doppler bins (+-10 kHz, 500 Hz step), N = 2P
-(reference form, two hypotheses), FP32, plus ACR C/N0 on the host.  The FP16 matrix-core
-form (f16 spectra, radix passes as f16 GEMMs, f64 carrier phase) then runs the same
-observation and is reported under extra.fp16_mfma with its difference to the FP32 peaks.
+Default workload (``--workload ca24``): GPS + SBAS + QZSS L1 C/A, 24 antennas, 2-bit real
+samples at 16.368 MHz, 10 ms records (163680 samples); seeded signals: 8 GPS, 2 SBAS and 1
+QZSS at C/N0 uniform in [38, 50] dB-Hz with per-SV Doppler and per-antenna carrier offsets
+from a 3 m disk; search 38 + 39 + 23 PRNs x 41 Doppler bins (+-10 kHz, 500 Hz step), N = 2P
+(reference form, two hypotheses), FP32, plus ACR C/N0 on the host.  The FP16 matrix-core form
+(f16 spectra, radix passes as f16 GEMMs, f64 carrier phase) then runs the same observation and
+is reported under extra.fp16_mfma with its difference to the FP32 peaks.
+
+Other workloads (same JSON line):
+  ca1      GPS L1 C/A, 1 antenna, 1-bit, 10 ms: 4 signals at 45 dB-Hz, PRN 1-32 x 21 bins
+           (+-5 kHz, 500 Hz).
+  pilots24 Galileo E1-C (P = 65472), BeiDou B1C and GPS L1C (P = 163680), 24 antennas, 1-bit,
+           40 ms: 6 signals per system at 40-48 dB-Hz; E1-C 50 SVs x 81 bins (125 Hz), B1C and
+           L1C 63 SVs x 201 bins (50 Hz), N = 2P.
 
 A step = one tga_search_device call on an observation already resident in HBM: decode +
-de-mean, forward spectra, all correlation rows, selection, copy of the 2400 peaks to the
-host, and the f64 ACR lookups.  Metric: search cells/s = sum(n_ant * n_prn * n_bin * P) / t.
+de-mean, forward spectra, all correlation rows, selection, copy of the peaks to the host, and
+the f64 ACR lookups.  Metric: search cells/s = sum(n_ant * n_prn * n_bin * P) / t.
 
 Prints one JSON line (see the repository's measurement contract).
 """
@@ -29,20 +36,61 @@ METRIC = "search cells/s (ant×PRN×Doppler×phase): FP16-MFMA vs FP32 GPU vs CP
 FS = 16.368e6
 IF = 4.092e6
 HBM_PEAK_GBS = 8000.0
+PERIOD_S = {"gps": 0.001, "sbas": 0.001, "qzss": 0.001, "galileo": 0.004, "beidou": 0.010,
+            "l1c": 0.010}
+PRNS = {"gps": range(1, 39), "sbas": range(120, 159), "qzss": range(184, 207),
+        "galileo": range(1, 51), "beidou": range(1, 64), "l1c": range(1, 64)}
 
 
-def config2_signals(rng):
-    sv = []
-    gps = rng.choice(np.arange(1, 33), 8, replace=False)
-    sbas = rng.choice(np.arange(120, 159), 2, replace=False)
-    qz = rng.choice(np.arange(184, 207), 1, replace=False)
-    for sysname, prns in (("gps", gps), ("sbas", sbas), ("qzss", qz)):
-        for prn in prns:
-            sv.append(dict(system=sysname, prn=int(prn), cn0=float(rng.uniform(38, 50)),
-                           doppler=float(rng.uniform(-9000, 9000)),
-                           delay=int(rng.integers(0, 16368)), phase=float(rng.uniform(0, 6.28)),
-                           az=float(rng.uniform(0, 360)), el=float(rng.uniform(15, 90))))
-    return sv
+def grid(half, step):
+    n = int(round(2 * half / step)) + 1
+    return [IF - half + step * i for i in range(n)]
+
+
+def sv(rng, system, prn, cn0_lo, cn0_hi, dop):
+    P = int(FS * PERIOD_S[system])
+    return dict(system=system, prn=int(prn), cn0=float(rng.uniform(cn0_lo, cn0_hi)),
+                doppler=float(rng.uniform(-dop, dop)), delay=int(rng.integers(0, P)),
+                phase=float(rng.uniform(0, 6.28)), az=float(rng.uniform(0, 360)),
+                el=float(rng.uniform(15, 90)))
+
+
+def workload(name, rng):
+    """-> dict(desc, n_ant, L, fmt, signals, systems=[(system, prns, fc)])."""
+    if name == "ca24":
+        sig = [sv(rng, "gps", p, 38, 50, 9000) for p in rng.choice(range(1, 33), 8, replace=False)]
+        sig += [sv(rng, "sbas", p, 38, 50, 9000)
+                for p in rng.choice(range(120, 159), 2, replace=False)]
+        sig += [sv(rng, "qzss", p, 38, 50, 9000)
+                for p in rng.choice(range(184, 207), 1, replace=False)]
+        fc = grid(10000.0, 500.0)
+        return dict(desc="GPS+SBAS+QZSS L1 C/A, 24 antennas, 2-bit, 16.368 MHz, 10 ms; "
+                         "100 PRNs x 41 bins (+-10 kHz, 500 Hz), N=2P=32736, FP32 + ACR",
+                    n_ant=24, L=163680, fmt=2, signals=sig,
+                    systems=[(s, list(PRNS[s]), fc) for s in ("gps", "sbas", "qzss")])
+    if name == "ca1":
+        sig = [sv(rng, "gps", p, 45, 45, 5000) for p in rng.choice(range(1, 33), 4, replace=False)]
+        return dict(desc="GPS L1 C/A, 1 antenna, 1-bit, 16.368 MHz, 10 ms; PRN 1-32 x 21 bins "
+                         "(+-5 kHz, 500 Hz), N=2P=32736, FP32 + ACR",
+                    n_ant=1, L=163680, fmt=1, signals=sig,
+                    systems=[("gps", list(range(1, 33)), grid(5000.0, 500.0))])
+    if name == "pilots24":
+        sig = []
+        for s, hi in (("galileo", 51), ("beidou", 64), ("l1c", 64)):
+            sig += [sv(rng, s, p, 40, 48, 5000) for p in rng.choice(range(1, hi), 6, replace=False)]
+        return dict(desc="Galileo E1-C (P=65472) + BeiDou B1C + GPS L1C (P=163680), 24 antennas, "
+                         "1-bit, 16.368 MHz, 40 ms; 50 x 81 bins (125 Hz) + 2 x 63 x 201 bins "
+                         "(50 Hz), N=2P, FP32 + ACR",
+                    n_ant=24, L=654720, fmt=1, signals=sig,
+                    systems=[("galileo", list(PRNS["galileo"]), grid(5000.0, 125.0)),
+                             ("beidou", list(PRNS["beidou"]), grid(5000.0, 50.0)),
+                             ("l1c", list(PRNS["l1c"]), grid(5000.0, 50.0))])
+    raise SystemExit(f"unknown workload {name}")
+
+
+ACR_TABLE = {"gps": "GPS_L1_CA_ACR_TABLE", "sbas": "GPS_L1_CA_ACR_TABLE",
+             "qzss": "GPS_L1_CA_ACR_TABLE", "galileo": "GALILEO_E1C_ACR_TABLE",
+             "beidou": "BEIDOU_B1C_ACR_TABLE", "l1c": "GPS_L1C_ACR_TABLE"}
 
 
 def load_traffic(path, kernel):
@@ -59,6 +107,7 @@ def load_traffic(path, kernel):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="ca24", choices=["ca24", "ca1", "pilots24"])
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -85,20 +134,19 @@ def main():
 
     assert torch.cuda.is_available(), "no GPU visible"
     rng = np.random.default_rng(args.seed)
-    n_ant, L = 24, 163680
-    signals = config2_signals(rng)
+    w = workload(args.workload, rng)
+    n_ant, L, fmt, signals = w["n_ant"], w["L"], w["fmt"], w["signals"]
     t0 = time.perf_counter()
-    packed = synth.synth_observation(n_ant, L, FS, signals, fmt=2, seed=args.seed)
+    packed = synth.synth_observation(n_ant, L, FS, signals, fmt=fmt, seed=args.seed)
     t_synth = time.perf_counter() - t0
-    fc = [IF - 10000.0 + 500.0 * i for i in range(41)]
-    systems = [tga.SystemSearch("gps", fc), tga.SystemSearch("sbas", fc),
-               tga.SystemSearch("qzss", fc)]
+    systems = [tga.SystemSearch(s, fc, prns) for s, prns, fc in w["systems"]]
     t0 = time.perf_counter()
-    plan = tga.Plan(FS, n_ant, L, systems, input_format=2)
+    plan = tga.Plan(FS, n_ant, L, systems, input_format=fmt)
     t_plan = time.perf_counter() - t0
-    n_prn = sum(len(p) for p in plan.prns)
-    P = 16368
-    cells = n_ant * n_prn * len(fc) * P
+    periods = [int(FS * PERIOD_S[s]) for s, _, _ in w["systems"]]
+    cells = sum(n_ant * len(prns) * len(fc) * P
+                for (s, prns, fc), P in zip(w["systems"], periods))
+    n_prn = sum(len(prns) for _, prns, _ in w["systems"])
 
     dev = torch.from_numpy(packed).cuda()
     torch.cuda.synchronize()
@@ -115,13 +163,13 @@ def main():
         peaks = pl.search_device(ptr)
         # ACR C/N0 per (antenna, PRN) on the host in f64 (acquisition.rs:341-357)
         ok = 0
-        for arr in peaks:
+        for (s, _, _), arr in zip(w["systems"], peaks):
             vm = arr["best_peak"].astype(np.float64)
             vs = arr["second_peak"].astype(np.float64)
             for a in range(vm.shape[0]):
                 for j in range(vm.shape[1]):
                     if vs[a, j] > 0 and vm[a, j] > vs[a, j] and tga.estimate_cn0(
-                            (vm[a, j] / vs[a, j]) ** 2, "GPS_L1_CA_ACR_TABLE") is not None:
+                            (vm[a, j] / vs[a, j]) ** 2, ACR_TABLE[s]) is not None:
                         ok += 1
         return peaks, ok
 
@@ -150,9 +198,9 @@ def main():
     traffic = load_traffic(args.traffic, plan.kernel_name(0))
 
     fp16 = None
-    if not args.no_fp16:
+    if not args.no_fp16 and "fused16368" in plan.describe():
         # the FP16 matrix-core form on the same observation, compared with the FP32 peaks
-        plan16 = tga.Plan(FS, n_ant, L, systems, input_format=2, precision="fp16")
+        plan16 = tga.Plan(FS, n_ant, L, systems, input_format=fmt, precision="fp16")
         dt16, peaks16, _ = timed(plan16)
         r_ms, r_n, r_bytes = plan16.timing(0)
         avg16 = r_ms / 1000.0 / max(r_n, 1)
@@ -174,27 +222,27 @@ def main():
                 "same_bin_and_codephase_as_fp32": f"{same}/{rel.size}",
                 "plan": plan16.describe().splitlines()[0]}
 
-    # detections: every injected SV found at its code phase and nearest bin on most antennas
+    # detections: every injected SV found at its code phase on most antennas
     found = 0
+    sysidx = {s: i for i, (s, _, _) in enumerate(w["systems"])}
     for s in signals:
-        si = {"gps": 0, "sbas": 1, "qzss": 2}[s["system"]]
+        si = sysidx[s["system"]]
+        P = periods[si]
         j = list(plan.prns[si]).index(s["prn"])
         cp = (P - s["delay"]) % P
-        hits = sum(int(peaks[si][a, j]["best_codephase"]) == cp for a in range(n_ant))
+        hits = sum(abs(int(peaks[si][a, j]["best_codephase"]) - cp) <= 1 for a in range(n_ant))
         found += hits >= n_ant // 2
 
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import cpu_port
         workers = min(16, os.cpu_count() or 1)
-        levels = synth.unpack_levels(packed, L, 2)
+        levels = synth.unpack_levels(packed, L, fmt)
         t_cells = t_s = 0.0
         n_done = n_ants = 0
-        prn_sets = [("gps", list(range(1, 39))), ("sbas", list(range(120, 159))),
-                    ("qzss", list(range(184, 207)))]
         for a in range(n_ant):  # whole antennas, then whole systems, until the budget is spent
-            for sysname, prns in prn_sets:
-                c, t, d = cpu_port.time_sample(levels[a], FS, sysname, prns, fc, workers,
+            for s, prns, fc in w["systems"]:
+                c, t, d = cpu_port.time_sample(levels[a], FS, s, prns, fc, workers,
                                                budget_s=max(args.cpu_budget - t_s, 0.0))
                 t_cells += c
                 t_s += t
@@ -203,19 +251,19 @@ def main():
             if t_s >= args.cpu_budget:
                 break
         cpu = {"value": t_cells / t_s, "unit": "cells/s", "cores": workers, "kind": "port",
-               "sample": f"{n_done} (antenna, PRN) searches over {n_ants} antenna(s) x 41 bins, "
-                         f"N=32736, complex64 pocketfft over {workers} threads, reference work "
-                         f"structure (2 replica + 41x3 FFTs per PRN), {t_s:.1f} s"}
+               "sample": f"{n_done} (antenna, PRN) searches over {n_ants} antenna(s), N=2P, "
+                         f"complex64 pocketfft over {workers} threads, reference work structure "
+                         f"(2 replica + 3 FFTs per bin per PRN), {t_s:.1f} s"}
 
     out = {
         "metric": METRIC, "value": value, "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "GPS+SBAS+QZSS L1 C/A, 24 antennas, 2-bit, 16.368 MHz, 10 ms; "
-                               "100 PRNs x 41 bins (+-10 kHz, 500 Hz), N=2P=32736, FP32 + ACR",
-                   "n_ant": n_ant, "n_prn": n_prn, "n_bins": len(fc), "code_period": P,
-                   "cells": cells, "form": "reference (dual hypothesis, folded P-point rows)",
-                   "plan": plan.describe().splitlines()[0]},
+        "config": {"workload": w["desc"], "name": args.workload, "n_ant": n_ant,
+                   "n_prn": n_prn, "n_bins": [len(fc) for _, _, fc in w["systems"]],
+                   "code_periods": periods, "cells": cells,
+                   "form": "reference (dual hypothesis, folded P-point rows)",
+                   "plan": plan.describe().strip().splitlines()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": plan.kernel_name(0), "kernel_ms": avg_rows_s * 1000.0,

@@ -198,6 +198,7 @@ template <> struct Dft<13> { __device__ __forceinline__ static void run(float2* 
 template <> struct Dft<31> { __device__ __forceinline__ static void run(float2* v) { DftOdd<31>::run(v); } };
 // 33 = 3 x 11 in registers
 template <> struct Dft<33> { __device__ __forceinline__ static void run(float2* v) { DftSplit<3, 11>::run(v); } };
+template <> struct Dft<10> { __device__ __forceinline__ static void run(float2* v) { DftSplit<2, 5>::run(v); } };
 
 // Twiddle W_n^e from the two-level table: tw1[e % split] * tw2[e / split].
 __device__ __forceinline__ float2 twiddle2(const float2* tw1, const float2* tw2, int split,

@@ -119,6 +119,8 @@ void Plan::build(const tga_plan_desc& d) {
       g->fc = fc;
       g->folded = fold;
       g->fused = fold && fused_supported(P, N, static_cast<int>(std::ceil(fs / 1.023e6)));
+      g->big = fold && !g->fused && big_supported(P, N);
+      g->big_r0 = g->big ? static_cast<int>(P / kFusedP) : 0;
       g->M = fold ? P : N;
     }
     e.group = static_cast<int>(g - groups.data());
@@ -199,6 +201,21 @@ void Plan::build(const tga_plan_desc& d) {
       g.d_X.alloc(g.n_xrow * M * sizeof(float2));
       upload(g.d_tw, fused_twiddles(), stream);
       autotune(g);
+    } else if (g.big) {
+      g.d_X.alloc(g.n_xrow * M * sizeof(float2));
+      g.d_Xs.alloc(g.n_xrow * M * sizeof(float2));
+      upload(g.d_tw, fused_twiddles(), stream);
+      upload(g.d_twb, big_twiddles(g.P), stream);
+      // sub-rows of at most ~2 GiB of rows at a time
+      const long long cap = std::max<long long>(1, (256LL << 20) / static_cast<long long>(M));
+      g.chunk_rows = std::min<long long>(cap, g.n_rows);
+      g.d_Y.alloc(static_cast<size_t>(g.chunk_rows) * M * sizeof(float2));
+      g.d_rowkey.alloc(static_cast<size_t>(g.n_rows) * sizeof(unsigned long long));
+      g.d_rownan.alloc(static_cast<size_t>(g.n_rows) * sizeof(int));
+      const size_t nw = static_cast<size_t>(n_ant) * n_prn;
+      g.d_win.alloc(nw * sizeof(int));
+      g.d_Zw.alloc(nw * M * sizeof(float2));
+      g.d_Zws.alloc(nw * M * sizeof(float2));
     } else {
       g.d_X.alloc(g.n_xrow * M * sizeof(float2));
       g.d_Xs.alloc(g.n_xrow * M * sizeof(float2));
@@ -329,6 +346,14 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
     } else if (g.fused) {
       launch_fused_rows(g.X, g.d_rep.as<float2>(), g.d_tw.as<float2>(), n_ant, g.n_bin, g.n_prn,
                         g.cw, g.d_rows.as<RowOut>(), st, g.rows_cfg);
+    } else if (g.big) {
+      for (long long r0 = 0; r0 < g.n_rows; r0 += g.chunk_rows) {
+        const long long r1 = std::min(g.n_rows, r0 + g.chunk_rows);
+        launch_big_rows(g.X, g.d_rep.as<float2>(), g.d_tw.as<float2>(), g.d_twb.as<float2>(),
+                        g.big_r0, g.n_prn, g.n_bin, r0, r1, g.d_Y.as<float2>(),
+                        g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(),
+                        g.d_rows.as<RowOut>(), st);
+      }
     } else {
       const FftPlan* fp = fft_plan(g.M);
       for (long long r0 = 0; r0 < g.n_rows; r0 += g.chunk_rows) {
@@ -348,7 +373,13 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
     // folded rows carry half the 2P-point gain; the FP16 form's powers carry kF16Scale^4
     const float gain = (g.folded ? 2.0f : 1.0f) / (g.fp16 ? kF16Scale * kF16Scale : 1.0f);
     launch_select(g.d_rows.as<RowOut>(), g.d_slots.as<PeakSlot>(), n_ant, g.n_prn, g.n_bin,
-                  gain, g.scale, d_peaks.as<tga_peak>(), d_nan.as<int>(), st);
+                  gain, g.scale, d_peaks.as<tga_peak>(), d_nan.as<int>(), st,
+                  g.big ? g.d_win.as<int>() : nullptr);
+    if (g.big)
+      launch_big_second(g.X, g.d_rep.as<float2>(), g.d_slots.as<PeakSlot>(),
+                        static_cast<int>(g.M), static_cast<int>(g.P), g.cw, n_ant, g.n_prn,
+                        g.n_bin, g.d_win.as<int>(), gain, g.scale, g.d_Zw.as<float2>(),
+                        g.d_Zws.as<float2>(), d_peaks.as<tga_peak>(), st);
   }
   time_end(2, st, 0.0);
 }
@@ -376,8 +407,9 @@ std::string Plan::describe() const {
                   "groups=%d tune_ms=%.3f/%.3f\n",
                   system_info(e.system).key, g.P, g.N, g.fc.size(), e.prns.size(), e.group,
                   g.fp16 ? "fused16368-fp16"
-                         : g.fused ? "fused16368"
-                                   : (g.folded ? "generic-folded" : "generic-unfolded"),
+                  : g.fused ? "fused16368"
+                  : g.big ? (g.big_r0 == 4 ? "split4x16368" : "split10x16368")
+                          : (g.folded ? "generic-folded" : "generic-unfolded"),
                   g.rows_cfg.variant, g.rows_cfg.groups, g.tune_ms[0], g.tune_ms[1]);
     s += buf;
   }

@@ -16,6 +16,8 @@ struct Group {
   size_t P = 0, N = 0, M = 0;  // code period, coherent length, transform length
   std::vector<double> fc;
   bool folded = false, fused = false, fp16 = false;
+  bool big = false;  // P = big_r0 x 16368: sub-row split through the fused LDS kernel
+  int big_r0 = 0;
   std::vector<std::vector<double>> codes;  // replicas, plan creation only
   std::vector<PeakSlot> slots;
   int n_prn = 0, n_bin = 0;
@@ -25,6 +27,7 @@ struct Group {
   int cw = 1;
   DevBuf d_fc, d_slots, d_tau, d_rep, d_X, d_Xs, d_rows, d_Z, d_Zs, d_tw;
   DevBuf d_rep16, d_X16;  // FP16 form: f16 (re, im) pairs scaled by kF16Scale
+  DevBuf d_Y, d_rowkey, d_rownan, d_twb, d_win, d_Zw, d_Zws;  // long-period split
   float2* X = nullptr;  // where the forward spectra of the last search landed
   FusedRowsConfig rows_cfg;  // rows-kernel shape picked by the autotuner
   double tune_ms[4] = {0, 0, 0, 0};

@@ -87,7 +87,25 @@ void launch_row_reduce(const float2* Z, int M, int P, int cw, long long n_rows, 
 // Per (ant, prn): last-max over bins of the row peak, [c,-c] only if strictly larger; writes
 // tga_peak through `slots`; sets *nan_flag if any row saw a NaN.  corr = sqrt(p)*gain*scale
 // with gain 2 for folded rows, 1 for unfolded ones.
+// `win` (optional): per (ant, prn) the winning row's bin * 2 + h.
 void launch_select(const RowOut* rows, const PeakSlot* slots, int n_ant, int n_prn, int n_bin,
-                   float gain, float scale, tga_peak* out, int* nan_flag, hipStream_t st);
+                   float gain, float scale, tga_peak* out, int* nan_flag, hipStream_t st,
+                   int* win = nullptr);
+
+// ---- long code periods P = R0 x 16368 (R0 = 4: E1-C, R0 = 10: B1C / L1C at 16.368 MHz) ------
+// Folded rows of length P are transformed as R0 twiddled sub-rows of 16368 points by the fused
+// LDS kernel; each row keeps only its argmax, and the second peak comes from a recomputation of
+// the winning row of every (antenna, PRN) pair.
+bool big_supported(size_t P, size_t N);
+// W_P^i (i < 512) then W_P^{512 i} (i < ceil(P / 512)).
+std::vector<float2> big_twiddles(size_t P);
+// rows [r0, r1) (ordering of launch_mul_rows): Y holds (r1 - r0) * P complex values
+void launch_big_rows(const float2* X, const float2* rep, const float2* tw, const float2* twb,
+                     int R0, int n_prn, int n_bin, long long r0, long long r1, float2* Y,
+                     unsigned long long* rowkey, int* rownan, RowOut* rows, hipStream_t st);
+// second peaks of the winning rows; Z, Zs hold n_ant * n_prn * M complex values
+void launch_big_second(const float2* X, const float2* rep, const PeakSlot* slots, int M, int P,
+                       int cw, int n_ant, int n_prn, int n_bin, const int* win, float gain,
+                       float scale, float2* Z, float2* Zs, tga_peak* out, hipStream_t st);
 
 }  // namespace tga

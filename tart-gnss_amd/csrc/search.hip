@@ -324,6 +324,12 @@ struct RowArgs {
   int per_group;
   int cw;
   RowOut* rows;
+  // sub-row mode (code periods R0 x 16368): sub-rows Y[s][P] of rows r0 + s / R0
+  const float2* Y;
+  long long n_sub;
+  long long r0;
+  unsigned long long* rowkey;
+  int* rownan;
 };
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -438,6 +444,44 @@ __device__ __forceinline__ void row_finish(const Top2& main_set, const Top2& ext
   }
 }
 
+// End of one sub-row of a long row: block argmax, folded into the row's 64-bit key (power
+// bits, sample index) by an atomic maximum, so the last maximum over all sub-rows wins.
+template <int NT>
+__device__ __forceinline__ void row_finish_key(const Top2& main_set, const Top2& extra_set,
+                                               bool bad, RowRed<NT / 64>& red, int k0, int r0f,
+                                               unsigned long long* key, int* nan) {
+  constexpr int NWR = NT / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float bp = main_set.best();
+  int bi = main_set.index();
+  arg_combine(bp, bi, extra_set.best(), extra_set.index());
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const float q = __shfl_xor(bp, off, 64);
+    const int j = __shfl_xor(bi, off, 64);
+    arg_combine(bp, bi, q, j);
+  }
+  const bool wave_bad = __ballot(bad) != 0;
+  if (lane == 0) {
+    red.p[wave] = bp;
+    red.i[wave] = bi;
+    red.bad[wave] = wave_bad ? 1 : 0;
+  }
+  lds_barrier();
+  if (threadIdx.x == 0) {
+    int any_bad = red.bad[0];
+    bp = red.p[0];
+    bi = red.i[0];
+#pragma unroll
+    for (int w = 1; w < NWR; ++w) {
+      arg_combine(bp, bi, red.p[w], red.i[w]);
+      any_bad |= red.bad[w];
+    }
+    atomicMax(key, pkey(bp, k0 + r0f * bi));
+    if (any_bad) atomicOr(nan, 1);
+  }
+}
+
 // Correlation rows.  Per (antenna, bin, hypothesis) block and per PRN of its group:
 //   pass A  radix 33 (NS = 1), 496 butterflies, operands conj(X) . R gathered straight from
 //           global memory (buffer loads) into registers
@@ -455,7 +499,7 @@ __device__ __forceinline__ void row_finish(const Top2& main_set, const Top2& ext
 // Each thread keeps the top two powers of sets whose indices lie more than 32 samples apart
 // pairwise, so at most one member of a set falls inside the +-cw exclusion window around the
 // block's argmax and the masked second maximum needs no second pass over the row.
-template <int VARIANT>
+template <int VARIANT, int SUB = 0>
 __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
   constexpr int NBA = P / 33;  // 496
   constexpr int NBB = P / 16;  // 1023
@@ -503,29 +547,16 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
       twl[2 * st + 1] = j == 0 ? make_float2(0.f, 0.f) : tw_lds(stw, (31 - j) * ln16);
     }
   }
-  // Blocks are dealt round-robin to the 8 XCDs: the PRN group follows blockIdx % groups so the
-  // replica spectra a group touches stay in one XCD's L2.
-  const int g = blockIdx.x % a.groups;
-  const int xrow = blockIdx.x / a.groups;
-  const int h = xrow & 1;
-  const int bin = (xrow >> 1) % a.n_bin;
-  const int ant = (xrow >> 1) / a.n_bin;
-  const int p0 = g * a.per_group;
-  const int p1 = min(p0 + a.per_group, a.n_prn);
-  const __amdgpu_buffer_rsrc_t xv = buffer_view(a.X + static_cast<size_t>(xrow) * P, P * 8);
-  __syncthreads();
-  for (int prn = p0; prn < p1; ++prn) {
-    const __amdgpu_buffer_rsrc_t rv =
-        buffer_view(a.rep + (static_cast<size_t>(prn) * 2 + h) * P, P * 8);
+  // One correlation row (or sub-row): `operand(ta, r)` gives pass A's input r of butterfly ta,
+  // `finish` reduces the row's top-2 sets.
+  auto transform = [&](auto operand, auto finish) {
     // ---- pass A
     int ta = tid;
     asm volatile("" : "+v"(ta));
     if (ta < NBA) {
       float2 v[33];
 #pragma unroll
-      for (int r = 0; r < 33; ++r)
-        v[r] = cconjmul(buffer_load_f2(xv, ta * 8, r * NBA * 8),
-                        buffer_load_f2(rv, ta * 8, r * NBA * 8));
+      for (int r = 0; r < 33; ++r) v[r] = operand(ta, r);
       Dft<33>::run(v);
 #pragma unroll
       for (int r = 0; r < 33; ++r) buf[ta * 33 + r] = v[r];
@@ -696,8 +727,46 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
         }
       }
     }
-    row_finish<T>(main_set, extra_set, accn != accn, a.cw, red,
-                  a.rows + ((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) * 2 + h);
+    finish(main_set, extra_set, accn != accn);
+  };
+  __syncthreads();
+  if constexpr (SUB == 0) {
+    // Blocks are dealt round-robin to the 8 XCDs: the PRN group follows blockIdx % groups so
+    // the replica spectra a group touches stay in one XCD's L2.
+    const int g = blockIdx.x % a.groups;
+    const int xrow = blockIdx.x / a.groups;
+    const int h = xrow & 1;
+    const int bin = (xrow >> 1) % a.n_bin;
+    const int ant = (xrow >> 1) / a.n_bin;
+    const int p0 = g * a.per_group;
+    const int p1 = min(p0 + a.per_group, a.n_prn);
+    const __amdgpu_buffer_rsrc_t xv = buffer_view(a.X + static_cast<size_t>(xrow) * P, P * 8);
+    for (int prn = p0; prn < p1; ++prn) {
+      const __amdgpu_buffer_rsrc_t rv =
+          buffer_view(a.rep + (static_cast<size_t>(prn) * 2 + h) * P, P * 8);
+      transform(
+          [&](int ta, int r) {
+            return cconjmul(buffer_load_f2(xv, ta * 8, r * NBA * 8),
+                            buffer_load_f2(rv, ta * 8, r * NBA * 8));
+          },
+          [&](const Top2& ms, const Top2& es, bool bad) {
+            row_finish<T>(ms, es, bad, a.cw, red,
+                          a.rows + ((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) *
+                                       2 + h);
+          });
+    }
+  } else {
+    // sub-row s of the chunk: row r0 + s / SUB, output samples k0 + SUB m (k0 = s mod SUB)
+    for (long long sidx = blockIdx.x; sidx < a.n_sub; sidx += gridDim.x) {
+      const long long rq = sidx / SUB;
+      const int k0 = static_cast<int>(sidx - rq * SUB);
+      const __amdgpu_buffer_rsrc_t yv = buffer_view(a.Y + sidx * P, P * 8);
+      transform([&](int ta, int r) { return buffer_load_f2(yv, ta * 8, r * NBA * 8); },
+                [&](const Top2& ms, const Top2& es, bool bad) {
+                  row_finish_key<T>(ms, es, bad, red, k0, SUB, a.rowkey + a.r0 + rq,
+                                    a.rownan + a.r0 + rq);
+                });
+    }
   }
 }
 
@@ -1089,12 +1158,118 @@ __global__ __launch_bounds__(kRedT) void k_row_reduce(const float2* __restrict__
   }
 }
 
+// ---- long code periods P = R0 x 16368 (four-step split of the row transform) --------------
+// With n = n1 + 16368 n0 and k = k0 + R0 k1, the P-point transform of z = conj(X) R is
+//   Z[k0 + R0 k1] = FFT_16368( W_P^{n1 k0} sum_{n0} z[n1 + 16368 n0] W_R0^{n0 k0} )[k1]:
+// k_big_pre forms the R0 twiddled sub-rows y_k0 of each row (one R0-point DFT per n1), and the
+// fused rows kernel in sub-row mode transforms each sub-row in LDS.
+template <int R0>
+__global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
+                                                 const float2* __restrict__ rep,
+                                                 const float2* __restrict__ twb, int n_prn,
+                                                 int n_bin, long long r0, long long total,
+                                                 float2* __restrict__ Y,
+                                                 unsigned long long* __restrict__ rowkey,
+                                                 int* __restrict__ rownan) {
+  constexpr int P1 = kFusedP;
+  constexpr int P = R0 * P1;
+  const long long gid = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (gid >= total) return;
+  const long long rr = gid / P1;
+  const int n1 = static_cast<int>(gid - rr * P1);
+  const long long r = r0 + rr;
+  const int h = static_cast<int>(r & 1);
+  const long long q = r >> 1;  // (ant*n_prn + prn)*n_bin + bin
+  const int bin = static_cast<int>(q % n_bin);
+  const long long ap = q / n_bin;
+  const int prn = static_cast<int>(ap % n_prn);
+  const int ant = static_cast<int>(ap / n_prn);
+  const float2* x = X + ((static_cast<long long>(ant) * n_bin + bin) * 2 + h) * P;
+  const float2* c = rep + (static_cast<long long>(prn) * 2 + h) * P;
+  float2 v[R0];
+#pragma unroll
+  for (int n0 = 0; n0 < R0; ++n0) v[n0] = cconjmul(x[n1 + P1 * n0], c[n1 + P1 * n0]);
+  Dft<R0>::run(v);
+  float2* y = Y + rr * P;
+  y[n1] = v[0];
+#pragma unroll
+  for (int k0 = 1; k0 < R0; ++k0) {
+    const int e = n1 * k0;  // < P: W_P^e from two 512-entry levels
+    y[static_cast<long long>(k0) * P1 + n1] = cmul(v[k0], cmul(twb[e & 511], twb[512 + (e >> 9)]));
+  }
+  if (n1 == 0) {
+    rowkey[r] = 0ull;
+    rownan[r] = 0;
+  }
+}
+
+// Row keys of long rows -> RowOut (p2 is filled later from the winning row only).
+__global__ void k_big_rowout(const unsigned long long* __restrict__ rowkey,
+                             const int* __restrict__ rownan, long long n_rows,
+                             RowOut* __restrict__ rows) {
+  const long long r = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (r >= n_rows) return;
+  const unsigned long long k = rowkey[r];
+  RowOut o;
+  o.p1 = __uint_as_float(static_cast<unsigned int>(k >> 32));
+  o.p2 = -1.0f;
+  o.i1 = static_cast<int32_t>(k & 0xffffffffu);
+  o.nan = rownan[r];
+  rows[r] = o;
+}
+
+// Z[w][i] = conj(X) R of the winning row win[w] of (antenna, PRN) pair w.
+__global__ void k_mul_win(const float2* __restrict__ X, const float2* __restrict__ rep, int M,
+                          int n_prn, int n_bin, const int* __restrict__ win, long long total,
+                          float2* __restrict__ Z) {
+  const long long gid = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (gid >= total) return;
+  const long long w = gid / M;
+  const int i = static_cast<int>(gid - w * M);
+  const int ant = static_cast<int>(w / n_prn);
+  const int prn = static_cast<int>(w - static_cast<long long>(ant) * n_prn);
+  const int bh = win[w];  // bin * 2 + h
+  const long long xrow = static_cast<long long>(ant) * n_bin * 2 + bh;
+  Z[gid] = cconjmul(X[xrow * M + i], rep[(static_cast<long long>(prn) * 2 + (bh & 1)) * M + i]);
+}
+
+// Second peak of each winning row around the selected code phase: max |z|^2 more than cw
+// samples (ring distance mod P) from it, scaled like the best peak, floor 1e-6
+// (correlate.rs:156-175).
+__global__ __launch_bounds__(kRedT) void k_second_win(const float2* __restrict__ Z, int M, int P,
+                                                      int cw, int n_prn,
+                                                      const PeakSlot* __restrict__ slots,
+                                                      float gain, float scale,
+                                                      tga_peak* __restrict__ out) {
+  __shared__ float sp[kRedT / 64];
+  const int w = blockIdx.x;
+  const int ant = w / n_prn;
+  const int prn = w - ant * n_prn;
+  const PeakSlot sl = slots[prn];
+  tga_peak* pk = out + sl.base + static_cast<long long>(ant) * sl.stride + sl.local;
+  const int main_cp = static_cast<int>(pk->best_codephase % static_cast<uint64_t>(P));
+  const float2* z = Z + static_cast<size_t>(w) * M;
+  float m2 = -1.0f;
+  for (int i = threadIdx.x; i < M; i += kRedT) {
+    if (ring_dist(i % P, main_cp, P) > cw) {
+      const float2 v = z[i];
+      m2 = fmaxf(m2, v.x * v.x + v.y * v.y);
+    }
+  }
+  m2 = block_max<kRedT>(m2, sp);
+  if (threadIdx.x == 0) {
+    float second = m2 >= 0.f ? (__fsqrt_rn(m2) * gain) * scale : -1.0f;
+    if (!(second > 0.0f)) second = 1e-6f;
+    pk->second_peak = second;
+  }
+}
+
 // gain = 2 for folded rows: an unnormalised P-point transform pair carries half the gain of
 // the reference's 2P-point pair (FFT_2P([c,c]) at bin 2k is 2 FFT_P(c) at bin k).  The factor
 // is a power of two, so corr = (sqrt(p) * gain) * scale rounds exactly like sqrt(4p) * scale.
 __global__ void k_select(const RowOut* __restrict__ rows, const PeakSlot* __restrict__ slots,
                          int n_ant, int n_prn, int n_bin, float gain, float scale,
-                         tga_peak* __restrict__ out, int* nan_flag) {
+                         tga_peak* __restrict__ out, int* nan_flag, int* __restrict__ win) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= n_ant * n_prn) return;
   const int ant = gid / n_prn;
@@ -1126,6 +1301,7 @@ __global__ void k_select(const RowOut* __restrict__ rows, const PeakSlot* __rest
   pk.best_freq_idx = static_cast<uint64_t>(bbin);
   pk.best_codephase = static_cast<uint64_t>(w.i1);
   out[s.base + static_cast<long long>(ant) * s.stride + s.local] = pk;
+  if (win) win[gid] = 2 * bbin + bh;
   if (nan) atomicOr(nan_flag, 1);
 }
 
@@ -1254,10 +1430,74 @@ void launch_row_reduce(const float2* Z, int M, int P, int cw, long long n_rows, 
 }
 
 void launch_select(const RowOut* rows, const PeakSlot* slots, int n_ant, int n_prn, int n_bin,
-                   float gain, float scale, tga_peak* out, int* nan_flag, hipStream_t st) {
+                   float gain, float scale, tga_peak* out, int* nan_flag, hipStream_t st,
+                   int* win) {
   const int total = n_ant * n_prn;
   hipLaunchKernelGGL(k_select, dim3(grid_for(total, 128)), dim3(128), 0, st, rows, slots, n_ant,
-                     n_prn, n_bin, gain, scale, out, nan_flag);
+                     n_prn, n_bin, gain, scale, out, nan_flag, win);
+  TGA_HIP(hipGetLastError());
+}
+
+bool big_supported(size_t P, size_t N) {
+  return N == 2 * P && (P == 4 * static_cast<size_t>(kFusedP) ||
+                        P == 10 * static_cast<size_t>(kFusedP));
+}
+
+std::vector<float2> big_twiddles(size_t P) {
+  const size_t n2 = (P + 511) / 512;
+  std::vector<float2> t(512 + n2);
+  for (size_t i = 0; i < 512 + n2; ++i) {
+    const unsigned long long e = i < 512 ? i : 512ull * (i - 512);
+    const double a = -2.0 * kPiD * static_cast<double>(e % P) / static_cast<double>(P);
+    t[i] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+  }
+  return t;
+}
+
+void launch_big_rows(const float2* X, const float2* rep, const float2* tw, const float2* twb,
+                     int R0, int n_prn, int n_bin, long long r0, long long r1, float2* Y,
+                     unsigned long long* rowkey, int* rownan, RowOut* rows, hipStream_t st) {
+  const long long nr = r1 - r0;
+  const long long total = nr * kFusedP;
+  if (R0 == 4)
+    hipLaunchKernelGGL(k_big_pre<4>, dim3(grid_for(total, 256)), dim3(256), 0, st, X, rep, twb,
+                       n_prn, n_bin, r0, total, Y, rowkey, rownan);
+  else if (R0 == 10)
+    hipLaunchKernelGGL(k_big_pre<10>, dim3(grid_for(total, 256)), dim3(256), 0, st, X, rep, twb,
+                       n_prn, n_bin, r0, total, Y, rowkey, rownan);
+  else
+    fail(TGA_ERR_UNSUPPORTED, "long-period split supports 4 and 10 sub-rows");
+  TGA_HIP(hipGetLastError());
+  fused::RowArgs a{};
+  a.tw = tw;
+  a.cw = 0;
+  a.Y = Y;
+  a.n_sub = nr * R0;
+  a.r0 = r0;
+  a.rowkey = rowkey;
+  a.rownan = rownan;
+  const unsigned blocks = static_cast<unsigned>(a.n_sub < 2048 ? a.n_sub : 2048);
+  if (R0 == 4)
+    hipLaunchKernelGGL((fused::k_rows<0, 4>), dim3(blocks), dim3(fused::T), 0, st, a);
+  else
+    hipLaunchKernelGGL((fused::k_rows<0, 10>), dim3(blocks), dim3(fused::T), 0, st, a);
+  TGA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_big_rowout, dim3(grid_for(nr, 256)), dim3(256), 0, st, rowkey + r0,
+                     rownan + r0, nr, rows + r0);
+  TGA_HIP(hipGetLastError());
+}
+
+void launch_big_second(const float2* X, const float2* rep, const PeakSlot* slots, int M, int P,
+                       int cw, int n_ant, int n_prn, int n_bin, const int* win, float gain,
+                       float scale, float2* Z, float2* Zs, tga_peak* out, hipStream_t st) {
+  const long long nw = static_cast<long long>(n_ant) * n_prn;
+  const long long total = nw * M;
+  hipLaunchKernelGGL(k_mul_win, dim3(grid_for(total, 256)), dim3(256), 0, st, X, rep, M, n_prn,
+                     n_bin, win, total, Z);
+  TGA_HIP(hipGetLastError());
+  float2* z = fft_forward(fft_plan(static_cast<size_t>(M)), Z, Zs, static_cast<size_t>(nw), st);
+  hipLaunchKernelGGL(k_second_win, dim3(static_cast<unsigned>(nw)), dim3(kRedT), 0, st, z, M, P,
+                     cw, n_prn, slots, gain, scale, out);
   TGA_HIP(hipGetLastError());
 }
 

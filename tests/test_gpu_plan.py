@@ -148,3 +148,33 @@ def test_plan_errors(gpu):
     with pytest.raises(tga.TgaError) as e:
         plan.search(np.zeros((3, plan.row_bytes), np.uint8))
     assert e.value.name == "TGA_ERR_ANTENNA_RANGE"
+
+
+def test_split_e1c_at_16368khz(gpu):
+    # E1-C at 16.368 MHz: P = 65472 = 4 x 16368, rows transformed as 4 twiddled sub-rows by the
+    # fused LDS kernel; the second peak comes from recomputing the winning row
+    fs, L = FS, 2 * 65472 + 500
+    # Doppler on the grid: 8 ms of coherent integration has its first null 125 Hz off
+    signals = [dict(system="galileo", prn=11, cn0=52.0, doppler=500.0, delay=40000),
+               dict(system="galileo", prn=27, cn0=49.0, doppler=-1500.0, delay=123)]
+    fc = grid(IF, 1500.0, 500.0)
+    systems = [("galileo", [3, 11, 27], fc)]
+    plan, peaks, packed = run_case(fs, L, 2, 1, systems, signals, seed=21)
+    assert "split4x16368" in plan.describe()
+    check_against_oracle(plan, peaks, packed, fs, L, 1, systems)
+    assert int(peaks[0][0, 1]["best_codephase"]) == (65472 - 40000) % 65472
+    assert int(peaks[0][1, 2]["best_codephase"]) == (65472 - 123) % 65472
+
+
+@pytest.mark.parametrize("system,prn", [("beidou", 19), ("l1c", 7)])
+def test_split_b1c_l1c_at_16368khz(gpu, system, prn):
+    # B1C / L1C at 16.368 MHz: P = 163680 = 10 x 16368 (BOC(1,1) half-chips)
+    fs, L = FS, 2 * 163680
+    # Doppler on the grid: 20 ms of coherent integration has its first null 50 Hz off
+    signals = [dict(system=system, prn=prn, cn0=50.0, doppler=-500.0, delay=99999)]
+    fc = grid(IF, 1000.0, 500.0)
+    systems = [(system, [prn, prn + 1], fc)]
+    plan, peaks, packed = run_case(fs, L, 1, 2, systems, signals, seed=prn)
+    assert "split10x16368" in plan.describe()
+    check_against_oracle(plan, peaks, packed, fs, L, 2, systems)
+    assert int(peaks[0][0, 0]["best_codephase"]) == (163680 - 99999) % 163680
