@@ -152,6 +152,15 @@ typedef struct tga_system_desc {
   int32_t n_fc;        /* number of Doppler bins (>= 1) */
   const double* fc;    /* absolute bin centre frequencies (Hz), e.g. IF + Doppler */
   int64_t num_samples; /* coherent length N; 0 selects the reference's min(2P, record) */
+  /* 0: the reference form (N-sample coherent correlation, [c,c] / [c,-c] hypotheses).
+   * E >= 1: non-coherent form, an extension with no counterpart in the reference: E
+   * consecutive one-period epochs (E P <= record_samples), |y_e|^2 summed over them with y_e
+   * the unnormalised inverse-transform correlation of epoch e; best_peak = sqrt(sum) / sqrt(P)
+   * (the reference form's scale for E = 1), second peak masked as in the reference form.  Needs the
+   * fused P = 16368 path (TGA_ERR_UNSUPPORTED otherwise); ACR C/N0 tables are not calibrated
+   * for it. */
+  int32_t nc_epochs;
+  int32_t pad;
 } tga_system_desc;
 
 /* Arithmetic form of the correlation rows.

@@ -187,3 +187,37 @@ def acquire_single(system, signal, pp, fs, center_freq, search_band, prn, period
     return {"prn": prn, "signal_strength": float(pk["best_peak"]),
             "second_peak": float(pk["second_peak"]), "codephase_frac": cp / float(period),
             "frequency": fc[pk["best_freq_idx"]] - center_freq, "peak": pk}
+
+
+def noncoherent(signal, pp, code_period, fc, P, epochs, fs, mode="fp32"):
+    """Non-coherent search (an extension, not a reference function): per bin,
+    S[m] = sum over epochs e < E of |y_e[m]|^2, y_e the unnormalised P-point circular
+    correlation (the inverse transform without 1/P, like rustfft's) of the wiped epoch
+    x[eP:(e+1)P] (carrier phase from the absolute sample index, correlate.rs:110-115 rounding
+    in fp32 mode) with one code period.  Strength sqrt(S) / sqrt(P), which for E = 1 is the
+    reference form's scale; last maximum in a row and across bins; second peak by the rule of
+    correlate.rs:156-175.  Returns the correlate_code dict."""
+    c = np.asarray(code_period, dtype=np.float64)[:P]
+    if mode == "fp32":
+        C = np.conj(np.fft.fft(c.astype(np.float32).astype(np.float64)))
+        scale = np.float32(1.0) / np.sqrt(np.float32(P))
+    else:
+        C = np.conj(np.fft.fft(c))
+        scale = 1.0 / math.sqrt(P)
+    rows = []
+    for f in fc:
+        x = wipe(signal, pp, f, epochs * P, mode).reshape(epochs, P)
+        y = np.fft.ifft(np.fft.fft(x, axis=1) * C[None, :], axis=1) * P
+        S = (np.abs(y) ** 2).sum(axis=0)
+        corr = np.sqrt(S)
+        corr = (corr.astype(np.float32) * scale).astype(np.float32) if mode == "fp32" \
+            else corr * scale
+        idx, val = last_argmax(corr)
+        rows.append((idx, val, corr))
+    fi = select_bins(rows)
+    idx, val, corr = rows[fi]
+    sp = second_peak(corr, idx, P, fs)
+    if mode == "fp32":
+        sp = np.float32(sp)
+    return {"best_peak": val, "second_peak": sp, "best_freq_idx": fi, "best_codephase": idx,
+            "rows": [(r[0], r[1]) for r in rows]}

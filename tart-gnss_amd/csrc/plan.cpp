@@ -89,8 +89,17 @@ void Plan::build(const tga_plan_desc& d) {
       if (!std::isfinite(f)) fail(TGA_ERR_INVALID_ARGUMENT, "bin frequencies must be finite");
     const size_t P = samples_per_period(sd.system, fs);
     if (P == 0) fail(TGA_ERR_INVALID_ARGUMENT, "sampling rate below one sample per period");
-    size_t N = sd.num_samples > 0 ? static_cast<size_t>(sd.num_samples)
-                                  : std::min<size_t>(2 * P, static_cast<size_t>(L));
+    const int nc = sd.nc_epochs;
+    if (nc < 0) fail(TGA_ERR_INVALID_ARGUMENT, "nc_epochs must be >= 0");
+    if (nc > 0) {
+      if (static_cast<long long>(nc) * static_cast<long long>(P) > L)
+        fail(TGA_ERR_INVALID_ARGUMENT, "nc_epochs periods exceed the record length");
+      if (P != static_cast<size_t>(kFusedP) || std::ceil(fs / 1.023e6) > 16)
+        fail(TGA_ERR_UNSUPPORTED, "the non-coherent form needs 16368-sample code periods");
+    }
+    size_t N = nc > 0 ? P
+                      : sd.num_samples > 0 ? static_cast<size_t>(sd.num_samples)
+                                           : std::min<size_t>(2 * P, static_cast<size_t>(L));
     if (N > static_cast<size_t>(L))
       fail(TGA_ERR_INVALID_ARGUMENT, "num_samples exceeds the record length");
     if (N > (1u << 26)) fail(TGA_ERR_UNSUPPORTED, "coherent length too large");
@@ -98,7 +107,7 @@ void Plan::build(const tga_plan_desc& d) {
     peak_off += static_cast<size_t>(n_ant) * e.prns.size();
     // replica of N samples (acquisition.rs:219); folding needs c[n+P] == c[n]
     std::vector<std::vector<double>> codes;
-    bool fold = (N == 2 * P);
+    bool fold = nc == 0 && N == 2 * P;
     for (int p : e.prns) {
       std::vector<double> c(N);
       resample(code_chips(sd.system, p), static_cast<double>(P), N, c.data());
@@ -110,7 +119,7 @@ void Plan::build(const tga_plan_desc& d) {
     // find or create the group
     Group* g = nullptr;
     for (auto& gg : groups)
-      if (gg.P == P && gg.N == N && gg.fc == fc && gg.folded == fold) g = &gg;
+      if (gg.P == P && gg.N == N && gg.fc == fc && gg.folded == fold && gg.nc == nc) g = &gg;
     if (!g) {
       groups.emplace_back();
       g = &groups.back();
@@ -118,7 +127,9 @@ void Plan::build(const tga_plan_desc& d) {
       g->N = N;
       g->fc = fc;
       g->folded = fold;
-      g->fused = fold && fused_supported(P, N, static_cast<int>(std::ceil(fs / 1.023e6)));
+      g->nc = nc;
+      g->fused = nc > 0 ||
+                 (fold && fused_supported(P, N, static_cast<int>(std::ceil(fs / 1.023e6))));
       g->big = fold && !g->fused && big_supported(P, N);
       g->big_r0 = g->big ? static_cast<int>(P / kFusedP) : 0;
       g->M = fold ? P : N;
@@ -182,9 +193,9 @@ void Plan::build(const tga_plan_desc& d) {
     g.d_rep = std::move(r == a.as<float2>() ? a : sc);
     g.codes.clear();
     g.codes.shrink_to_fit();
-    g.fp16 = g.fused && precision == TGA_PRECISION_FP16;
+    g.fp16 = g.fused && g.nc == 0 && precision == TGA_PRECISION_FP16;
     // forward spectra and per-row results
-    g.n_xrow = static_cast<size_t>(n_ant) * n_bin * (g.folded ? 2 : 1);
+    g.n_xrow = static_cast<size_t>(n_ant) * n_bin * (g.nc > 0 ? g.nc : g.folded ? 2 : 1);
     g.n_rows = static_cast<long long>(n_ant) * n_prn * n_bin * 2;
     g.d_rows.alloc(static_cast<size_t>(g.n_rows) * sizeof(RowOut));
     g.n_prn = n_prn;
@@ -196,6 +207,9 @@ void Plan::build(const tga_plan_desc& d) {
       TGA_HIP(hipStreamSynchronize(stream));
       g.d_rep.release();
       g.d_X16.alloc(g.n_xrow * M * sizeof(uint32_t));
+      upload(g.d_tw, fused_twiddles(), stream);
+    } else if (g.nc > 0) {
+      g.d_X.alloc(g.n_xrow * M * sizeof(float2));
       upload(g.d_tw, fused_twiddles(), stream);
     } else if (g.fused) {
       g.d_X.alloc(g.n_xrow * M * sizeof(float2));
@@ -228,7 +242,10 @@ void Plan::build(const tga_plan_desc& d) {
     // algorithmic bytes of the rows stage: each row reads its forward spectrum and replica
     // spectrum once (M complex f32 each)
     const double elem = g.fp16 ? 4.0 : 8.0;  // bytes per complex value
-    g.row_bytes_alg = static_cast<double>(g.n_rows) * 2.0 * static_cast<double>(M) * elem;
+    // non-coherent rows: one (X, R) pair per epoch, half as many rows (one hypothesis)
+    const double per_row = g.nc > 0 ? 0.5 * g.nc : 1.0;
+    g.row_bytes_alg =
+        static_cast<double>(g.n_rows) * per_row * 2.0 * static_cast<double>(M) * elem;
     g.fwd_bytes_alg = static_cast<double>(g.n_xrow) * static_cast<double>(M) * elem;
   }
   TGA_HIP(hipStreamSynchronize(stream));
@@ -316,7 +333,12 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
   time_begin(1, st);
   for (auto& g : groups) {
     const int M = static_cast<int>(g.M);
-    if (g.fp16) {
+    if (g.nc > 0) {
+      launch_fused_forward_nc(d_in, row_bytes, fmt, d_means.as<double>(), g.d_fc.as<double>(),
+                              n_ant, g.n_bin, g.nc, phase_const, g.d_tw.as<float2>(),
+                              g.d_X.as<float2>(), st);
+      g.X = g.d_X.as<float2>();
+    } else if (g.fp16) {
       launch_fused_forward16(d_in, row_bytes, fmt, d_means.as<double>(), g.d_fc.as<double>(),
                              n_ant, g.n_bin, phase_const, g.d_tau.as<float2>(),
                              g.d_tw.as<float2>(), g.d_X16.as<uint32_t>(), st);
@@ -340,7 +362,10 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
   time_begin(0, st);
   for (auto& g : groups) {
     const int M = static_cast<int>(g.M);
-    if (g.fp16) {
+    if (g.nc > 0) {
+      launch_fused_rows_nc(g.X, g.d_rep.as<float2>(), g.d_tw.as<float2>(), n_ant, g.n_bin,
+                           g.n_prn, g.nc, g.cw, g.d_rows.as<RowOut>(), 8, st);
+    } else if (g.fp16) {
       launch_fused_rows16(g.d_X16.as<uint32_t>(), g.d_rep16.as<uint32_t>(), g.d_tw.as<float2>(),
                           n_ant, g.n_bin, g.n_prn, g.cw, g.d_rows.as<RowOut>(), 8, st);
     } else if (g.fused) {
@@ -406,7 +431,8 @@ std::string Plan::describe() const {
                   "%s: P=%zu N=%zu bins=%zu prns=%zu group=%d path=%s rows_variant=%d "
                   "groups=%d tune_ms=%.3f/%.3f\n",
                   system_info(e.system).key, g.P, g.N, g.fc.size(), e.prns.size(), e.group,
-                  g.fp16 ? "fused16368-fp16"
+                  g.nc > 0 ? "fused16368-noncoherent"
+                  : g.fp16 ? "fused16368-fp16"
                   : g.fused ? "fused16368"
                   : g.big ? (g.big_r0 == 4 ? "split4x16368" : "split10x16368")
                           : (g.folded ? "generic-folded" : "generic-unfolded"),
@@ -525,9 +551,11 @@ const char* tga_plan_kernel_name(const tga_plan* plan, int32_t kernel) {
   const Plan* p = reinterpret_cast<const Plan*>(plan);
   const bool fused = p && !p->groups.empty() && p->groups[0].fused;
   const bool fp16 = fused && p->groups[0].fp16;
+  const bool big = p && !p->groups.empty() && p->groups[0].big;
   switch (kernel) {
-    case 0: return fp16 ? "k_rows16" : fused ? "k_rows" : "k_mul_rows+k_pass+k_row_reduce";
-    case 1: return fused ? "k_forward" : "k_wipe_obs+k_pass";
+    case 0: return fp16 ? "k_rows16" : fused ? "k_rows" : big ? "k_big_pre+k_rows<0,R0>"
+                                                              : "k_mul_rows+k_pass+k_row_reduce";
+    case 1: return fused ? "k_forward" : "k_fold_obs+k_pass";
     case 2: return "k_select";
   }
   return "";

@@ -16,6 +16,7 @@ struct Group {
   size_t P = 0, N = 0, M = 0;  // code period, coherent length, transform length
   std::vector<double> fc;
   bool folded = false, fused = false, fp16 = false;
+  int nc = 0;        // non-coherent form: one-period epochs summed (0 = reference form)
   bool big = false;  // P = big_r0 x 16368: sub-row split through the fused LDS kernel
   int big_r0 = 0;
   std::vector<std::vector<double>> codes;  // replicas, plan creation only

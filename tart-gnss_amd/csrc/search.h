@@ -46,6 +46,16 @@ struct FusedRowsConfig {
 void launch_fused_rows(const float2* X, const float2* rep, const float2* tw, int n_ant,
                        int n_bin, int n_prn, int cw, RowOut* rows, hipStream_t st,
                        const FusedRowsConfig& cfg);
+// Non-coherent form (P = 16368): X[(ant*n_bin + bin)*epochs + e][P] are the spectra of the
+// one-period epochs; rows[((ant*n_prn + prn)*n_bin + bin)*2] hold the top-2 of the power sums
+// over epochs (the h = 1 slot is a row that never wins the selection).
+void launch_fused_forward_nc(const uint8_t* packed, size_t row_bytes, int fmt,
+                             const double* means, const double* fc, int n_ant, int n_bin,
+                             int epochs, double phase_const, const float2* tw, float2* X,
+                             hipStream_t st);
+void launch_fused_rows_nc(const float2* X, const float2* rep, const float2* tw, int n_ant,
+                          int n_bin, int n_prn, int epochs, int cw, RowOut* rows, int groups,
+                          hipStream_t st);
 // FP16 form of the fused path: X and the replica spectra are stored as f16 (re, im) pairs of
 // the value times kF16Scale (so |conj(X) R| stays far inside the f16 range for 1- and 2-bit
 // input), the radix passes run as f16 GEMMs on the matrix cores with f32 accumulation, and the

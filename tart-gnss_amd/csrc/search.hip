@@ -256,6 +256,7 @@ struct FwdArgs {
   const float2* tw;
   float2* X;     // FP32 form
   uint32_t* X16;  // FP16 form: f16 (re, im) pairs of X * kF16Scale
+  int epochs;     // non-coherent form: one-period epochs per (antenna, bin)
 };
 
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
@@ -268,25 +269,34 @@ __device__ __forceinline__ uint32_t pack_f16(float2 v, float s) {
                                               static_cast<_Float16>(v.y * s)});
 }
 
-template <bool F16>
+// Forward spectra.  Reference form: xrow = (ant * n_bin + bin) * 2 + h, the folded sum (h = 0)
+// or twisted difference (h = 1) of the two periods.  Non-coherent form (NC): xrow =
+// (ant * n_bin + bin) * epochs + e, the e-th one-period epoch.
+template <bool F16, bool NC = false>
 __global__ __launch_bounds__(T) void k_forward(FwdArgs a) {
   __shared__ float2 buf[P];
   __shared__ float2 stw[256];
   const int tid = threadIdx.x;
   for (int i = tid; i < 256; i += T) stw[i] = a.tw[i];
   const int xrow = blockIdx.x;
-  const int h = xrow & 1;
-  const int bin = (xrow >> 1) % a.n_bin;
-  const int ant = (xrow >> 1) / a.n_bin;
+  const int h = NC ? 0 : (xrow & 1);
+  const int e = NC ? xrow % a.epochs : 0;
+  const int ab = NC ? xrow / a.epochs : (xrow >> 1);
+  const int bin = ab % a.n_bin;
+  const int ant = ab / a.n_bin;
   const uint8_t* row = a.packed + static_cast<size_t>(ant) * a.row_bytes;
   const double mean = a.means[ant];
   const double f = a.fc[bin];
   for (int n = tid; n < P; n += T) {
-    const float2 x0 = F16 ? wipe_obs_f64(row, a.fmt, n, mean, a.pc, f)
-                          : wipe_obs(row, a.fmt, n, mean, a.pc, f);
-    const float2 x1 = F16 ? wipe_obs_f64(row, a.fmt, n + P, mean, a.pc, f)
-                          : wipe_obs(row, a.fmt, n + P, mean, a.pc, f);
-    buf[n] = h == 0 ? cadd(x0, x1) : cmul(csub(x0, x1), a.tau[n]);
+    if constexpr (NC) {
+      buf[n] = wipe_obs(row, a.fmt, n + static_cast<long long>(e) * P, mean, a.pc, f);
+    } else {
+      const float2 x0 = F16 ? wipe_obs_f64(row, a.fmt, n, mean, a.pc, f)
+                            : wipe_obs(row, a.fmt, n, mean, a.pc, f);
+      const float2 x1 = F16 ? wipe_obs_f64(row, a.fmt, n + P, mean, a.pc, f)
+                            : wipe_obs(row, a.fmt, n + P, mean, a.pc, f);
+      buf[n] = h == 0 ? cadd(x0, x1) : cmul(csub(x0, x1), a.tau[n]);
+    }
   }
   __syncthreads();
   passes_to_last(buf, stw, tid);
@@ -330,6 +340,7 @@ struct RowArgs {
   long long r0;
   unsigned long long* rowkey;
   int* rownan;
+  int epochs;  // non-coherent form: one-period epochs per (antenna, bin)
 };
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -499,7 +510,7 @@ __device__ __forceinline__ void row_finish_key(const Top2& main_set, const Top2&
 // Each thread keeps the top two powers of sets whose indices lie more than 32 samples apart
 // pairwise, so at most one member of a set falls inside the +-cw exclusion window around the
 // block's argmax and the masked second maximum needs no second pass over the row.
-template <int VARIANT, int SUB = 0>
+template <int VARIANT, int SUB = 0, bool NC = false>
 __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
   constexpr int NBA = P / 33;  // 496
   constexpr int NBB = P / 16;  // 1023
@@ -547,9 +558,10 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
       twl[2 * st + 1] = j == 0 ? make_float2(0.f, 0.f) : tw_lds(stw, (31 - j) * ln16);
     }
   }
-  // One correlation row (or sub-row): `operand(ta, r)` gives pass A's input r of butterfly ta,
-  // `finish` reduces the row's top-2 sets.
-  auto transform = [&](auto operand, auto finish) {
+  // One transform of a correlation row (or sub-row): `operand(ta, r)` gives pass A's input r of
+  // butterfly ta; every output power goes to `sink(slot, p, idx)` with slot < 31 for the
+  // thread's main set (outputs more than 32 samples apart) and slot 31 / 32 for its extra set.
+  auto transform = [&](auto operand, auto sink) {
     // ---- pass A
     int ta = tid;
     asm volatile("" : "+v"(ta));
@@ -593,8 +605,6 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
       lds_barrier();
     }
     // ---- pass C
-    Top2 main_set, extra_set;
-    float accn = 0.0f;  // sum of powers: NaN-propagating detector
     if constexpr (VARIANT == 0) {
       int tc = tid;
       asm volatile("" : "+v"(tc));
@@ -627,14 +637,12 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
         // kp = 0: every root is 1 and the sine sums vanish, so Y_0 = ar + i ai
         const float y1r = ar + bi, y1i = ai - br;  // Y_kp
         const float p1 = y1r * y1r + y1i * y1i;
-        accn += p1;
-        extra_set.add(p1, q + kp * NBC);
+        sink(31, p1, q + kp * NBC);
         if (kp > 0) {
           const float y2r = ar - bi, y2i = ai + br;  // Y_{31-kp}
           const float p2 = y2r * y2r + y2i * y2i;
-          accn += p2;
           // the two outputs of a unit lie (31 - 2 kp) 528 > 32 samples apart
-          extra_set.add(p2, q + (31 - kp) * NBC);
+          sink(32, p2, q + (31 - kp) * NBC);
         }
       }
 #pragma unroll
@@ -643,8 +651,7 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
 #pragma unroll
       for (int r = 0; r < 31; ++r) {
         const float pv = v[r].x * v[r].x + v[r].y * v[r].y;
-        accn += pv;
-        main_set.add(pv, tc + r * NBC);  // outputs of one butterfly lie 528 apart
+        sink(r, pv, tc + r * NBC);  // outputs of one butterfly lie 528 apart
       }
     } else {
       int gc = grp;
@@ -706,7 +713,7 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
           const int ts = b0 + u;
           if (ts < ntile) {
             const int q = 16 * (wave + 8 * ts) + ln16;
-            Top2& set = ts < 4 ? main_set : extra_set;
+            const int slot = ts < 4 ? 0 : 31;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
               const int kk = 4 * gc + e;  // output row of the GEMM tile
@@ -714,23 +721,67 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
               const float br = acc[u][2][e], bi = acc[u][3][e];
               const float y1r = ar + bi, y1i = ai - br;  // Y_kk
               const float p1 = y1r * y1r + y1i * y1i;
-              accn += p1;
-              set.add(p1, q + kk * NBC);
+              sink(slot, p1, q + kk * NBC);
               if (kk > 0) {
                 const float y2r = ar - bi, y2i = ai + br;  // Y_{31-kk}
                 const float p2 = y2r * y2r + y2i * y2i;
-                accn += p2;
-                set.add(p2, q + (31 - kk) * NBC);
+                sink(slot, p2, q + (31 - kk) * NBC);
               }
             }
           }
         }
       }
     }
-    finish(main_set, extra_set, accn != accn);
   };
   __syncthreads();
-  if constexpr (SUB == 0) {
+  if constexpr (NC) {
+    // Non-coherent form: block (antenna, bin) x PRN group; per PRN, a.epochs one-period
+    // transforms whose powers are summed per output before the top-2 selection.
+    const int g = blockIdx.x % a.groups;
+    const int ab = blockIdx.x / a.groups;
+    const int bin = ab % a.n_bin;
+    const int ant = ab / a.n_bin;
+    const int p0 = g * a.per_group;
+    const int p1 = min(p0 + a.per_group, a.n_prn);
+    for (int prn = p0; prn < p1; ++prn) {
+      const __amdgpu_buffer_rsrc_t rv =
+          buffer_view(a.rep + static_cast<size_t>(prn) * 2 * P, P * 8);
+      float acc[33];
+#pragma unroll
+      for (int i = 0; i < 33; ++i) acc[i] = 0.0f;
+      for (int e = 0; e < a.epochs; ++e) {
+        const __amdgpu_buffer_rsrc_t xv = buffer_view(
+            a.X + (static_cast<size_t>(ab) * a.epochs + e) * P, P * 8);
+        transform(
+            [&](int ta, int r) {
+              return cconjmul(buffer_load_f2(xv, ta * 8, r * NBA * 8),
+                              buffer_load_f2(rv, ta * 8, r * NBA * 8));
+            },
+            [&](int slot, float p, int) { acc[slot] += p; });
+        lds_barrier();  // the next epoch's pass A overwrites the LDS row
+      }
+      // the outputs of each slot (same indices in every epoch, see pass C)
+      Top2 ms, es;
+      float accn = 0.0f;
+#pragma unroll
+      for (int r = 0; r < 31; ++r) {
+        accn += acc[r];
+        ms.add(acc[r], tid + r * NBC);
+      }
+      if (tid < 16 * LEFT) {
+        const int b = tid & (LEFT - 1), kp = tid >> 4, q = T + b;
+        accn += acc[31];
+        es.add(acc[31], q + kp * NBC);
+        if (kp > 0) {
+          accn += acc[32];
+          es.add(acc[32], q + (31 - kp) * NBC);
+        }
+      }
+      RowOut* out = a.rows + ((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) * 2;
+      row_finish<T>(ms, es, accn != accn, a.cw, red, out);
+      if (tid == 0) out[1] = RowOut{0.0f, -1.0f, 0, 0};  // single hypothesis: h = 1 never wins
+    }
+  } else if constexpr (SUB == 0) {
     // Blocks are dealt round-robin to the 8 XCDs: the PRN group follows blockIdx % groups so
     // the replica spectra a group touches stay in one XCD's L2.
     const int g = blockIdx.x % a.groups;
@@ -744,16 +795,19 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
     for (int prn = p0; prn < p1; ++prn) {
       const __amdgpu_buffer_rsrc_t rv =
           buffer_view(a.rep + (static_cast<size_t>(prn) * 2 + h) * P, P * 8);
+      Top2 ms, es;
+      float accn = 0.0f;  // sum of powers: NaN-propagating detector
       transform(
           [&](int ta, int r) {
             return cconjmul(buffer_load_f2(xv, ta * 8, r * NBA * 8),
                             buffer_load_f2(rv, ta * 8, r * NBA * 8));
           },
-          [&](const Top2& ms, const Top2& es, bool bad) {
-            row_finish<T>(ms, es, bad, a.cw, red,
-                          a.rows + ((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) *
-                                       2 + h);
+          [&](int slot, float p, int idx) {
+            accn += p;
+            (slot < 31 ? ms : es).add(p, idx);
           });
+      row_finish<T>(ms, es, accn != accn, a.cw, red,
+                    a.rows + ((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) * 2 + h);
     }
   } else {
     // sub-row s of the chunk: row r0 + s / SUB, output samples k0 + SUB m (k0 = s mod SUB)
@@ -761,11 +815,15 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
       const long long rq = sidx / SUB;
       const int k0 = static_cast<int>(sidx - rq * SUB);
       const __amdgpu_buffer_rsrc_t yv = buffer_view(a.Y + sidx * P, P * 8);
+      Top2 ms, es;
+      float accn = 0.0f;
       transform([&](int ta, int r) { return buffer_load_f2(yv, ta * 8, r * NBA * 8); },
-                [&](const Top2& ms, const Top2& es, bool bad) {
-                  row_finish_key<T>(ms, es, bad, red, k0, SUB, a.rowkey + a.r0 + rq,
-                                    a.rownan + a.r0 + rq);
+                [&](int slot, float p, int idx) {
+                  accn += p;
+                  (slot < 31 ? ms : es).add(p, idx);
                 });
+      row_finish_key<T>(ms, es, accn != accn, red, k0, SUB, a.rowkey + a.r0 + rq,
+                        a.rownan + a.r0 + rq);
     }
   }
 }
@@ -1340,7 +1398,7 @@ std::vector<float2> fused_twiddles() {
 void launch_fused_forward(const uint8_t* packed, size_t row_bytes, int fmt, const double* means,
                           const double* fc, int n_ant, int n_bin, double phase_const,
                           const float2* tau, const float2* tw, float2* X, hipStream_t st) {
-  fused::FwdArgs a{packed, row_bytes, fmt, means, fc, n_bin, phase_const, tau, tw, X, nullptr};
+  fused::FwdArgs a{packed, row_bytes, fmt, means, fc, n_bin, phase_const, tau, tw, X, nullptr, 0};
   hipLaunchKernelGGL(fused::k_forward<false>, dim3(n_ant * n_bin * 2), dim3(fused::T), 0, st, a);
   TGA_HIP(hipGetLastError());
 }
@@ -1361,11 +1419,36 @@ void launch_fused_rows(const float2* X, const float2* rep, const float2* tw, int
   TGA_HIP(hipGetLastError());
 }
 
+void launch_fused_forward_nc(const uint8_t* packed, size_t row_bytes, int fmt,
+                             const double* means, const double* fc, int n_ant, int n_bin,
+                             int epochs, double phase_const, const float2* tw, float2* X,
+                             hipStream_t st) {
+  fused::FwdArgs a{packed, row_bytes, fmt, means, fc, n_bin, phase_const, nullptr, tw, X,
+                   nullptr, epochs};
+  hipLaunchKernelGGL((fused::k_forward<false, true>), dim3(n_ant * n_bin * epochs),
+                     dim3(fused::T), 0, st, a);
+  TGA_HIP(hipGetLastError());
+}
+
+void launch_fused_rows_nc(const float2* X, const float2* rep, const float2* tw, int n_ant,
+                          int n_bin, int n_prn, int epochs, int cw, RowOut* rows, int groups,
+                          hipStream_t st) {
+  groups = n_prn < groups ? n_prn : groups;
+  const int per = (n_prn + groups - 1) / groups;
+  fused::RowArgs a{X, rep, tw, n_bin, n_prn, groups, per, cw, rows};
+  a.epochs = epochs;
+  const long long blocks = static_cast<long long>(n_ant) * n_bin * groups;
+  hipLaunchKernelGGL((fused::k_rows<0, 0, true>), dim3(static_cast<unsigned>(blocks)),
+                     dim3(fused::T), 0, st, a);
+  TGA_HIP(hipGetLastError());
+}
+
 void launch_fused_forward16(const uint8_t* packed, size_t row_bytes, int fmt,
                             const double* means, const double* fc, int n_ant, int n_bin,
                             double phase_const, const float2* tau, const float2* tw,
                             uint32_t* X16, hipStream_t st) {
-  fused::FwdArgs a{packed, row_bytes, fmt, means, fc, n_bin, phase_const, tau, tw, nullptr, X16};
+  fused::FwdArgs a{packed, row_bytes, fmt, means, fc, n_bin, phase_const, tau, tw, nullptr, X16,
+                   0};
   hipLaunchKernelGGL(fused::k_forward<true>, dim3(n_ant * n_bin * 2), dim3(fused::T), 0, st, a);
   TGA_HIP(hipGetLastError());
 }

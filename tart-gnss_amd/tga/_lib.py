@@ -45,7 +45,7 @@ class AcqResult(C.Structure):
 class SystemDesc(C.Structure):
     _fields_ = [("system", C.c_int32), ("n_prn", C.c_int32), ("prns", C.POINTER(C.c_int32)),
                 ("n_fc", C.c_int32), ("fc", C.POINTER(C.c_double)),
-                ("num_samples", C.c_int64)]
+                ("num_samples", C.c_int64), ("nc_epochs", C.c_int32), ("pad", C.c_int32)]
 
 
 class PlanDesc(C.Structure):

@@ -189,11 +189,13 @@ def estimate_cn0(r_a, table):
 @dataclass
 class SystemSearch:
     """One system of a batched plan: PRNs (None = full range), absolute bin frequencies,
-    coherent length (0 = two code periods, as the reference)."""
+    coherent length (0 = two code periods, as the reference), and nc_epochs > 0 for the
+    non-coherent form (that many one-period epochs, powers summed)."""
     system: str
     fc: list
     prns: list = None
     num_samples: int = 0
+    nc_epochs: int = 0
 
 
 PEAK_DTYPE = np.dtype([("best_peak", np.float32), ("second_peak", np.float32),
@@ -216,6 +218,7 @@ class Plan:
             descs[i].n_fc = len(fc)
             descs[i].fc = _ptr(fc, C.c_double)
             descs[i].num_samples = int(s.num_samples)
+            descs[i].nc_epochs = int(s.nc_epochs)
             if s.prns is None:
                 descs[i].n_prn = 0
                 descs[i].prns = None

@@ -178,3 +178,55 @@ def test_split_b1c_l1c_at_16368khz(gpu, system, prn):
     assert "split10x16368" in plan.describe()
     check_against_oracle(plan, peaks, packed, fs, L, 2, systems)
     assert int(peaks[0][0, 0]["best_codephase"]) == (163680 - 99999) % 163680
+
+
+def check_nc_against_oracle(plan, peaks, packed, fs, L, fmt, systems, epochs):
+    from oracle import correlate as ocor
+    levels = synth.unpack_levels(packed, L, fmt)
+    for si, (name, prns, fc) in enumerate(systems):
+        P = ocodes.samples_per_period(name, fs)
+        for a in range(plan.n_ant):
+            lv = levels[a].astype(np.float64)
+            sig = (lv - lv.sum() / len(lv)).astype(np.float32)
+            pp = ocor.phasepoints(fs, epochs * P)
+            for j, prn in enumerate(plan.prns[si]):
+                code = ocodes.code_resampled(name, prn, float(P), P)
+                ref = ocor.noncoherent(sig, pp, code, fc, P, epochs, fs)
+                got = peaks[si][a, j]
+                what = f"{name} {prn} ant {a}"
+                assert abs(float(got["best_peak"]) - ref["best_peak"]) <= \
+                    1e-5 * ref["best_peak"], what
+                vals = sorted((v for _, v in ref["rows"]), reverse=True)
+                if len(vals) > 1 and vals[1] >= vals[0] * (1 - 2e-5):
+                    continue  # near-tie between bins
+                assert int(got["best_freq_idx"]) == ref["best_freq_idx"], what
+                assert int(got["best_codephase"]) == ref["best_codephase"], what
+                assert abs(float(got["second_peak"]) - ref["second_peak"]) <= \
+                    1e-5 * ref["second_peak"], what
+
+
+@pytest.mark.parametrize("fmt", [1, 2])
+def test_noncoherent_plan_matches_oracle(gpu, fmt):
+    # 10 one-period epochs of a 10 ms record, |y|^2 summed (an extension of the reference)
+    L, E = 163680, 10
+    signals = [dict(system="gps", prn=6, cn0=41.0, doppler=1500.0, delay=7000),
+               dict(system="sbas", prn=133, cn0=44.0, doppler=-2000.0, delay=300)]
+    fc = grid(IF, 2500.0, 500.0)
+    systems = [("gps", [6, 7], fc), ("sbas", [133], fc)]
+    packed = synth.synth_observation(2, L, FS, signals, fmt=fmt, seed=40 + fmt)
+    plan = tga.Plan(FS, 2, L, [tga.SystemSearch(n, f, p, nc_epochs=E) for n, p, f in systems],
+                    input_format=fmt)
+    assert "fused16368-noncoherent" in plan.describe()
+    peaks = plan.search(packed)
+    check_nc_against_oracle(plan, peaks, packed, FS, L, fmt, systems, E)
+    for a in range(2):
+        assert int(peaks[0][a, 0]["best_codephase"]) == (16368 - 7000) % 16368
+        assert int(peaks[1][a, 0]["best_codephase"]) == (16368 - 300) % 16368
+
+
+def test_noncoherent_plan_errors(gpu):
+    fc = [IF]
+    with pytest.raises(tga.TgaError):  # more epochs than the record holds
+        tga.Plan(FS, 1, 5 * 16368, [tga.SystemSearch("gps", fc, [1], nc_epochs=6)])
+    with pytest.raises(tga.TgaError):  # no fused path for E1-C periods
+        tga.Plan(FS, 1, 4 * 65472, [tga.SystemSearch("galileo", fc, [1], nc_epochs=2)])

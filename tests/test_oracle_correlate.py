@@ -165,3 +165,54 @@ def test_folding_identity(flip):
             assert np.allclose(y[:P], fold[bi][h], rtol=1e-9, atol=1e-9)
             full.append(y)
         assert ref[bi][1] == pytest.approx(max(full[0].max(), full[1].max()), rel=1e-12)
+
+
+def test_noncoherent_one_epoch_is_the_single_period_correlation():
+    # E = 1: the non-coherent search of one period equals correlate_code with N = P, where
+    # both hypotheses coincide (correlate.rs:53-56)
+    P, fs = 1023, 1.023e6
+    code = codes.code_resampled("gps", 5, float(P), P)
+    rng = np.random.default_rng(3)
+    sig = (np.roll(code, -100) + rng.standard_normal(P)).astype(np.float32)
+    pp = oc.phasepoints(fs, P)
+    fc = [-500.0, 0.0, 500.0]
+    cc, ca = oc.code_spectra(code, P)
+    ref = oc.correlate_code(sig, pp, cc, ca, fc, P, P, fs)
+    nc = oc.noncoherent(sig, pp, code, fc, P, 1, fs)
+    assert nc["best_freq_idx"] == ref["best_freq_idx"]
+    assert nc["best_codephase"] == ref["best_codephase"]
+    assert nc["best_peak"] == pytest.approx(float(ref["best_peak"]), rel=1e-6)
+    assert nc["second_peak"] == pytest.approx(float(ref["second_peak"]), rel=1e-6)
+
+
+def test_noncoherent_matches_brute_force():
+    # E = 3 epochs of P = 64 against sum_e |sum_n x_e[n] c[(n - m) mod P]|^2 written out; the
+    # unnormalised inverse transform carries a factor P, so the strength is sqrt(P S)
+    P, E, fs = 64, 3, 64000.0
+    rng = np.random.default_rng(11)
+    code = np.where(rng.random(P) < 0.5, -1.0, 1.0)
+    sig = rng.standard_normal(E * P).astype(np.float32)
+    pp = oc.phasepoints(fs, E * P)
+    f = 1000.0
+    x = oc.wipe(sig, pp, f, E * P, "fp64").reshape(E, P)
+    S = np.zeros(P)
+    for m in range(P):
+        for e in range(E):
+            S[m] += abs(sum(x[e, n] * code[(n - m) % P] for n in range(P))) ** 2
+    nc = oc.noncoherent(sig, pp, code, [f], P, E, fs, mode="fp64")
+    want = np.sqrt(S * P)
+    assert nc["best_codephase"] == int(np.flatnonzero(want == want.max())[-1])
+    assert nc["best_peak"] == pytest.approx(want.max(), rel=1e-12)
+
+
+def test_noncoherent_recovers_weak_signal():
+    # 10 one-period epochs detect a delay that a single period misses more often
+    P, fs, E = 1023, 1.023e6, 10
+    code = codes.code_resampled("gps", 9, float(P), P)
+    rng = np.random.default_rng(8)
+    n = np.arange(E * P)
+    sig = (0.35 * code[(n + 300) % P] + rng.standard_normal(E * P)).astype(np.float32)
+    pp = oc.phasepoints(fs, E * P)
+    nc = oc.noncoherent(sig, pp, code, [0.0], P, E, fs)
+    assert nc["best_codephase"] == (P - 300) % P
+    assert nc["best_peak"] > 3 * nc["second_peak"] / 2
