@@ -109,7 +109,8 @@ void launch_select(const RowOut* rows, const PeakSlot* slots, int n_ant, int n_p
 bool big_supported(size_t P, size_t N);
 // W_P^i (i < 512) then W_P^{512 i} (i < ceil(P / 512)).
 std::vector<float2> big_twiddles(size_t P);
-// rows [r0, r1) (ordering of launch_mul_rows): Y holds (r1 - r0) * P complex values
+// rows [r0, r1) of the xrow-major order I = xrow * n_prn + prn (rows sharing a forward spectrum
+// are consecutive); results land at the usual row index; Y holds (r1 - r0) * P complex values
 void launch_big_rows(const float2* X, const float2* rep, const float2* tw, const float2* twb,
                      int R0, int n_prn, int n_bin, long long r0, long long r1, float2* Y,
                      unsigned long long* rowkey, int* rownan, RowOut* rows, hipStream_t st);

@@ -122,6 +122,17 @@ __device__ __forceinline__ float block_max(float v, float* sp) {
   return v;
 }
 
+// Long rows are processed in xrow-major order, I = xrow * n_prn + prn with xrow =
+// (ant * n_bin + bin) * 2 + h, so consecutive rows share their forward spectrum in the caches;
+// this maps I to the row index ((ant * n_prn + prn) * n_bin + bin) * 2 + h.
+__device__ __forceinline__ long long big_row(long long I, int n_prn, int n_bin) {
+  const long long xrow = I / n_prn;
+  const int prn = static_cast<int>(I - xrow * n_prn);
+  const long long ant = xrow / (2 * n_bin);
+  const int bh = static_cast<int>(xrow - ant * 2 * n_bin);
+  return ((ant * n_prn + prn) * n_bin) * 2 + bh;
+}
+
 __device__ __forceinline__ int ring_dist(int a, int b, int P) {
   int d = a > b ? a - b : b - a;
   return d < P - d ? d : P - d;
@@ -822,8 +833,8 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
                   accn += p;
                   (slot < 31 ? ms : es).add(p, idx);
                 });
-      row_finish_key<T>(ms, es, accn != accn, red, k0, SUB, a.rowkey + a.r0 + rq,
-                        a.rownan + a.r0 + rq);
+      const long long r = big_row(a.r0 + rq, a.n_prn, a.n_bin);
+      row_finish_key<T>(ms, es, accn != accn, red, k0, SUB, a.rowkey + r, a.rownan + r);
     }
   }
 }
@@ -1235,14 +1246,11 @@ __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
   if (gid >= total) return;
   const long long rr = gid / P1;
   const int n1 = static_cast<int>(gid - rr * P1);
-  const long long r = r0 + rr;
-  const int h = static_cast<int>(r & 1);
-  const long long q = r >> 1;  // (ant*n_prn + prn)*n_bin + bin
-  const int bin = static_cast<int>(q % n_bin);
-  const long long ap = q / n_bin;
-  const int prn = static_cast<int>(ap % n_prn);
-  const int ant = static_cast<int>(ap / n_prn);
-  const float2* x = X + ((static_cast<long long>(ant) * n_bin + bin) * 2 + h) * P;
+  const long long I = r0 + rr;  // xrow-major order (big_row)
+  const long long xrow = I / n_prn;
+  const int prn = static_cast<int>(I - xrow * n_prn);
+  const int h = static_cast<int>(xrow & 1);
+  const float2* x = X + xrow * P;
   const float2* c = rep + (static_cast<long long>(prn) * 2 + h) * P;
   float2 v[R0];
 #pragma unroll
@@ -1256,6 +1264,7 @@ __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
     y[static_cast<long long>(k0) * P1 + n1] = cmul(v[k0], cmul(twb[e & 511], twb[512 + (e >> 9)]));
   }
   if (n1 == 0) {
+    const long long r = big_row(I, n_prn, n_bin);
     rowkey[r] = 0ull;
     rownan[r] = 0;
   }
@@ -1263,10 +1272,11 @@ __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
 
 // Row keys of long rows -> RowOut (p2 is filled later from the winning row only).
 __global__ void k_big_rowout(const unsigned long long* __restrict__ rowkey,
-                             const int* __restrict__ rownan, long long n_rows,
-                             RowOut* __restrict__ rows) {
-  const long long r = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-  if (r >= n_rows) return;
+                             const int* __restrict__ rownan, long long r0, long long n_rows,
+                             int n_prn, int n_bin, RowOut* __restrict__ rows) {
+  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n_rows) return;
+  const long long r = big_row(r0 + i, n_prn, n_bin);
   const unsigned long long k = rowkey[r];
   RowOut o;
   o.p1 = __uint_as_float(static_cast<unsigned int>(k >> 32));
@@ -1554,6 +1564,8 @@ void launch_big_rows(const float2* X, const float2* rep, const float2* tw, const
   fused::RowArgs a{};
   a.tw = tw;
   a.cw = 0;
+  a.n_prn = n_prn;
+  a.n_bin = n_bin;
   a.Y = Y;
   a.n_sub = nr * R0;
   a.r0 = r0;
@@ -1565,8 +1577,8 @@ void launch_big_rows(const float2* X, const float2* rep, const float2* tw, const
   else
     hipLaunchKernelGGL((fused::k_rows<0, 10>), dim3(blocks), dim3(fused::T), 0, st, a);
   TGA_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_big_rowout, dim3(grid_for(nr, 256)), dim3(256), 0, st, rowkey + r0,
-                     rownan + r0, nr, rows + r0);
+  hipLaunchKernelGGL(k_big_rowout, dim3(grid_for(nr, 256)), dim3(256), 0, st, rowkey, rownan,
+                     r0, nr, n_prn, n_bin, rows);
   TGA_HIP(hipGetLastError());
 }
 
