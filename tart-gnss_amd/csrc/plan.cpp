@@ -42,6 +42,9 @@ Plan::~Plan() {
     if (t.start) (void)hipEventDestroy(t.start);
     if (t.stop) (void)hipEventDestroy(t.stop);
   }
+  for (hipEvent_t e : {ev_fwd, ev_pre[0], ev_pre[1], ev_rows[0], ev_rows[1]})
+    if (e) (void)hipEventDestroy(e);
+  if (side) (void)hipStreamDestroy(side);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -223,7 +226,12 @@ void Plan::build(const tga_plan_desc& d) {
       // sub-rows of at most ~2 GiB of rows at a time
       const long long cap = std::max<long long>(1, (256LL << 20) / static_cast<long long>(M));
       g.chunk_rows = std::min<long long>(cap, g.n_rows);
-      g.d_Y.alloc(static_cast<size_t>(g.chunk_rows) * M * sizeof(float2));
+      for (auto& y : g.d_Y) y.alloc(static_cast<size_t>(g.chunk_rows) * M * sizeof(float2));
+      if (!side) {
+        TGA_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        for (hipEvent_t* e : {&ev_fwd, &ev_pre[0], &ev_pre[1], &ev_rows[0], &ev_rows[1]})
+          TGA_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+      }
       g.d_rowkey.alloc(static_cast<size_t>(g.n_rows) * sizeof(unsigned long long));
       g.d_rownan.alloc(static_cast<size_t>(g.n_rows) * sizeof(int));
       const size_t nw = static_cast<size_t>(n_ant) * n_prn;
@@ -372,12 +380,23 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
       launch_fused_rows(g.X, g.d_rep.as<float2>(), g.d_tw.as<float2>(), n_ant, g.n_bin, g.n_prn,
                         g.cw, g.d_rows.as<RowOut>(), st, g.rows_cfg);
     } else if (g.big) {
-      for (long long r0 = 0; r0 < g.n_rows; r0 += g.chunk_rows) {
+      // the pre-pass of chunk c + 1 (side stream) overlaps the transforms of chunk c
+      TGA_HIP(hipEventRecord(ev_fwd, st));
+      TGA_HIP(hipStreamWaitEvent(side, ev_fwd, 0));
+      int c = 0;
+      for (long long r0 = 0; r0 < g.n_rows; r0 += g.chunk_rows, ++c) {
         const long long r1 = std::min(g.n_rows, r0 + g.chunk_rows);
-        launch_big_rows(g.X, g.d_rep.as<float2>(), g.d_tw.as<float2>(), g.d_twb.as<float2>(),
-                        g.big_r0, g.n_prn, g.n_bin, r0, r1, g.d_Y.as<float2>(),
-                        g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(),
-                        g.d_rows.as<RowOut>(), st);
+        const int b = c & 1;
+        if (c >= 2) TGA_HIP(hipStreamWaitEvent(side, ev_rows[b], 0));
+        launch_big_pre(g.X, g.d_rep.as<float2>(), g.d_twb.as<float2>(), g.big_r0, g.n_prn,
+                       g.n_bin, r0, r1, g.d_Y[b].as<float2>(),
+                       g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(), side);
+        TGA_HIP(hipEventRecord(ev_pre[b], side));
+        TGA_HIP(hipStreamWaitEvent(st, ev_pre[b], 0));
+        launch_big_sub(g.d_tw.as<float2>(), g.big_r0, g.n_prn, g.n_bin, r0, r1,
+                       g.d_Y[b].as<float2>(), g.d_rowkey.as<unsigned long long>(),
+                       g.d_rownan.as<int>(), g.d_rows.as<RowOut>(), st);
+        TGA_HIP(hipEventRecord(ev_rows[b], st));
       }
     } else {
       const FftPlan* fp = fft_plan(g.M);

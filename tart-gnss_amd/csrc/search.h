@@ -109,11 +109,16 @@ void launch_select(const RowOut* rows, const PeakSlot* slots, int n_ant, int n_p
 bool big_supported(size_t P, size_t N);
 // W_P^i (i < 512) then W_P^{512 i} (i < ceil(P / 512)).
 std::vector<float2> big_twiddles(size_t P);
-// rows [r0, r1) of the xrow-major order I = xrow * n_prn + prn (rows sharing a forward spectrum
-// are consecutive); results land at the usual row index; Y holds (r1 - r0) * P complex values
-void launch_big_rows(const float2* X, const float2* rep, const float2* tw, const float2* twb,
-                     int R0, int n_prn, int n_bin, long long r0, long long r1, float2* Y,
-                     unsigned long long* rowkey, int* rownan, RowOut* rows, hipStream_t st);
+// Rows [r0, r1) of the xrow-major order I = xrow * n_prn + prn (rows sharing a forward spectrum
+// are consecutive); results land at the usual row index.  launch_big_pre writes the twiddled
+// sub-rows of the rows to Y ((r1 - r0) * P complex values) and clears their keys;
+// launch_big_sub transforms them and writes the rows' RowOut.
+void launch_big_pre(const float2* X, const float2* rep, const float2* twb, int R0, int n_prn,
+                    int n_bin, long long r0, long long r1, float2* Y, unsigned long long* rowkey,
+                    int* rownan, hipStream_t st);
+void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, long long r0, long long r1,
+                    const float2* Y, unsigned long long* rowkey, int* rownan, RowOut* rows,
+                    hipStream_t st);
 // second peaks of the winning rows; Z, Zs hold n_ant * n_prn * M complex values
 void launch_big_second(const float2* X, const float2* rep, const PeakSlot* slots, int M, int P,
                        int cw, int n_ant, int n_prn, int n_bin, const int* win, float gain,

@@ -1547,9 +1547,9 @@ std::vector<float2> big_twiddles(size_t P) {
   return t;
 }
 
-void launch_big_rows(const float2* X, const float2* rep, const float2* tw, const float2* twb,
-                     int R0, int n_prn, int n_bin, long long r0, long long r1, float2* Y,
-                     unsigned long long* rowkey, int* rownan, RowOut* rows, hipStream_t st) {
+void launch_big_pre(const float2* X, const float2* rep, const float2* twb, int R0, int n_prn,
+                    int n_bin, long long r0, long long r1, float2* Y, unsigned long long* rowkey,
+                    int* rownan, hipStream_t st) {
   const long long nr = r1 - r0;
   const long long total = nr * kFusedP;
   if (R0 == 4)
@@ -1561,6 +1561,12 @@ void launch_big_rows(const float2* X, const float2* rep, const float2* tw, const
   else
     fail(TGA_ERR_UNSUPPORTED, "long-period split supports 4 and 10 sub-rows");
   TGA_HIP(hipGetLastError());
+}
+
+void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, long long r0, long long r1,
+                    const float2* Y, unsigned long long* rowkey, int* rownan, RowOut* rows,
+                    hipStream_t st) {
+  const long long nr = r1 - r0;
   fused::RowArgs a{};
   a.tw = tw;
   a.cw = 0;

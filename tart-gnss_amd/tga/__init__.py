@@ -17,6 +17,7 @@ no CPU fallback: if libtga.so or a gfx950 device is missing, calls raise.
   Plan.acquire_all_json / Plan.peaks_json / format_f64
                        CombinedOutput JSON (lib.rs:31-45, main.rs:242-255, 893)
   rfi                  rfi.rs:142-184
+  read_observation     Observation::from_hdf5 (observation.rs:48-92), host-side HDF5 reader
 """
 from ._lib import (ACR_TABLES, INPUT_1BIT, INPUT_2BIT, SYSTEM_IDS, TgaError, lib)  # noqa: F401
 from .api import (AcquisitionResult, CorrelationPeak, Plan, SystemSearch,  # noqa: F401
@@ -26,3 +27,4 @@ from .api import (AcquisitionResult, CorrelationPeak, Plan, SystemSearch,  # noq
                   correlate_code, device_count, e1c_code_resampled, estimate_cn0, format_f64,
                   gold_code,
                   l1c_code_resampled, qzss_gold_code, rfi, sbas_gold_code)
+from .observation import Observation, read_observation  # noqa: F401,E402
