@@ -133,7 +133,7 @@ void Plan::build(const tga_plan_desc& d) {
       g->nc = nc;
       g->fused = nc > 0 ||
                  (fold && fused_supported(P, N, static_cast<int>(std::ceil(fs / 1.023e6))));
-      g->big = fold && !g->fused && big_supported(P, N);
+      g->big = nc == 0 && !g->fused && big_supported(P, N, fold);
       g->big_r0 = g->big ? static_cast<int>(P / kFusedP) : 0;
       g->M = fold ? P : N;
     }
@@ -251,7 +251,8 @@ void Plan::build(const tga_plan_desc& d) {
     // spectrum once (M complex f32 each)
     const double elem = g.fp16 ? 4.0 : 8.0;  // bytes per complex value
     // non-coherent rows: one (X, R) pair per epoch, half as many rows (one hypothesis)
-    const double per_row = g.nc > 0 ? 0.5 * g.nc : 1.0;
+    // unfolded split rows: only h = 0 is transformed (the hypotheses coincide)
+    const double per_row = g.nc > 0 ? 0.5 * g.nc : (g.big && !g.folded) ? 0.5 : 1.0;
     g.row_bytes_alg =
         static_cast<double>(g.n_rows) * per_row * 2.0 * static_cast<double>(M) * elem;
     g.fwd_bytes_alg = static_cast<double>(g.n_xrow) * static_cast<double>(M) * elem;
@@ -384,16 +385,18 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
       TGA_HIP(hipEventRecord(ev_fwd, st));
       TGA_HIP(hipStreamWaitEvent(side, ev_fwd, 0));
       int c = 0;
-      for (long long r0 = 0; r0 < g.n_rows; r0 += g.chunk_rows, ++c) {
-        const long long r1 = std::min(g.n_rows, r0 + g.chunk_rows);
+      const int hs = g.folded ? 1 : 2;
+      const long long n_enum = g.n_rows / hs;
+      for (long long r0 = 0; r0 < n_enum; r0 += g.chunk_rows, ++c) {
+        const long long r1 = std::min(n_enum, r0 + g.chunk_rows);
         const int b = c & 1;
         if (c >= 2) TGA_HIP(hipStreamWaitEvent(side, ev_rows[b], 0));
         launch_big_pre(g.X, g.d_rep.as<float2>(), g.d_twb.as<float2>(), g.big_r0, g.n_prn,
-                       g.n_bin, r0, r1, g.d_Y[b].as<float2>(),
+                       g.n_bin, hs, r0, r1, g.d_Y[b].as<float2>(),
                        g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(), side);
         TGA_HIP(hipEventRecord(ev_pre[b], side));
         TGA_HIP(hipStreamWaitEvent(st, ev_pre[b], 0));
-        launch_big_sub(g.d_tw.as<float2>(), g.big_r0, g.n_prn, g.n_bin, r0, r1,
+        launch_big_sub(g.d_tw.as<float2>(), g.big_r0, g.n_prn, g.n_bin, hs, r0, r1,
                        g.d_Y[b].as<float2>(), g.d_rowkey.as<unsigned long long>(),
                        g.d_rownan.as<int>(), g.d_rows.as<RowOut>(), st);
         TGA_HIP(hipEventRecord(ev_rows[b], st));
@@ -422,7 +425,7 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
     if (g.big)
       launch_big_second(g.X, g.d_rep.as<float2>(), g.d_slots.as<PeakSlot>(),
                         static_cast<int>(g.M), static_cast<int>(g.P), g.cw, n_ant, g.n_prn,
-                        g.n_bin, g.d_win.as<int>(), gain, g.scale, g.d_Zw.as<float2>(),
+                        g.n_bin, g.folded, g.d_win.as<int>(), gain, g.scale, g.d_Zw.as<float2>(),
                         g.d_Zws.as<float2>(), d_peaks.as<tga_peak>(), st);
   }
   time_end(2, st, 0.0);

@@ -106,22 +106,26 @@ void launch_select(const RowOut* rows, const PeakSlot* slots, int n_ant, int n_p
 // Folded rows of length P are transformed as R0 twiddled sub-rows of 16368 points by the fused
 // LDS kernel; each row keeps only its argmax, and the second peak comes from a recomputation of
 // the winning row of every (antenna, PRN) pair.
-bool big_supported(size_t P, size_t N);
+// folded rows (N = 2P) or unfolded single-period rows (N = P, a 10 ms B1C / L1C record)
+bool big_supported(size_t P, size_t N, bool folded);
 // W_P^i (i < 512) then W_P^{512 i} (i < ceil(P / 512)).
 std::vector<float2> big_twiddles(size_t P);
 // Rows [r0, r1) of the xrow-major order I = xrow * n_prn + prn (rows sharing a forward spectrum
 // are consecutive); results land at the usual row index.  launch_big_pre writes the twiddled
 // sub-rows of the rows to Y ((r1 - r0) * P complex values) and clears their keys;
 // launch_big_sub transforms them and writes the rows' RowOut.
+// hs = 1: every (bin, h) row; hs = 2: unfolded rows, h = 0 only (the [c,-c] hypothesis equals
+// [c,c] when N <= P, correlate.rs:53-56, and gets a row that never wins the selection).
 void launch_big_pre(const float2* X, const float2* rep, const float2* twb, int R0, int n_prn,
-                    int n_bin, long long r0, long long r1, float2* Y, unsigned long long* rowkey,
-                    int* rownan, hipStream_t st);
-void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, long long r0, long long r1,
-                    const float2* Y, unsigned long long* rowkey, int* rownan, RowOut* rows,
-                    hipStream_t st);
+                    int n_bin, int hs, long long r0, long long r1, float2* Y,
+                    unsigned long long* rowkey, int* rownan, hipStream_t st);
+void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int hs, long long r0,
+                    long long r1, const float2* Y, unsigned long long* rowkey, int* rownan,
+                    RowOut* rows, hipStream_t st);
 // second peaks of the winning rows; Z, Zs hold n_ant * n_prn * M complex values
 void launch_big_second(const float2* X, const float2* rep, const PeakSlot* slots, int M, int P,
-                       int cw, int n_ant, int n_prn, int n_bin, const int* win, float gain,
-                       float scale, float2* Z, float2* Zs, tga_peak* out, hipStream_t st);
+                       int cw, int n_ant, int n_prn, int n_bin, bool folded, const int* win,
+                       float gain, float scale, float2* Z, float2* Zs, tga_peak* out,
+                       hipStream_t st);
 
 }  // namespace tga

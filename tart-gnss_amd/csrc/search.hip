@@ -125,9 +125,10 @@ __device__ __forceinline__ float block_max(float v, float* sp) {
 // Long rows are processed in xrow-major order, I = xrow * n_prn + prn with xrow =
 // (ant * n_bin + bin) * 2 + h, so consecutive rows share their forward spectrum in the caches;
 // this maps I to the row index ((ant * n_prn + prn) * n_bin + bin) * 2 + h.
-__device__ __forceinline__ long long big_row(long long I, int n_prn, int n_bin) {
-  const long long xrow = I / n_prn;
-  const int prn = static_cast<int>(I - xrow * n_prn);
+// hs = 2 enumerates h = 0 only (unfolded rows, whose two hypotheses coincide).
+__device__ __forceinline__ long long big_row(long long I, int n_prn, int n_bin, int hs = 1) {
+  const long long xrow = (I / n_prn) * hs;
+  const int prn = static_cast<int>(I - (I / n_prn) * n_prn);
   const long long ant = xrow / (2 * n_bin);
   const int bh = static_cast<int>(xrow - ant * 2 * n_bin);
   return ((ant * n_prn + prn) * n_bin) * 2 + bh;
@@ -352,6 +353,7 @@ struct RowArgs {
   unsigned long long* rowkey;
   int* rownan;
   int epochs;  // non-coherent form: one-period epochs per (antenna, bin)
+  int hs;      // sub-row mode: 2 when only h = 0 rows are enumerated (unfolded rows)
 };
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -833,7 +835,7 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
                   accn += p;
                   (slot < 31 ? ms : es).add(p, idx);
                 });
-      const long long r = big_row(a.r0 + rq, a.n_prn, a.n_bin);
+      const long long r = big_row(a.r0 + rq, a.n_prn, a.n_bin, a.hs);
       row_finish_key<T>(ms, es, accn != accn, red, k0, SUB, a.rowkey + r, a.rownan + r);
     }
   }
@@ -1236,7 +1238,7 @@ template <int R0>
 __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
                                                  const float2* __restrict__ rep,
                                                  const float2* __restrict__ twb, int n_prn,
-                                                 int n_bin, long long r0, long long total,
+                                                 int n_bin, int hs, long long r0, long long total,
                                                  float2* __restrict__ Y,
                                                  unsigned long long* __restrict__ rowkey,
                                                  int* __restrict__ rownan) {
@@ -1247,10 +1249,10 @@ __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
   const long long rr = gid / P1;
   const int n1 = static_cast<int>(gid - rr * P1);
   const long long I = r0 + rr;  // xrow-major order (big_row)
-  const long long xrow = I / n_prn;
-  const int prn = static_cast<int>(I - xrow * n_prn);
+  const long long xrow = (I / n_prn) * hs;
+  const int prn = static_cast<int>(I - (I / n_prn) * n_prn);
   const int h = static_cast<int>(xrow & 1);
-  const float2* x = X + xrow * P;
+  const float2* x = X + (hs == 1 ? xrow : xrow >> 1) * P;  // unfolded: one spectrum per bin
   const float2* c = rep + (static_cast<long long>(prn) * 2 + h) * P;
   float2 v[R0];
 #pragma unroll
@@ -1264,7 +1266,7 @@ __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
     y[static_cast<long long>(k0) * P1 + n1] = cmul(v[k0], cmul(twb[e & 511], twb[512 + (e >> 9)]));
   }
   if (n1 == 0) {
-    const long long r = big_row(I, n_prn, n_bin);
+    const long long r = big_row(I, n_prn, n_bin, hs);
     rowkey[r] = 0ull;
     rownan[r] = 0;
   }
@@ -1273,10 +1275,10 @@ __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
 // Row keys of long rows -> RowOut (p2 is filled later from the winning row only).
 __global__ void k_big_rowout(const unsigned long long* __restrict__ rowkey,
                              const int* __restrict__ rownan, long long r0, long long n_rows,
-                             int n_prn, int n_bin, RowOut* __restrict__ rows) {
+                             int n_prn, int n_bin, int hs, RowOut* __restrict__ rows) {
   const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
   if (i >= n_rows) return;
-  const long long r = big_row(r0 + i, n_prn, n_bin);
+  const long long r = big_row(r0 + i, n_prn, n_bin, hs);
   const unsigned long long k = rowkey[r];
   RowOut o;
   o.p1 = __uint_as_float(static_cast<unsigned int>(k >> 32));
@@ -1284,12 +1286,13 @@ __global__ void k_big_rowout(const unsigned long long* __restrict__ rowkey,
   o.i1 = static_cast<int32_t>(k & 0xffffffffu);
   o.nan = rownan[r];
   rows[r] = o;
+  if (hs == 2) rows[r + 1] = RowOut{0.0f, -1.0f, 0, 0};  // the coinciding hypothesis never wins
 }
 
 // Z[w][i] = conj(X) R of the winning row win[w] of (antenna, PRN) pair w.
 __global__ void k_mul_win(const float2* __restrict__ X, const float2* __restrict__ rep, int M,
-                          int n_prn, int n_bin, const int* __restrict__ win, long long total,
-                          float2* __restrict__ Z) {
+                          int n_prn, int n_bin, int folded, const int* __restrict__ win,
+                          long long total, float2* __restrict__ Z) {
   const long long gid = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
   if (gid >= total) return;
   const long long w = gid / M;
@@ -1298,7 +1301,7 @@ __global__ void k_mul_win(const float2* __restrict__ X, const float2* __restrict
   const int prn = static_cast<int>(w - static_cast<long long>(ant) * n_prn);
   const int bh = win[w];  // bin * 2 + h
   const long long xrow = static_cast<long long>(ant) * n_bin * 2 + bh;
-  Z[gid] = cconjmul(X[xrow * M + i], rep[(static_cast<long long>(prn) * 2 + (bh & 1)) * M + i]);
+  Z[gid] = cconjmul(X[(folded ? xrow : xrow >> 1) * M + i], rep[(static_cast<long long>(prn) * 2 + (bh & 1)) * M + i]);
 }
 
 // Second peak of each winning row around the selected code phase: max |z|^2 more than cw
@@ -1531,9 +1534,9 @@ void launch_select(const RowOut* rows, const PeakSlot* slots, int n_ant, int n_p
   TGA_HIP(hipGetLastError());
 }
 
-bool big_supported(size_t P, size_t N) {
-  return N == 2 * P && (P == 4 * static_cast<size_t>(kFusedP) ||
-                        P == 10 * static_cast<size_t>(kFusedP));
+bool big_supported(size_t P, size_t N, bool folded) {
+  return (folded ? N == 2 * P : N == P) && (P == 4 * static_cast<size_t>(kFusedP) ||
+                                             P == 10 * static_cast<size_t>(kFusedP));
 }
 
 std::vector<float2> big_twiddles(size_t P) {
@@ -1548,26 +1551,27 @@ std::vector<float2> big_twiddles(size_t P) {
 }
 
 void launch_big_pre(const float2* X, const float2* rep, const float2* twb, int R0, int n_prn,
-                    int n_bin, long long r0, long long r1, float2* Y, unsigned long long* rowkey,
-                    int* rownan, hipStream_t st) {
+                    int n_bin, int hs, long long r0, long long r1, float2* Y,
+                    unsigned long long* rowkey, int* rownan, hipStream_t st) {
   const long long nr = r1 - r0;
   const long long total = nr * kFusedP;
   if (R0 == 4)
     hipLaunchKernelGGL(k_big_pre<4>, dim3(grid_for(total, 256)), dim3(256), 0, st, X, rep, twb,
-                       n_prn, n_bin, r0, total, Y, rowkey, rownan);
+                       n_prn, n_bin, hs, r0, total, Y, rowkey, rownan);
   else if (R0 == 10)
     hipLaunchKernelGGL(k_big_pre<10>, dim3(grid_for(total, 256)), dim3(256), 0, st, X, rep, twb,
-                       n_prn, n_bin, r0, total, Y, rowkey, rownan);
+                       n_prn, n_bin, hs, r0, total, Y, rowkey, rownan);
   else
     fail(TGA_ERR_UNSUPPORTED, "long-period split supports 4 and 10 sub-rows");
   TGA_HIP(hipGetLastError());
 }
 
-void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, long long r0, long long r1,
-                    const float2* Y, unsigned long long* rowkey, int* rownan, RowOut* rows,
-                    hipStream_t st) {
+void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int hs, long long r0,
+                    long long r1, const float2* Y, unsigned long long* rowkey, int* rownan,
+                    RowOut* rows, hipStream_t st) {
   const long long nr = r1 - r0;
   fused::RowArgs a{};
+  a.hs = hs;
   a.tw = tw;
   a.cw = 0;
   a.n_prn = n_prn;
@@ -1584,17 +1588,18 @@ void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, long long r0
     hipLaunchKernelGGL((fused::k_rows<0, 10>), dim3(blocks), dim3(fused::T), 0, st, a);
   TGA_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_big_rowout, dim3(grid_for(nr, 256)), dim3(256), 0, st, rowkey, rownan,
-                     r0, nr, n_prn, n_bin, rows);
+                     r0, nr, n_prn, n_bin, hs, rows);
   TGA_HIP(hipGetLastError());
 }
 
 void launch_big_second(const float2* X, const float2* rep, const PeakSlot* slots, int M, int P,
-                       int cw, int n_ant, int n_prn, int n_bin, const int* win, float gain,
-                       float scale, float2* Z, float2* Zs, tga_peak* out, hipStream_t st) {
+                       int cw, int n_ant, int n_prn, int n_bin, bool folded, const int* win,
+                       float gain, float scale, float2* Z, float2* Zs, tga_peak* out,
+                       hipStream_t st) {
   const long long nw = static_cast<long long>(n_ant) * n_prn;
   const long long total = nw * M;
   hipLaunchKernelGGL(k_mul_win, dim3(grid_for(total, 256)), dim3(256), 0, st, X, rep, M, n_prn,
-                     n_bin, win, total, Z);
+                     n_bin, folded ? 1 : 0, win, total, Z);
   TGA_HIP(hipGetLastError());
   float2* z = fft_forward(fft_plan(static_cast<size_t>(M)), Z, Zs, static_cast<size_t>(nw), st);
   hipLaunchKernelGGL(k_second_win, dim3(static_cast<unsigned>(nw)), dim3(kRedT), 0, st, z, M, P,

@@ -230,3 +230,16 @@ def test_noncoherent_plan_errors(gpu):
         tga.Plan(FS, 1, 5 * 16368, [tga.SystemSearch("gps", fc, [1], nc_epochs=6)])
     with pytest.raises(tga.TgaError):  # no fused path for E1-C periods
         tga.Plan(FS, 1, 4 * 65472, [tga.SystemSearch("galileo", fc, [1], nc_epochs=2)])
+
+
+def test_split_unfolded_10ms_b1c(gpu):
+    # a 10 ms record holds one B1C period: N = P = 163680, unfolded rows through the split path,
+    # only the [c,c] rows transformed (correlate.rs:53-56: the hypotheses coincide)
+    fs, L = FS, 163680
+    signals = [dict(system="beidou", prn=30, cn0=55.0, doppler=1000.0, delay=12345)]
+    fc = grid(IF, 1000.0, 500.0)
+    systems = [("beidou", [30, 31], fc)]
+    plan, peaks, packed = run_case(fs, L, 2, 1, systems, signals, seed=12)
+    assert "split10x16368" in plan.describe()
+    check_against_oracle(plan, peaks, packed, fs, L, 1, systems)
+    assert int(peaks[0][0, 0]["best_codephase"]) == (163680 - 12345) % 163680
