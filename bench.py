@@ -17,7 +17,8 @@ Other workloads (same JSON line):
 
 A step = one tga_search_device call on an observation already resident in HBM: decode +
 de-mean, forward spectra, all correlation rows, selection, copy of the peaks to the host, and
-the f64 ACR lookups.  Metric: search cells/s = sum(n_ant * n_prn * n_bin * P) / t.
+the f64 ACR lookups.  After the warm-up every step replays the plan's search as one hipGraph
+whose timed stages carry event-record nodes (the kernel times of the roofline).  Metric: search cells/s = sum(n_ant * n_prn * n_bin * P) / t.
 
 Prints one JSON line (see the repository's measurement contract).
 """
@@ -88,9 +89,35 @@ def workload(name, rng):
     raise SystemExit(f"unknown workload {name}")
 
 
-ACR_TABLE = {"gps": "GPS_L1_CA_ACR_TABLE", "sbas": "GPS_L1_CA_ACR_TABLE",
-             "qzss": "GPS_L1_CA_ACR_TABLE", "galileo": "GALILEO_E1C_ACR_TABLE",
-             "beidou": "BEIDOU_B1C_ACR_TABLE", "l1c": "GPS_L1C_ACR_TABLE"}
+class DeviceInput:
+    """The observation in device memory, through the HIP runtime the library itself links (the
+    first one loaded), so the process holds one runtime and searches replay as graphs with
+    event-timed stages."""
+
+    def __init__(self, packed):
+        import ctypes as C
+        import tga
+        tga.lib()
+        self._C = C
+        self.hip = C.CDLL("libamdhip64.so.7")
+        self._p = C.c_void_p()
+        n = C.c_size_t(packed.nbytes)
+        self._check(self.hip.hipMalloc(C.byref(self._p), n), "hipMalloc")
+        self._check(self.hip.hipMemcpy(self._p, packed.ctypes.data_as(C.c_void_p), n, 1),
+                    "hipMemcpy")  # 1 = hipMemcpyHostToDevice
+        self.sync()
+
+    @property
+    def ptr(self):
+        return self._p.value
+
+    def sync(self):
+        self._check(self.hip.hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+    @staticmethod
+    def _check(rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed with HIP error {rc}")
 
 
 def load_traffic(path, kernel):
@@ -125,14 +152,15 @@ def main():
               "shards by antenna with no collective, see the design notes", file=sys.stderr)
         sys.exit(2)
 
-    import torch
+    if args.calib_copy:
+        import torch  # the calibration copy below is a torch kernel; torch loads first
     import tga
     from tga import synth
 
     global np
     import numpy as np
 
-    assert torch.cuda.is_available(), "no GPU visible"
+    assert tga.device_count() > 0, "no GPU visible"
     rng = np.random.default_rng(args.seed)
     w = workload(args.workload, rng)
     n_ant, L, fmt, signals = w["n_ant"], w["L"], w["fmt"], w["signals"]
@@ -148,9 +176,9 @@ def main():
                 for (s, prns, fc), P in zip(w["systems"], periods))
     n_prn = sum(len(prns) for _, prns, _ in w["systems"])
 
-    dev = torch.from_numpy(packed).cuda()
-    torch.cuda.synchronize()
-    ptr = dev.data_ptr()
+    dev = DeviceInput(packed)
+    sync = dev.sync
+    ptr = dev.ptr
     if args.calib_copy:
         src = torch.ones(args.calib_copy * (1 << 20) // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -161,28 +189,20 @@ def main():
 
     def step(pl):
         peaks = pl.search_device(ptr)
-        # ACR C/N0 per (antenna, PRN) on the host in f64 (acquisition.rs:341-357)
-        ok = 0
-        for (s, _, _), arr in zip(w["systems"], peaks):
-            vm = arr["best_peak"].astype(np.float64)
-            vs = arr["second_peak"].astype(np.float64)
-            for a in range(vm.shape[0]):
-                for j in range(vm.shape[1]):
-                    if vs[a, j] > 0 and vm[a, j] > vs[a, j] and tga.estimate_cn0(
-                            (vm[a, j] / vs[a, j]) ** 2, ACR_TABLE[s]) is not None:
-                        ok += 1
-        return peaks, ok
+        # ACR C/N0 per (antenna, PRN) in f64 on the host (acquisition.rs:341-357), natively
+        cn0 = pl.cn0(peaks)
+        return peaks, int(sum(np.count_nonzero(~np.isnan(c)) for c in cn0))
 
     def timed(pl):
         """Warm-up, then exactly args.steps timed steps; returns (seconds, last peaks, cn0)."""
         for _ in range(args.warmup):
             step(pl)
         pl.set_timing(True)
-        torch.cuda.synchronize()
+        sync()
         t0 = time.perf_counter()
         for _ in range(args.steps):
             pk, n_ok = step(pl)
-        torch.cuda.synchronize()
+        sync()
         dt = time.perf_counter() - t0
         return dt, pk, n_ok
 
@@ -271,7 +291,8 @@ def main():
         "cpu_baseline": cpu,
         "extra": {"forward_ms": fwd_ms / max(fwd_n, 1), "select_ms": sel_ms / max(sel_n, 1),
                   "plan_create_s": t_plan, "synth_s": t_synth, "cn0_valid": n_cn0,
-                  "injected_found": f"{found}/{len(signals)}", "fp16_mfma": fp16},
+                  "injected_found": f"{found}/{len(signals)}",
+                  "graph": plan.describe().strip().splitlines()[-1], "fp16_mfma": fp16},
     }
     print(json.dumps(out))
 

@@ -211,6 +211,14 @@ int tga_search(tga_plan* plan, const uint8_t* packed, size_t n_ant, size_t row_b
  * Peaks are copied to `out` in host memory before the call returns. */
 int tga_search_device(tga_plan* plan, const void* d_packed, void* stream, tga_peak* out);
 
+/* hipGraph replay, on by default.  A plan's first search launches its kernels one by one; the
+ * second captures the whole search (kernel nodes only, reading the plan's own input buffer)
+ * into a graph; every later search is one graph launch, preceded by a device-to-device copy
+ * of d_packed into that buffer when d_packed is another address.  While timing is on, and for
+ * plans with a Bluestein transform, searches run eagerly.  enable = 0 drops the graph.
+ * No counterpart in the reference (its searches are rayon loops on the CPU). */
+int tga_plan_set_graph(tga_plan* plan, int32_t enable);
+
 /* Per-kernel timing with HIP events recorded around each launch on the launch stream.
  * kernel: 0 = correlate rows (the dominant kernel), 1 = forward spectra, 2 = select. */
 int tga_plan_set_timing(tga_plan* plan, int32_t enable);
@@ -275,6 +283,37 @@ int tga_acquire_all_json(tga_plan* plan, const uint8_t* packed, size_t n_ant,
 int tga_peaks_json(tga_plan* plan, const tga_peak* peaks, const double* center_freq,
                    const int64_t* antenna_numbers, int32_t cn0, double filter_phase_mad,
                    double filter_freq_mad, char** json);
+
+/* ACR C/N0 of every peak of a previous tga_search for `plan` (acquisition.rs:341-357 per
+ * (antenna, PRN)): cn0[i] for peaks[i], from its system's table, with the reference's gate
+ * (v_s > 0 and v_m > v_s) and lookup; NaN where the reference has None.  Host-only, f64. */
+int tga_peaks_cn0(const tga_plan* plan, const tga_peak* peaks, double* cn0);
+
+/* ---- antenna-relative C/N0 report (--test-antennas) -------------------------------------- */
+
+/* antenna_test::run (antenna_test.rs:105-222) and its JSON (serde_json::to_string_pretty of
+ * AntennaTestOutput, main.rs:848-880), written into a library-owned string.
+ * Rows are the cn0_acr vectors of CombinedOutput in its order (gps, galileo, beidou, sbas,
+ * l1c, qzss; each sorted by label): sv[r] is the label, cn0[r * n_ant + a] the estimate of
+ * antenna column a, NaN where that antenna has none (Option::None); a row of NaN is a
+ * satellite without cn0_acr.  A satellite enters the reference set when some antenna reaches
+ * min_cn0 (44.0 in the reference CLI); each antenna column is scored by the median of its
+ * estimates over that set; columns without any estimate stay unscored (null fields).
+ * Needs no GPU. */
+int tga_antenna_report_json(size_t n_sv, const char* const* sv, const double* cn0,
+                            size_t n_ant, const int64_t* antenna_numbers, double min_cn0,
+                            char** json);
+/* The same report from peaks a previous tga_search returned for `plan`, with the C/N0
+ * estimates forced on (main.rs:181-182) and the MAD filters applied first (main.rs:820-845). */
+int tga_peaks_antenna_report_json(tga_plan* plan, const tga_peak* peaks,
+                                  const double* center_freq, const int64_t* antenna_numbers,
+                                  double filter_phase_mad, double filter_freq_mad,
+                                  double min_cn0, char** json);
+/* Search one packed observation with `plan`, then tga_peaks_antenna_report_json. */
+int tga_acquire_antenna_report_json(tga_plan* plan, const uint8_t* packed, size_t n_ant,
+                                    size_t row_bytes, const double* center_freq,
+                                    const int64_t* antenna_numbers, double filter_phase_mad,
+                                    double filter_freq_mad, double min_cn0, char** json);
 void tga_free_string(char* s);
 /* The JSON text of one f64 (ryu shortest round-trip digits in serde_json's layout). */
 int tga_format_f64(double x, char* buf, size_t cap);

@@ -19,4 +19,6 @@ Pinning (what independently checks each function here):
   stats      -- the reference's median/MAD known answers.
   jsonfmt    -- shortest round-trip float formatting checked against float() round trip
                 and hand-written serde_json/ryu expectations.
+  antenna_test -- the eleven known-answer cases of antenna_test.rs's unit tests
+                (tests/golden/antenna_test_kats.json: inputs and the asserted fields).
 """

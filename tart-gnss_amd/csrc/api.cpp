@@ -80,6 +80,9 @@ int plan_guard_impl(void (*fn)(void*), void* ctx) {
 
 void check_hip(hipError_t e, const char* what, const char* file, int line) {
   if (e == hipSuccess) return;
+  // a failed call also leaves its code as the thread's last error: take it, so the next
+  // launch check of a later, unrelated call does not report it again
+  (void)hipGetLastError();
   const char* fname = std::strrchr(file, '/') ? std::strrchr(file, '/') + 1 : file;
   std::string msg = std::string(what) + ": " + hipGetErrorString(e) + " (" + fname + ":" +
                     std::to_string(line) + ")";

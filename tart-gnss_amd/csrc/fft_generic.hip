@@ -385,6 +385,8 @@ const FftPlan* fft_plan(size_t n) {
 
 size_t fft_plan_size(const FftPlan* plan) { return plan->n; }
 
+bool fft_plan_bluestein(const FftPlan* plan) { return plan->m > 0; }
+
 TwiddleView fft_twiddles(const FftPlan* plan) {
   TwiddleView v;
   v.tw1 = plan->tw.as<float2>();

@@ -74,6 +74,8 @@ const FftPlan* fft_plan(size_t n);  // cached, thread-safe
 float2* fft_forward(const FftPlan* plan, float2* data, float2* scratch, size_t batch,
                     hipStream_t stream);
 size_t fft_plan_size(const FftPlan* plan);
+// Bluestein plans allocate work buffers and drain the stream per call: not capturable.
+bool fft_plan_bluestein(const FftPlan* plan);
 // Twiddle table W_n^e = tw1[e % split] * tw2[e / split] (device pointers) used by kernels.
 struct TwiddleView {
   const float2* tw1;

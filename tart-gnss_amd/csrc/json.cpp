@@ -74,6 +74,17 @@ struct SvRecord {
   double phase_median = 0, phase_mad = 0, freq_median = 0, freq_mad = 0;
 };
 
+void copy_out(const std::string& s, char** json) {
+  char* out = new char[s.size() + 1];
+  std::memcpy(out, s.c_str(), s.size() + 1);
+  *json = out;
+}
+
+struct SystemOut {
+  const char* key;
+  std::vector<SvRecord> recs;
+};
+
 class Writer {
  public:
   std::string out;
@@ -116,14 +127,14 @@ class Writer {
 
 }  // namespace
 
-std::string acquire_all_json(Plan& p, const tga_peak* peaks, const double* center,
-                             const int64_t* antenna_numbers, bool cn0, double filt_phase,
-                             double filt_freq) {
+// The per-SV records of every plan system, in CombinedOutput's field order (lib.rs:33-44),
+// after the MAD filters and the sort by label: the data acquire_all_* hands to the JSON
+// writer and to antenna_test::run.
+std::vector<SystemOut> collect_outputs(Plan& p, const tga_peak* peaks, const double* center,
+                                       bool cn0, double filt_phase, double filt_freq) {
   const int order[TGA_NUM_SYSTEMS] = {TGA_SYS_GPS, TGA_SYS_GALILEO, TGA_SYS_BEIDOU,
                                       TGA_SYS_SBAS, TGA_SYS_L1C, TGA_SYS_QZSS};
-  Writer w;
-  w.raw("{");
-  bool first_sys = true;
+  std::vector<SystemOut> outs;
   for (int sysid : order) {
     int si = -1;
     for (size_t i = 0; i < p.systems.size(); ++i)
@@ -136,7 +147,8 @@ std::string acquire_all_json(Plan& p, const tga_peak* peaks, const double* cente
     const Group& g = p.groups[e.group];
     const SystemInfo& info = system_info(sysid);
     const double P = static_cast<double>(g.P);
-    std::vector<SvRecord> recs;
+    SystemOut so;
+    so.key = info.key;
     for (size_t j = 0; j < e.prns.size(); ++j) {
       SvRecord r;
       char lab[32];
@@ -172,15 +184,27 @@ std::string acquire_all_json(Plan& p, const tga_peak* peaks, const double* cente
       // MAD filters (main.rs:242-255): a missing MAD always passes
       if (r.has_stats && filt_phase >= 0 && !(r.phase_mad <= filt_phase)) continue;
       if (r.has_stats && filt_freq >= 0 && !(r.freq_mad <= filt_freq)) continue;
-      recs.push_back(std::move(r));
+      so.recs.push_back(std::move(r));
     }
-    std::stable_sort(recs.begin(), recs.end(),
+    std::stable_sort(so.recs.begin(), so.recs.end(),
                      [](const SvRecord& a, const SvRecord& b) { return a.sv < b.sv; });
-    w.key(info.key, 1, first_sys);
+    outs.push_back(std::move(so));
+  }
+  return outs;
+}
+
+std::string combined_json(const std::vector<SystemOut>& outs, const int64_t* antenna_numbers,
+                          size_t n_ant) {
+  Writer w;
+  w.raw("{");
+  bool first_sys = true;
+  for (const SystemOut& so : outs) {
+    const std::vector<SvRecord>& recs = so.recs;
+    w.key(so.key, 1, first_sys);
     first_sys = false;
     w.raw("{");
     w.key("antenna_numbers", 2, true);
-    w.array(p.n_ant, 2, [&](size_t i) { w.raw(std::to_string(antenna_numbers[i])); });
+    w.array(n_ant, 2, [&](size_t i) { w.raw(std::to_string(antenna_numbers[i])); });
     w.key("results", 2, false);
     w.array(recs.size(), 2, [&](size_t i) {
       const SvRecord& r = recs[i];
@@ -226,6 +250,109 @@ std::string acquire_all_json(Plan& p, const tga_peak* peaks, const double* cente
   return w.out;
 }
 
+std::string acquire_all_json(Plan& p, const tga_peak* peaks, const double* center,
+                             const int64_t* antenna_numbers, bool cn0, double filt_phase,
+                             double filt_freq) {
+  return combined_json(collect_outputs(p, peaks, center, cn0, filt_phase, filt_freq),
+                       antenna_numbers, static_cast<size_t>(p.n_ant));
+}
+
+// antenna_test::run (antenna_test.rs:105-222) over rows (sv label, per-antenna C/N0 or NaN
+// for "no estimate"), in CombinedOutput order, and the report as serde_json::to_string_pretty
+// writes AntennaTestOutput (main.rs:880): every field present, None as null.
+//   1. reference set: rows with some antenna at C/N0 >= min_cn0 (a row without any estimate,
+//      cn0_acr None, never qualifies);
+//   2. per antenna column, the median of its estimates over the reference set; columns with
+//      no estimate at all are unscored;
+//   3. rank scored antennas by median, descending, ties by antenna number; offset = median -
+//      best median; n_sats = estimates >= min_cn0 in the column; antennas listed by number.
+std::string antenna_report_json(const std::vector<std::string>& svs,
+                                const std::vector<std::vector<double>>& cn0,
+                                const int64_t* antenna_numbers, size_t n_ant, double min_cn0) {
+  std::vector<size_t> cand;
+  for (size_t r = 0; r < svs.size(); ++r) {
+    bool any = false;
+    for (size_t a = 0; a < n_ant; ++a) any = any || (!std::isnan(cn0[r][a]) && cn0[r][a] >= min_cn0);
+    if (any) cand.push_back(r);
+  }
+  struct Stat {
+    int64_t antenna;
+    size_t n_sats;
+    bool scored;
+    double median, offset;
+    size_t rank;
+  };
+  std::vector<Stat> stats;
+  if (!cand.empty()) {
+    std::vector<std::vector<double>> col(n_ant);
+    for (size_t r : cand)
+      for (size_t a = 0; a < n_ant; ++a)
+        if (!std::isnan(cn0[r][a])) col[a].push_back(cn0[r][a]);
+    std::vector<std::pair<size_t, double>> ranked;  // (column, median)
+    for (size_t a = 0; a < n_ant; ++a)
+      if (!col[a].empty()) ranked.emplace_back(a, median(col[a]));
+    std::stable_sort(ranked.begin(), ranked.end(), [&](const auto& x, const auto& y) {
+      if (x.second != y.second) return x.second > y.second;
+      return antenna_numbers[x.first] < antenna_numbers[y.first];
+    });
+    const double best = ranked[0].second;
+    for (size_t a = 0; a < n_ant; ++a) {
+      Stat s{antenna_numbers[a], 0, false, 0.0, 0.0, 0};
+      for (double c : col[a]) s.n_sats += c >= min_cn0;
+      stats.push_back(s);
+    }
+    for (size_t k = 0; k < ranked.size(); ++k) {
+      Stat& s = stats[ranked[k].first];
+      s.scored = true;
+      s.median = ranked[k].second;
+      s.offset = ranked[k].second - best;
+      s.rank = k + 1;
+    }
+    std::stable_sort(stats.begin(), stats.end(),
+                     [](const Stat& x, const Stat& y) { return x.antenna < y.antenna; });
+  }
+  Writer w;
+  w.raw("{");
+  w.key("antenna_numbers", 1, true);
+  w.array(n_ant, 1, [&](size_t i) { w.raw(std::to_string(antenna_numbers[i])); });
+  w.key("min_cn0_db_hz", 1, false);
+  w.raw(json_f64(min_cn0));
+  w.key("n_reference_satellites", 1, false);
+  w.raw(std::to_string(cand.size()));
+  w.key("reference_satellites", 1, false);
+  w.array(cand.size(), 1, [&](size_t i) { w.str(svs[cand[i]]); });
+  w.key("antennas", 1, false);
+  w.array(stats.size(), 1, [&](size_t i) {
+    const Stat& s = stats[i];
+    w.raw("{");
+    w.key("antenna", 3, true);
+    w.raw(std::to_string(s.antenna));
+    w.key("n_sats", 3, false);
+    w.raw(std::to_string(s.n_sats));
+    w.key("median_cn0_db_hz", 3, false);
+    w.raw(s.scored ? json_f64(s.median) : std::string("null"));
+    w.key("offset_db", 3, false);
+    w.raw(s.scored ? json_f64(s.offset) : std::string("null"));
+    w.key("rank", 3, false);
+    w.raw(s.scored ? std::to_string(s.rank) : std::string("null"));
+    w.nl(2);
+    w.raw("}");
+  });
+  w.key("per_satellite", 1, false);
+  w.array(cand.size(), 1, [&](size_t i) {
+    w.raw("{");
+    w.key("sv", 3, true);
+    w.str(svs[cand[i]]);
+    w.key("cn0_db_hz", 3, false);
+    w.array(n_ant, 3, [&](size_t a) { w.raw(json_f64(cn0[cand[i]][a])); });
+    w.nl(2);
+    w.raw("}");
+  });
+  w.nl(0);
+  w.raw("}");
+  return w.out;
+}
+
 }  // namespace tga
 
 using namespace tga;
@@ -250,13 +377,11 @@ int tga_acquire_all_json(tga_plan* plan, const uint8_t* packed, size_t n_ant, si
     *json = nullptr;
     Plan* p = reinterpret_cast<Plan*>(plan);
     std::vector<tga_peak> peaks(p->n_peaks);
-    if (tga_search(plan, packed, n_ant, row_bytes, peaks.data()) != TGA_OK)
-      fail(TGA_ERR_DEVICE, tga_last_error());
-    const std::string s = acquire_all_json(*p, peaks.data(), center_freq, antenna_numbers,
-                                           cn0 != 0, filter_phase_mad, filter_freq_mad);
-    char* out = new char[s.size() + 1];
-    std::memcpy(out, s.c_str(), s.size() + 1);
-    *json = out;
+    if (int rc = tga_search(plan, packed, n_ant, row_bytes, peaks.data()); rc != TGA_OK)
+      fail(rc, tga_last_error());
+    copy_out(acquire_all_json(*p, peaks.data(), center_freq, antenna_numbers, cn0 != 0,
+                              filter_phase_mad, filter_freq_mad),
+             json);
   });
 }
 
@@ -268,11 +393,89 @@ int tga_peaks_json(tga_plan* plan, const tga_peak* peaks, const double* center_f
       fail(TGA_ERR_INVALID_ARGUMENT, "null argument");
     *json = nullptr;
     Plan* p = reinterpret_cast<Plan*>(plan);
-    const std::string s = acquire_all_json(*p, peaks, center_freq, antenna_numbers, cn0 != 0,
-                                           filter_phase_mad, filter_freq_mad);
-    char* out = new char[s.size() + 1];
-    std::memcpy(out, s.c_str(), s.size() + 1);
-    *json = out;
+    copy_out(acquire_all_json(*p, peaks, center_freq, antenna_numbers, cn0 != 0,
+                              filter_phase_mad, filter_freq_mad),
+             json);
+  });
+}
+
+int tga_peaks_cn0(const tga_plan* plan, const tga_peak* peaks, double* cn0) {
+  return plan_guard([&] {
+    if (!plan || !peaks || !cn0) fail(TGA_ERR_INVALID_ARGUMENT, "null argument");
+    const Plan* p = reinterpret_cast<const Plan*>(plan);
+    for (const SysEntry& e : p->systems) {
+      const int table = system_info(e.system).acr_table;
+      const size_t n = static_cast<size_t>(p->n_ant) * e.prns.size();
+      for (size_t i = e.peak_offset; i < e.peak_offset + n; ++i) {
+        const double vm = peaks[i].best_peak, vs = peaks[i].second_peak;
+        double c = 0.0;
+        const bool ok = vs > 0.0 && vm > vs && estimate_cn0((vm / vs) * (vm / vs), table, &c);
+        cn0[i] = ok ? c : std::nan("");
+      }
+    }
+  });
+}
+
+int tga_antenna_report_json(size_t n_sv, const char* const* sv, const double* cn0,
+                            size_t n_ant, const int64_t* antenna_numbers, double min_cn0,
+                            char** json) {
+  return plan_guard([&] {
+    if (!json || (n_sv && (!sv || (n_ant && !cn0))) || (n_ant && !antenna_numbers))
+      fail(TGA_ERR_INVALID_ARGUMENT, "null argument");
+    *json = nullptr;
+    std::vector<std::string> svs(n_sv);
+    std::vector<std::vector<double>> rows(n_sv, std::vector<double>(n_ant));
+    for (size_t r = 0; r < n_sv; ++r) {
+      if (!sv[r]) fail(TGA_ERR_INVALID_ARGUMENT, "null sv label");
+      svs[r] = sv[r];
+      for (size_t a = 0; a < n_ant; ++a) rows[r][a] = cn0[r * n_ant + a];
+    }
+    copy_out(antenna_report_json(svs, rows, antenna_numbers, n_ant, min_cn0), json);
+  });
+}
+
+int tga_peaks_antenna_report_json(tga_plan* plan, const tga_peak* peaks,
+                                  const double* center_freq, const int64_t* antenna_numbers,
+                                  double filter_phase_mad, double filter_freq_mad,
+                                  double min_cn0, char** json) {
+  return plan_guard([&] {
+    if (!plan || !peaks || !center_freq || !antenna_numbers || !json)
+      fail(TGA_ERR_INVALID_ARGUMENT, "null argument");
+    *json = nullptr;
+    Plan* p = reinterpret_cast<Plan*>(plan);
+    const size_t n_ant = static_cast<size_t>(p->n_ant);
+    std::vector<std::string> svs;
+    std::vector<std::vector<double>> rows;
+    // --test-antennas forces the C/N0 estimates (main.rs:181-182)
+    for (const SystemOut& so : collect_outputs(*p, peaks, center_freq, true, filter_phase_mad,
+                                               filter_freq_mad))
+      for (const SvRecord& r : so.recs) {
+        if (!r.has_cn0) continue;  // cn0_acr None
+        svs.push_back(r.sv);
+        rows.emplace_back(n_ant);
+        for (size_t a = 0; a < n_ant; ++a)
+          rows.back()[a] = r.cn0_ok[a] ? r.cn0[a] : std::nan("");
+      }
+    copy_out(antenna_report_json(svs, rows, antenna_numbers, n_ant, min_cn0), json);
+  });
+}
+
+int tga_acquire_antenna_report_json(tga_plan* plan, const uint8_t* packed, size_t n_ant,
+                                    size_t row_bytes, const double* center_freq,
+                                    const int64_t* antenna_numbers, double filter_phase_mad,
+                                    double filter_freq_mad, double min_cn0, char** json) {
+  return plan_guard([&] {
+    if (!plan || !packed || !json) fail(TGA_ERR_INVALID_ARGUMENT, "null argument");
+    *json = nullptr;
+    Plan* p = reinterpret_cast<Plan*>(plan);
+    std::vector<tga_peak> peaks(p->n_peaks);
+    if (int rc = tga_search(plan, packed, n_ant, row_bytes, peaks.data()); rc != TGA_OK)
+      fail(rc, tga_last_error());
+    if (int rc = tga_peaks_antenna_report_json(plan, peaks.data(), center_freq,
+                                               antenna_numbers, filter_phase_mad,
+                                               filter_freq_mad, min_cn0, json);
+        rc != TGA_OK)
+      fail(rc, tga_last_error());
   });
 }
 

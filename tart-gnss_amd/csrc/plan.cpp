@@ -19,6 +19,9 @@ namespace tga {
 
 namespace {
 
+// Thrown out of a capture whose HIP runtime refuses external event records.
+struct NoTimedCapture {};
+
 template <class T>
 void upload(DevBuf& b, const std::vector<T>& h, hipStream_t st) {
   b.alloc(h.size() * sizeof(T));
@@ -44,8 +47,64 @@ Plan::~Plan() {
   }
   for (hipEvent_t e : {ev_fwd, ev_pre[0], ev_pre[1], ev_rows[0], ev_rows[1]})
     if (e) (void)hipEventDestroy(e);
+  drop_graph();
   if (side) (void)hipStreamDestroy(side);
   if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Plan::drop_graph() {
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (graph) (void)hipGraphDestroy(graph);
+  gexec = nullptr;
+  graph = nullptr;
+}
+
+void Plan::run(const uint8_t* d_in, hipStream_t st) {
+  if (st == nullptr) st = stream;
+  if (!graph_enabled || !graphable || !warmed || (timing && !timed_graphs)) {
+    search(d_in, st);
+    warmed = true;  // lazily built transform plans now exist: nothing allocates any more
+    return;
+  }
+  if (gexec && graph_timed != timing) drop_graph();
+  if (!gexec) {
+    TGA_HIP(hipStreamSynchronize(stream));
+    graph_slots.fill(false);
+    capturing = true;
+    TGA_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    bool refused = false;
+    try {
+      search(d_input.as<uint8_t>(), stream);
+    } catch (...) {
+      capturing = false;
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
+      try {
+        throw;
+      } catch (const NoTimedCapture&) {
+        refused = true;
+      }
+    }
+    if (refused) {
+      // this runtime records no event nodes in a capture: timed searches run eagerly
+      timed_graphs = false;
+      search(d_in, st);
+      return;
+    }
+    capturing = false;
+    TGA_HIP(hipStreamEndCapture(stream, &graph));
+    TGA_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    graph_timed = timing;
+  }
+  if (d_in != d_input.as<uint8_t>())
+    TGA_HIP(hipMemcpyAsync(d_input.p, d_in, static_cast<size_t>(n_ant) * row_bytes,
+                           hipMemcpyDeviceToDevice, st));
+  TGA_HIP(hipGraphLaunch(gexec, st));
+  ++graph_launches;
+  for (size_t k = 0; k < timers.size(); ++k)
+    if (graph_timed && graph_slots[k]) timers[k].pending = true;
 }
 
 void Plan::build(const tga_plan_desc& d) {
@@ -257,11 +316,17 @@ void Plan::build(const tga_plan_desc& d) {
         static_cast<double>(g.n_rows) * per_row * 2.0 * static_cast<double>(M) * elem;
     g.fwd_bytes_alg = static_cast<double>(g.n_xrow) * static_cast<double>(M) * elem;
   }
+  for (auto& g : groups)
+    if (fft_plan_bluestein(fft_plan(g.M))) graphable = false;
   TGA_HIP(hipStreamSynchronize(stream));
   for (auto& t : timers) {
     TGA_HIP(hipEventCreate(&t.start));
     TGA_HIP(hipEventCreate(&t.stop));
+    // a timer event captured as an event-record node must have been recorded once before
+    TGA_HIP(hipEventRecord(t.start, stream));
+    TGA_HIP(hipEventRecord(t.stop, stream));
   }
+  TGA_HIP(hipStreamSynchronize(stream));
 }
 
 // Rows-kernel autotuner: times every candidate shape once (after one warm-up launch) on the
@@ -306,15 +371,35 @@ void Plan::autotune(Group& g) {
   (void)hipEventDestroy(e1);
 }
 
+// While a graph is being captured the records become event-record nodes of the graph, and
+// every replay of it sets the timers again.
+void Plan::time_record(hipEvent_t ev, int slot, hipStream_t st, const char* what) {
+  const hipError_t e = hipEventRecordWithFlags(
+      ev, st, capturing ? hipEventRecordExternal : hipEventRecordDefault);
+  if (e == hipSuccess) return;
+  (void)hipGetLastError();
+  if (capturing && e == hipErrorInvalidValue) throw NoTimedCapture{};
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  (void)hipStreamIsCapturing(st, &cs);
+  fail(TGA_ERR_DEVICE, std::string("hipEventRecordWithFlags(") + what + " of timer " +
+                           std::to_string(slot) + (capturing ? ", external" : ", default") +
+                           ", stream capture status " + std::to_string(static_cast<int>(cs)) +
+                           "): " + hipGetErrorString(e));
+}
+
 void Plan::time_begin(int slot, hipStream_t st) {
-  if (timing) TGA_HIP(hipEventRecord(timers[slot].start, st));
+  if (!timing) return;
+  time_record(timers[slot].start, slot, st, "start");
 }
 
 void Plan::time_end(int slot, hipStream_t st, double bytes) {
   if (!timing) return;
-  TGA_HIP(hipEventRecord(timers[slot].stop, st));
-  timers[slot].pending = true;
+  time_record(timers[slot].stop, slot, st, "stop");
   timers[slot].bytes = bytes;
+  if (capturing)
+    graph_slots[slot] = true;
+  else
+    timers[slot].pending = true;
 }
 
 void Plan::time_collect() {
@@ -331,8 +416,7 @@ void Plan::time_collect() {
 
 void Plan::search(const uint8_t* d_in, hipStream_t st) {
   if (st == nullptr) st = stream;
-  TGA_HIP(hipMemsetAsync(d_nan.p, 0, sizeof(int), st));
-  launch_ant_sums(d_in, row_bytes, n_ant, L, fmt, d_means.as<double>(), st);
+  launch_ant_sums(d_in, row_bytes, n_ant, L, fmt, d_means.as<double>(), d_nan.as<int>(), st);
   double rows_bytes = 0.0, fwd_bytes = 0.0;
   for (auto& g : groups) {
     rows_bytes += g.row_bytes_alg;
@@ -461,6 +545,11 @@ std::string Plan::describe() const {
                   g.rows_cfg.variant, g.rows_cfg.groups, g.tune_ms[0], g.tune_ms[1]);
     s += buf;
   }
+  s += !graphable         ? "graph=off (Bluestein transform)"
+       : !graph_enabled   ? "graph=off"
+       : gexec            ? "graph=replay launches=" + std::to_string(graph_launches)
+                          : "graph=pending (captured at the second search)";
+  s += timed_graphs ? "\n" : " timed=eager (no event nodes in captures)\n";
   return s;
 }
 
@@ -528,7 +617,7 @@ int tga_search(tga_plan* plan, const uint8_t* packed, size_t n_ant, size_t row_b
                                          std::to_string(p->row_bytes));
     TGA_HIP(hipMemcpyAsync(p->d_input.p, packed, n_ant * row_bytes, hipMemcpyHostToDevice,
                            p->stream));
-    p->search(p->d_input.as<uint8_t>(), p->stream);
+    p->run(p->d_input.as<uint8_t>(), p->stream);
     p->finish(out, p->stream);
   });
 }
@@ -538,7 +627,7 @@ int tga_search_device(tga_plan* plan, const void* d_packed, void* stream, tga_pe
     if (!plan || !d_packed || !out) fail(TGA_ERR_INVALID_ARGUMENT, "null argument");
     Plan* p = reinterpret_cast<Plan*>(plan);
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : p->stream;
-    p->search(static_cast<const uint8_t*>(d_packed), st);
+    p->run(static_cast<const uint8_t*>(d_packed), st);
     p->finish(out, st);
   });
 }
@@ -553,6 +642,16 @@ int tga_plan_set_timing(tga_plan* plan, int32_t enable) {
       t.launches = 0;
       t.pending = false;
     }
+  });
+}
+
+int tga_plan_set_graph(tga_plan* plan, int32_t enable) {
+  return plan_guard([&] {
+    if (!plan) fail(TGA_ERR_INVALID_ARGUMENT, "null plan");
+    Plan* p = reinterpret_cast<Plan*>(plan);
+    TGA_HIP(hipStreamSynchronize(p->stream));
+    p->graph_enabled = enable != 0;
+    if (!p->graph_enabled) p->drop_graph();
   });
 }
 

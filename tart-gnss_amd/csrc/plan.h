@@ -66,12 +66,29 @@ struct Plan {
   hipEvent_t ev_fwd = nullptr, ev_pre[2] = {nullptr, nullptr}, ev_rows[2] = {nullptr, nullptr};
   bool timing = false;
   std::array<Timer, 3> timers;
+  // hipGraph replay: the first search of a plan runs eagerly, the second captures the whole
+  // search (reading d_input) into `gexec`, later ones replay it.  With timing on, the timer
+  // records are captured as event-record nodes (graph_slots: the timers the graph sets); a
+  // change of the timing switch captures again; a runtime without event nodes in captures
+  // runs timed searches eagerly.  Plans with a Bluestein transform (which allocates per
+  // call) stay eager.
+  bool graph_enabled = true, graphable = true, warmed = false;
+  bool capturing = false, graph_timed = false;
+  bool timed_graphs = true;  // cleared when the runtime refuses event nodes in a capture
+  std::array<bool, 3> graph_slots{};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  long long graph_launches = 0;
 
   ~Plan();
   void build(const tga_plan_desc& d);
   void search(const uint8_t* d_in, hipStream_t st);
+  // Enqueue one search of d_in on st: eager, or a copy into d_input plus a graph replay.
+  void run(const uint8_t* d_in, hipStream_t st);
+  void drop_graph();
   void finish(tga_peak* out, hipStream_t st);
   std::string describe() const;
+  void time_record(hipEvent_t ev, int slot, hipStream_t st, const char* what);
   void time_begin(int slot, hipStream_t st);
   void time_end(int slot, hipStream_t st, double bytes);
   void time_collect();

@@ -140,10 +140,14 @@ __device__ __forceinline__ int ring_dist(int a, int b, int P) {
 }
 
 // ---- antenna sums ---------------------------------------------------------------------------
+// Also clears the search's NaN flag (block 0), so a search has no memset between kernels and
+// a captured graph of it holds kernel nodes only.
 __global__ __launch_bounds__(256) void k_ant_sums(const uint8_t* __restrict__ packed,
                                                   size_t row_bytes, long long L, int fmt,
-                                                  double* __restrict__ means) {
+                                                  double* __restrict__ means,
+                                                  int* __restrict__ nan_flag) {
   __shared__ long long red[4];
+  if (nan_flag != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *nan_flag = 0;
   const uint8_t* row = packed + static_cast<size_t>(blockIdx.x) * row_bytes;
   long long acc = 0;
   for (long long n = threadIdx.x; n < L; n += 256) acc += decode_level(row, fmt, n);
@@ -409,18 +413,75 @@ struct Top2 {
   }
 };
 
-// LDS scratch of one row's reduction: per-wave argmax, masked maximum and bad-value flag.
+// Order-preserving map of NaN-free floats onto unsigned ints (larger float, larger image), and
+// its inverse: maxima of images are maxima of the floats, with no float compares.
+__device__ __forceinline__ unsigned int ford(float p) {
+  const unsigned int b = __float_as_uint(p);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float funord(unsigned int u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+// Unsigned maximum over each row of 16 lanes, delivered to every lane of the row, by four DPP
+// moves: swap neighbours (quad_perm 1,0,3,2), swap pairs (quad_perm 2,3,0,1), mirror each half
+// row of 8 (row_half_mirror), mirror the row (row_mirror).  Every lane of the wave must be
+// active: DPP reads the source lane's register whether or not that lane executes.
+__device__ __forceinline__ unsigned int row16_umax(unsigned int v) {
+  const int x = static_cast<int>(v);
+  v = max(v, static_cast<unsigned int>(__builtin_amdgcn_update_dpp(x, x, 0xB1, 0xF, 0xF, false)));
+  const int y = static_cast<int>(v);
+  v = max(v, static_cast<unsigned int>(__builtin_amdgcn_update_dpp(y, y, 0x4E, 0xF, 0xF, false)));
+  const int z = static_cast<int>(v);
+  v = max(v, static_cast<unsigned int>(__builtin_amdgcn_update_dpp(z, z, 0x141, 0xF, 0xF, false)));
+  const int w = static_cast<int>(v);
+  v = max(v, static_cast<unsigned int>(__builtin_amdgcn_update_dpp(w, w, 0x140, 0xF, 0xF, false)));
+  return v;
+}
+
+// Unsigned maximum over the wave (wave-uniform result): rows of 16 by DPP, then the four row
+// maxima read out of lanes 0, 16, 32 and 48.
+__device__ __forceinline__ unsigned int wave_umax(unsigned int v) {
+  v = row16_umax(v);
+  const unsigned int a = static_cast<unsigned int>(__builtin_amdgcn_readlane(static_cast<int>(v), 0));
+  const unsigned int b = static_cast<unsigned int>(__builtin_amdgcn_readlane(static_cast<int>(v), 16));
+  const unsigned int c = static_cast<unsigned int>(__builtin_amdgcn_readlane(static_cast<int>(v), 32));
+  const unsigned int d = static_cast<unsigned int>(__builtin_amdgcn_readlane(static_cast<int>(v), 48));
+  return max(max(a, b), max(c, d));
+}
+
+// Argmax with the last-maximum rule (largest power, then largest index) over the wave, as
+// (power image, 1 + index): two unsigned maxima instead of a 64-bit key.
+__device__ __forceinline__ void wave_argmax(float p, int i, unsigned int& pw, unsigned int& iw) {
+  const unsigned int pk = ford(p);
+  pw = wave_umax(pk);
+  iw = wave_umax(pk == pw ? static_cast<unsigned int>(i + 1) : 0u);
+}
+
+// The same over at most 16 per-wave results held by lanes 0..15 (other lanes hold 0 / 0).
+__device__ __forceinline__ void lanes_argmax(unsigned int pk, unsigned int ik, unsigned int& pw,
+                                             unsigned int& iw) {
+  pw = static_cast<unsigned int>(__builtin_amdgcn_readlane(static_cast<int>(row16_umax(pk)), 0));
+  iw = static_cast<unsigned int>(
+      __builtin_amdgcn_readlane(static_cast<int>(row16_umax(pk == pw ? ik : 0u)), 0));
+}
+
+// LDS scratch of one row's reduction: per-wave argmax (power image, 1 + index), masked maximum
+// (image) and bad-value flag.
 template <int NW_>
 struct RowRed {
-  float p[NW_];
-  int i[NW_];
-  float m[NW_];
+  static_assert(NW_ <= 16, "cross-wave reductions read one wave per lane of a 16-lane row");
+  unsigned int pk[NW_];
+  unsigned int ik[NW_];
+  unsigned int m[NW_];
   int bad[NW_];
 };
 
 // End of one correlation row: block argmax (last maximum), the masked second maximum around
-// it, and the NaN / overflow flag, with two workgroup barriers.  Safe to call once per row in
-// a loop: every LDS slot is read before the barrier that precedes its next write.
+// it, and the NaN / overflow flag, with two workgroup barriers.  Waves reduce with DPP moves;
+// across waves, lane w reads wave w's slot and the same 16-lane reduction combines them, so no
+// thread walks the slots one by one.  Safe to call once per row in a loop: every LDS slot is
+// read before the barrier that precedes its next write.  Called by every thread of the block.
 template <int NT>
 __device__ __forceinline__ void row_finish(const Top2& main_set, const Top2& extra_set,
                                            bool bad, int cw, RowRed<NT / 64>& red, RowOut* out) {
@@ -429,42 +490,35 @@ __device__ __forceinline__ void row_finish(const Top2& main_set, const Top2& ext
   float bp = main_set.best();
   int bi = main_set.index();
   arg_combine(bp, bi, extra_set.best(), extra_set.index());
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    const float q = __shfl_xor(bp, off, 64);
-    const int j = __shfl_xor(bi, off, 64);
-    arg_combine(bp, bi, q, j);
-  }
+  unsigned int pw, iw;
+  wave_argmax(bp, bi, pw, iw);
   const bool wave_bad = __ballot(bad) != 0;
   if (lane == 0) {
-    red.p[wave] = bp;
-    red.i[wave] = bi;
+    red.pk[wave] = pw;
+    red.ik[wave] = iw;
     red.bad[wave] = wave_bad ? 1 : 0;
   }
   lds_barrier();
-  bp = red.p[0];
-  bi = red.i[0];
-  int any_bad = red.bad[0];
-#pragma unroll
-  for (int w = 1; w < NWR; ++w) {
-    arg_combine(bp, bi, red.p[w], red.i[w]);
-    any_bad |= red.bad[w];
-  }
-  float m = fmaxf(main_set.masked(bi, cw), extra_set.masked(bi, cw));
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
-  if (lane == 0) red.m[wave] = m;
+  const bool mine = lane < NWR;
+  lanes_argmax(mine ? red.pk[lane] : 0u, mine ? red.ik[lane] : 0u, pw, iw);
+  const bool any_bad = __ballot(mine && red.bad[mine ? lane : 0] != 0) != 0;
+  bp = funord(pw);
+  bi = static_cast<int>(iw) - 1;
+  const unsigned int mw =
+      wave_umax(ford(fmaxf(main_set.masked(bi, cw), extra_set.masked(bi, cw))));
+  if (lane == 0) red.m[wave] = mw;
   lds_barrier();
-  if (threadIdx.x == 0) {
-    float m2 = red.m[0];
-#pragma unroll
-    for (int w = 1; w < NWR; ++w) m2 = fmaxf(m2, red.m[w]);
-    RowOut o;
-    o.p1 = bp;
-    o.p2 = m2;
-    o.i1 = bi;
-    o.nan = any_bad;
-    *out = o;
+  if (wave == 0) {
+    const unsigned int m2 = static_cast<unsigned int>(
+        __builtin_amdgcn_readlane(static_cast<int>(row16_umax(mine ? red.m[lane] : 0u)), 0));
+    if (lane == 0) {
+      RowOut o;
+      o.p1 = bp;
+      o.p2 = funord(m2);
+      o.i1 = bi;
+      o.nan = any_bad ? 1 : 0;
+      *out = o;
+    }
   }
 }
 
@@ -479,30 +533,23 @@ __device__ __forceinline__ void row_finish_key(const Top2& main_set, const Top2&
   float bp = main_set.best();
   int bi = main_set.index();
   arg_combine(bp, bi, extra_set.best(), extra_set.index());
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    const float q = __shfl_xor(bp, off, 64);
-    const int j = __shfl_xor(bi, off, 64);
-    arg_combine(bp, bi, q, j);
-  }
+  unsigned int pw, iw;
+  wave_argmax(bp, bi, pw, iw);
   const bool wave_bad = __ballot(bad) != 0;
   if (lane == 0) {
-    red.p[wave] = bp;
-    red.i[wave] = bi;
+    red.pk[wave] = pw;
+    red.ik[wave] = iw;
     red.bad[wave] = wave_bad ? 1 : 0;
   }
   lds_barrier();
-  if (threadIdx.x == 0) {
-    int any_bad = red.bad[0];
-    bp = red.p[0];
-    bi = red.i[0];
-#pragma unroll
-    for (int w = 1; w < NWR; ++w) {
-      arg_combine(bp, bi, red.p[w], red.i[w]);
-      any_bad |= red.bad[w];
+  if (wave == 0) {
+    const bool mine = lane < NWR;
+    lanes_argmax(mine ? red.pk[lane] : 0u, mine ? red.ik[lane] : 0u, pw, iw);
+    const bool any_bad = __ballot(mine && red.bad[mine ? lane : 0] != 0) != 0;
+    if (lane == 0) {
+      atomicMax(key, pkey(funord(pw), k0 + r0f * (static_cast<int>(iw) - 1)));
+      if (any_bad) atomicOr(nan, 1);
     }
-    atomicMax(key, pkey(bp, k0 + r0f * bi));
-    if (any_bad) atomicOr(nan, 1);
   }
 }
 
@@ -1384,9 +1431,9 @@ inline unsigned grid_for(long long total, int block) {
 
 // ---- launchers -------------------------------------------------------------------------------
 void launch_ant_sums(const uint8_t* packed, size_t row_bytes, int n_ant, int64_t L, int fmt,
-                     double* means, hipStream_t st) {
+                     double* means, int* nan_flag, hipStream_t st) {
   hipLaunchKernelGGL(k_ant_sums, dim3(n_ant), dim3(256), 0, st, packed, row_bytes,
-                     static_cast<long long>(L), fmt, means);
+                     static_cast<long long>(L), fmt, means, nan_flag);
   TGA_HIP(hipGetLastError());
 }
 

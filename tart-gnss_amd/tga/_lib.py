@@ -91,6 +91,7 @@ PROTOTYPES = {
     "tga_search": (C.c_int, [C.c_void_p, _u8p, C.c_size_t, C.c_size_t, C.POINTER(Peak)]),
     "tga_search_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Peak)]),
     "tga_plan_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
+    "tga_plan_set_graph": (C.c_int, [C.c_void_p, C.c_int32]),
     "tga_plan_get_timing": (C.c_int, [C.c_void_p, C.c_int32, _dp, C.POINTER(C.c_int64), _dp]),
     "tga_plan_kernel_name": (C.c_char_p, [C.c_void_p, C.c_int32]),
     "tga_plan_describe": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
@@ -103,6 +104,17 @@ PROTOTYPES = {
                                        C.c_double, C.POINTER(C.c_void_p)]),
     "tga_peaks_json": (C.c_int, [C.c_void_p, C.POINTER(Peak), _dp, C.POINTER(C.c_int64),
                                  C.c_int32, C.c_double, C.c_double, C.POINTER(C.c_void_p)]),
+    "tga_peaks_cn0": (C.c_int, [C.c_void_p, C.POINTER(Peak), _dp]),
+    "tga_antenna_report_json": (C.c_int, [C.c_size_t, C.POINTER(C.c_char_p), _dp, C.c_size_t,
+                                          C.POINTER(C.c_int64), C.c_double,
+                                          C.POINTER(C.c_void_p)]),
+    "tga_peaks_antenna_report_json": (C.c_int, [C.c_void_p, C.POINTER(Peak), _dp,
+                                                C.POINTER(C.c_int64), C.c_double, C.c_double,
+                                                C.c_double, C.POINTER(C.c_void_p)]),
+    "tga_acquire_antenna_report_json": (C.c_int, [C.c_void_p, _u8p, C.c_size_t, C.c_size_t,
+                                                  _dp, C.POINTER(C.c_int64), C.c_double,
+                                                  C.c_double, C.c_double,
+                                                  C.POINTER(C.c_void_p)]),
     "tga_free_string": (None, [C.c_void_p]),
     "tga_format_f64": (C.c_int, [C.c_double, C.c_char_p, C.c_size_t]),
 }

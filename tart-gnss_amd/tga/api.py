@@ -1,5 +1,7 @@
 """High-level wrappers over the C ABI (see the package docstring for the reference mapping)."""
 import ctypes as C
+import json
+import math
 from dataclasses import dataclass
 
 import numpy as np
@@ -123,6 +125,16 @@ def correlate_code(signal_f32, phasepoints, code_complex, code_complex_alt, fc, 
                                    int(code_period_samples), float(sampling_freq),
                                    C.byref(pk)))
     return CorrelationPeak(pk.best_peak, pk.second_peak, pk.best_freq_idx, pk.best_codephase)
+
+
+def search_frequencies(center_freq, search_band, freq_bin_size=300.0):
+    """The bin grid of acquire_full and its siblings (acquisition.rs:189-198): round(2 band /
+    300 Hz) + 1 bins spread evenly over [center - band, center + band]; a Plan's grid."""
+    x = 2.0 * search_band / freq_bin_size
+    n = math.floor(x) + (1 if x - math.floor(x) >= 0.5 else 0) + 1  # f64::round: half away
+    if n == 1:
+        return [float(center_freq)]
+    return [center_freq - search_band + i * (2.0 * search_band) / (n - 1.0) for i in range(n)]
 
 
 def acquire_single(system, signal_f32, phasepoints, sampling_freq, center_freq, search_band,
@@ -301,6 +313,10 @@ class Plan:
     def set_timing(self, enable=True):
         check(lib().tga_plan_set_timing(self._h, 1 if enable else 0))
 
+    def set_graph(self, enable=True):
+        """hipGraph replay of the whole search from the second search on (default on)."""
+        check(lib().tga_plan_set_graph(self._h, 1 if enable else 0))
+
     def timing(self, kernel=0):
         ms = C.c_double(0)
         n = C.c_int64(0)
@@ -317,37 +333,110 @@ class Plan:
         check(lib().tga_plan_describe(self._h, buf, 4096))
         return buf.value.decode()
 
-    def _json(self, fn, first, center_freq, antenna_numbers, cn0, filter_phase_mad,
-              filter_freq_mad):
+    def _json(self, fn, first, center_freq, antenna_numbers, tail):
         cf = _f64(center_freq if np.ndim(center_freq) else [center_freq] * len(self.systems))
         an = np.ascontiguousarray(antenna_numbers if antenna_numbers is not None
                                   else range(self.n_ant), dtype=np.int64)
         if len(cf) != len(self.systems) or len(an) != self.n_ant:
             raise ValueError("center_freq per system and antenna_numbers per antenna")
         out = C.c_void_p()
-        fpm = -1.0 if filter_phase_mad is None else float(filter_phase_mad)
-        ffm = -1.0 if filter_freq_mad is None else float(filter_freq_mad)
-        check(fn(self._h, *first, _ptr(cf, C.c_double), _ptr(an, C.c_int64),
-                 1 if cn0 else 0, fpm, ffm, C.byref(out)))
-        try:
-            return C.string_at(out.value).decode()
-        finally:
-            lib().tga_free_string(out)
+        check(fn(self._h, *first, _ptr(cf, C.c_double), _ptr(an, C.c_int64), *tail,
+                 C.byref(out)))
+        return _take_string(out)
+
+    @staticmethod
+    def _filters(filter_phase_mad, filter_freq_mad):
+        return (-1.0 if filter_phase_mad is None else float(filter_phase_mad),
+                -1.0 if filter_freq_mad is None else float(filter_freq_mad))
+
+    def _packed_first(self, packed):
+        pk = np.ascontiguousarray(packed, dtype=np.uint8)
+        return (_ptr(pk, C.c_uint8), pk.shape[0], pk.shape[1]), pk
+
+    @staticmethod
+    def _peaks_first(peaks):
+        flat = np.ascontiguousarray(np.concatenate([p.reshape(-1) for p in peaks]))
+        return (flat.ctypes.data_as(C.POINTER(L.Peak)),), flat
 
     def acquire_all_json(self, packed, center_freq=4.092e6, antenna_numbers=None, cn0=False,
                          filter_phase_mad=None, filter_freq_mad=None):
         """Search one observation and return the reference's CombinedOutput JSON text."""
-        pk = np.ascontiguousarray(packed, dtype=np.uint8)
-        return self._json(lib().tga_acquire_all_json,
-                          (_ptr(pk, C.c_uint8), pk.shape[0], pk.shape[1]), center_freq,
-                          antenna_numbers, cn0, filter_phase_mad, filter_freq_mad)
+        first, _keep = self._packed_first(packed)
+        return self._json(lib().tga_acquire_all_json, first, center_freq, antenna_numbers,
+                          (1 if cn0 else 0, *self._filters(filter_phase_mad, filter_freq_mad)))
 
     def peaks_json(self, peaks, center_freq=4.092e6, antenna_numbers=None, cn0=False,
                    filter_phase_mad=None, filter_freq_mad=None):
         """JSON of peaks returned by search() (list per system of [n_ant, n_prn] arrays)."""
-        flat = np.ascontiguousarray(np.concatenate([p.reshape(-1) for p in peaks]))
-        return self._json(lib().tga_peaks_json, (flat.ctypes.data_as(C.POINTER(L.Peak)),),
-                          center_freq, antenna_numbers, cn0, filter_phase_mad, filter_freq_mad)
+        first, _keep = self._peaks_first(peaks)
+        return self._json(lib().tga_peaks_json, first, center_freq, antenna_numbers,
+                          (1 if cn0 else 0, *self._filters(filter_phase_mad, filter_freq_mad)))
+
+    def cn0(self, peaks):
+        """ACR C/N0 (dB-Hz, f64) of search() peaks, per system an [n_ant, n_prn] array with
+        NaN where the reference has None (acquisition.rs:341-357); computed natively."""
+        first, flat = self._peaks_first(peaks)
+        out = np.empty(flat.shape[0], dtype=np.float64)
+        check(lib().tga_peaks_cn0(self._h, *first, _ptr(out, C.c_double)))
+        res, o = [], 0
+        for p in peaks:
+            res.append(out[o:o + p.size].reshape(p.shape))
+            o += p.size
+        return res
+
+    def test_antennas_json(self, packed, center_freq=4.092e6, antenna_numbers=None,
+                           min_cn0=44.0, filter_phase_mad=None, filter_freq_mad=None):
+        """Search one observation and return the --test-antennas report (main.rs:848-880):
+        C/N0 estimates forced on, MAD filters first, then antenna_test::run."""
+        first, _keep = self._packed_first(packed)
+        return self._json(lib().tga_acquire_antenna_report_json, first, center_freq,
+                          antenna_numbers,
+                          (*self._filters(filter_phase_mad, filter_freq_mad), float(min_cn0)))
+
+    def peaks_test_antennas_json(self, peaks, center_freq=4.092e6, antenna_numbers=None,
+                                 min_cn0=44.0, filter_phase_mad=None, filter_freq_mad=None):
+        """The --test-antennas report of peaks returned by search()."""
+        first, _keep = self._peaks_first(peaks)
+        return self._json(lib().tga_peaks_antenna_report_json, first, center_freq,
+                          antenna_numbers,
+                          (*self._filters(filter_phase_mad, filter_freq_mad), float(min_cn0)))
+
+
+def _take_string(out):
+    try:
+        return C.string_at(out.value).decode()
+    finally:
+        lib().tga_free_string(out)
+
+
+def antenna_test_json(combined, min_cn0=44.0):
+    """antenna_test::run (antenna_test.rs:105-222) on a CombinedOutput -- the dict (or JSON
+    text) that acquire_all_json writes -- returned as the report's JSON text.  Host-only."""
+    if isinstance(combined, (str, bytes)):
+        combined = json.loads(combined)
+    antenna_numbers, svs, rows = [], [], []
+    for key in ("gps", "galileo", "beidou", "sbas", "l1c", "qzss"):  # lib.rs:33-44
+        out = combined.get(key)
+        if out is None:
+            continue
+        if not antenna_numbers:
+            antenna_numbers = list(out["antenna_numbers"])
+        for r in out["results"]:
+            cn0 = r.get("cn0_acr")
+            if cn0 is None:
+                continue
+            svs.append(r["sv"])
+            rows.append([np.nan if c is None else float(c) for c in cn0])
+    n_ant = len(antenna_numbers)
+    if any(len(r) != n_ant for r in rows):
+        raise ValueError("cn0_acr rows must have one entry per antenna")
+    an = np.ascontiguousarray(antenna_numbers, dtype=np.int64)
+    cn = np.ascontiguousarray(np.array(rows, dtype=np.float64).reshape(len(rows), n_ant))
+    labels = (C.c_char_p * max(1, len(svs)))(*[s.encode() for s in svs])
+    out = C.c_void_p()
+    check(lib().tga_antenna_report_json(len(svs), labels, _ptr(cn, C.c_double), n_ant,
+                                        _ptr(an, C.c_int64), float(min_cn0), C.byref(out)))
+    return _take_string(out)
 
 
 def format_f64(x):
