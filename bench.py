@@ -5,8 +5,8 @@ samples at 16.368 MHz, 10 ms records (163680 samples); seeded signals: 8 GPS, 2 
 QZSS at C/N0 uniform in [38, 50] dB-Hz with per-SV Doppler and per-antenna carrier offsets
 from a 3 m disk; search 38 + 39 + 23 PRNs x 41 Doppler bins (+-10 kHz, 500 Hz step), N = 2P
 (reference form, two hypotheses), FP32, plus ACR C/N0 on the host.  The FP16 matrix-core form
-(f16 spectra, radix passes as f16 GEMMs, f64 carrier phase) then runs the same observation and
-is reported under extra.fp16_mfma with its difference to the FP32 peaks.
+(f16 spectra or f16 sub-rows, radix passes as f16 GEMMs, f64 carrier phase) then runs the same
+observation and is reported under extra.fp16_mfma with its difference to the FP32 peaks.
 
 Other workloads (same JSON line):
   ca1      GPS L1 C/A, 1 antenna, 1-bit, 10 ms: 4 signals at 45 dB-Hz, PRN 1-32 x 21 bins
@@ -218,7 +218,8 @@ def main():
     traffic = load_traffic(args.traffic, plan.kernel_name(0))
 
     fp16 = None
-    if not args.no_fp16 and "fused16368" in plan.describe():
+    desc = plan.describe()
+    if not args.no_fp16 and ("fused16368" in desc or "split" in desc):
         # the FP16 matrix-core form on the same observation, compared with the FP32 peaks
         plan16 = tga.Plan(FS, n_ant, L, systems, input_format=fmt, precision="fp16")
         dt16, peaks16, _ = timed(plan16)

@@ -256,6 +256,7 @@ void Plan::build(const tga_plan_desc& d) {
     g.codes.clear();
     g.codes.shrink_to_fit();
     g.fp16 = g.fused && g.nc == 0 && precision == TGA_PRECISION_FP16;
+    g.big16 = g.big && precision == TGA_PRECISION_FP16;
     // forward spectra and per-row results
     g.n_xrow = static_cast<size_t>(n_ant) * n_bin * (g.nc > 0 ? g.nc : g.folded ? 2 : 1);
     g.n_rows = static_cast<long long>(n_ant) * n_prn * n_bin * 2;
@@ -270,6 +271,7 @@ void Plan::build(const tga_plan_desc& d) {
       g.d_rep.release();
       g.d_X16.alloc(g.n_xrow * M * sizeof(uint32_t));
       upload(g.d_tw, fused_twiddles(), stream);
+      autotune(g);
     } else if (g.nc > 0) {
       g.d_X.alloc(g.n_xrow * M * sizeof(float2));
       upload(g.d_tw, fused_twiddles(), stream);
@@ -285,7 +287,8 @@ void Plan::build(const tga_plan_desc& d) {
       // sub-rows of at most ~2 GiB of rows at a time
       const long long cap = std::max<long long>(1, (256LL << 20) / static_cast<long long>(M));
       g.chunk_rows = std::min<long long>(cap, g.n_rows);
-      for (auto& y : g.d_Y) y.alloc(static_cast<size_t>(g.chunk_rows) * M * sizeof(float2));
+      const size_t y_elem = g.big16 ? sizeof(uint32_t) : sizeof(float2);
+      for (auto& y : g.d_Y) y.alloc(static_cast<size_t>(g.chunk_rows) * M * y_elem);
       if (!side) {
         TGA_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
         for (hipEvent_t* e : {&ev_fwd, &ev_pre[0], &ev_pre[1], &ev_rows[0], &ev_rows[1]})
@@ -330,41 +333,60 @@ void Plan::build(const tga_plan_desc& d) {
 }
 
 // Rows-kernel autotuner: times every candidate shape once (after one warm-up launch) on the
-// group's real sizes, with zeroed spectra, and keeps the fastest.  TGA_FUSED_VARIANT=0|1 pins
-// the radix-31 implementation (tests use it to cover both).
+// group's real sizes, with zeroed spectra, and keeps the fastest.  Candidates: the radix-31
+// implementation of the f32 kernel (VALU codelet or matrix cores) and the number of XCD-local
+// PRN groups (8, 16 or 32: more groups give more, shorter workgroups, each over fewer PRNs).
+// TGA_FUSED_VARIANT=0|1 pins the radix-31 implementation (tests use it to cover both) and
+// TGA_ROWS_GROUPS=n the grouping.
 void Plan::autotune(Group& g) {
-  const char* pin = std::getenv("TGA_FUSED_VARIANT");
+  const char* pin_v = std::getenv("TGA_FUSED_VARIANT");
+  const char* pin_g = std::getenv("TGA_ROWS_GROUPS");
+  std::vector<int> grs;
+  if (pin_g) {
+    grs.push_back(std::max(1, std::atoi(pin_g)));
+  } else {
+    for (int gr : {8, 16, 32})  // groupings beyond the PRN count only add empty groups
+      if (gr == 8 || gr / 2 < g.n_prn) grs.push_back(gr);
+  }
   std::vector<FusedRowsConfig> cands;
-  for (int v = 0; v < 2; ++v) {
-    if (pin && std::atoi(pin) != v) continue;
-    FusedRowsConfig c;
-    c.variant = v;
-    c.groups = 8;
-    cands.push_back(c);
+  for (int v = 0; v < (g.fp16 ? 1 : 2); ++v) {
+    if (pin_v && !g.fp16 && std::atoi(pin_v) != v) continue;
+    for (int gr : grs) {
+      FusedRowsConfig c;
+      c.variant = v;
+      c.groups = gr;
+      cands.push_back(c);
+    }
   }
-  if (cands.size() == 1) {
-    g.rows_cfg = cands[0];
-    return;
-  }
-  TGA_HIP(hipMemsetAsync(g.d_X.p, 0, g.d_X.bytes, stream));
+  g.rows_cfg = cands[0];
+  g.tune.clear();
+  if (cands.size() == 1) return;
+  DevBuf& xs = g.fp16 ? g.d_X16 : g.d_X;
+  TGA_HIP(hipMemsetAsync(xs.p, 0, xs.bytes, stream));
+  auto launch = [&](const FusedRowsConfig& c) {
+    if (g.fp16)
+      launch_fused_rows16(g.d_X16.as<uint32_t>(), g.d_rep16.as<uint32_t>(), g.d_tw.as<float2>(),
+                          n_ant, g.n_bin, g.n_prn, g.cw, g.d_rows.as<RowOut>(), c.groups, stream);
+    else
+      launch_fused_rows(g.d_X.as<float2>(), g.d_rep.as<float2>(), g.d_tw.as<float2>(), n_ant,
+                        g.n_bin, g.n_prn, g.cw, g.d_rows.as<RowOut>(), stream, c);
+  };
   hipEvent_t e0, e1;
   TGA_HIP(hipEventCreate(&e0));
   TGA_HIP(hipEventCreate(&e1));
   float best = 1e30f;
-  for (size_t i = 0; i < cands.size(); ++i) {
-    launch_fused_rows(g.d_X.as<float2>(), g.d_rep.as<float2>(), g.d_tw.as<float2>(), n_ant,
-                      g.n_bin, g.n_prn, g.cw, g.d_rows.as<RowOut>(), stream, cands[i]);
+  for (const FusedRowsConfig& c : cands) {
+    launch(c);
     TGA_HIP(hipEventRecord(e0, stream));
-    launch_fused_rows(g.d_X.as<float2>(), g.d_rep.as<float2>(), g.d_tw.as<float2>(), n_ant,
-                      g.n_bin, g.n_prn, g.cw, g.d_rows.as<RowOut>(), stream, cands[i]);
+    launch(c);
     TGA_HIP(hipEventRecord(e1, stream));
     TGA_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
     TGA_HIP(hipEventElapsedTime(&ms, e0, e1));
-    if (i < 4) g.tune_ms[i] = ms;
+    g.tune.push_back({c, static_cast<double>(ms)});
     if (ms < best) {
       best = ms;
-      g.rows_cfg = cands[i];
+      g.rows_cfg = c;
     }
   }
   (void)hipEventDestroy(e0);
@@ -443,10 +465,11 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
     } else {
       if (g.folded)
         launch_fold_obs(d_in, row_bytes, fmt, d_means.as<double>(), g.d_fc.as<double>(), n_ant,
-                        g.n_bin, M, phase_const, g.d_tau.as<float2>(), g.d_X.as<float2>(), st);
+                        g.n_bin, M, phase_const, g.d_tau.as<float2>(), g.d_X.as<float2>(),
+                        g.big16, st);
       else
         launch_wipe_obs(d_in, row_bytes, fmt, d_means.as<double>(), g.d_fc.as<double>(), n_ant,
-                        g.n_bin, M, phase_const, g.d_X.as<float2>(), st);
+                        g.n_bin, M, phase_const, g.d_X.as<float2>(), g.big16, st);
       g.X = fft_forward(fft_plan(g.M), g.d_X.as<float2>(), g.d_Xs.as<float2>(), g.n_xrow, st);
     }
   }
@@ -460,7 +483,8 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
                            g.n_prn, g.nc, g.cw, g.d_rows.as<RowOut>(), 8, st);
     } else if (g.fp16) {
       launch_fused_rows16(g.d_X16.as<uint32_t>(), g.d_rep16.as<uint32_t>(), g.d_tw.as<float2>(),
-                          n_ant, g.n_bin, g.n_prn, g.cw, g.d_rows.as<RowOut>(), 8, st);
+                          n_ant, g.n_bin, g.n_prn, g.cw, g.d_rows.as<RowOut>(),
+                          g.rows_cfg.groups, st);
     } else if (g.fused) {
       launch_fused_rows(g.X, g.d_rep.as<float2>(), g.d_tw.as<float2>(), n_ant, g.n_bin, g.n_prn,
                         g.cw, g.d_rows.as<RowOut>(), st, g.rows_cfg);
@@ -476,12 +500,12 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
         const int b = c & 1;
         if (c >= 2) TGA_HIP(hipStreamWaitEvent(side, ev_rows[b], 0));
         launch_big_pre(g.X, g.d_rep.as<float2>(), g.d_twb.as<float2>(), g.big_r0, g.n_prn,
-                       g.n_bin, hs, r0, r1, g.d_Y[b].as<float2>(),
+                       g.n_bin, hs, r0, r1, g.d_Y[b].p, g.big16,
                        g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(), side);
         TGA_HIP(hipEventRecord(ev_pre[b], side));
         TGA_HIP(hipStreamWaitEvent(st, ev_pre[b], 0));
         launch_big_sub(g.d_tw.as<float2>(), g.big_r0, g.n_prn, g.n_bin, hs, r0, r1,
-                       g.d_Y[b].as<float2>(), g.d_rowkey.as<unsigned long long>(),
+                       g.d_Y[b].p, g.big16, g.d_rowkey.as<unsigned long long>(),
                        g.d_rownan.as<int>(), g.d_rows.as<RowOut>(), st);
         TGA_HIP(hipEventRecord(ev_rows[b], st));
       }
@@ -501,15 +525,17 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
   time_end(0, st, rows_bytes);
   time_begin(2, st);
   for (auto& g : groups) {
-    // folded rows carry half the 2P-point gain; the FP16 form's powers carry kF16Scale^4
-    const float gain = (g.folded ? 2.0f : 1.0f) / (g.fp16 ? kF16Scale * kF16Scale : 1.0f);
+    // folded rows carry half the 2P-point gain; the FP16 forms' values carry kF16Scale^2
+    // (fused) or kSplitF16Scale (split); the winners' second peaks are recomputed in f32
+    const float fold = g.folded ? 2.0f : 1.0f;
+    const float gain = fold / (g.fp16 ? kF16Scale * kF16Scale : g.big16 ? kSplitF16Scale : 1.0f);
     launch_select(g.d_rows.as<RowOut>(), g.d_slots.as<PeakSlot>(), n_ant, g.n_prn, g.n_bin,
                   gain, g.scale, d_peaks.as<tga_peak>(), d_nan.as<int>(), st,
                   g.big ? g.d_win.as<int>() : nullptr);
     if (g.big)
       launch_big_second(g.X, g.d_rep.as<float2>(), g.d_slots.as<PeakSlot>(),
                         static_cast<int>(g.M), static_cast<int>(g.P), g.cw, n_ant, g.n_prn,
-                        g.n_bin, g.folded, g.d_win.as<int>(), gain, g.scale, g.d_Zw.as<float2>(),
+                        g.n_bin, g.folded, g.d_win.as<int>(), fold, g.scale, g.d_Zw.as<float2>(),
                         g.d_Zws.as<float2>(), d_peaks.as<tga_peak>(), st);
   }
   time_end(2, st, 0.0);
@@ -535,15 +561,23 @@ std::string Plan::describe() const {
     char buf[256];
     std::snprintf(buf, sizeof buf,
                   "%s: P=%zu N=%zu bins=%zu prns=%zu group=%d path=%s rows_variant=%d "
-                  "groups=%d tune_ms=%.3f/%.3f\n",
+                  "groups=%d tune_ms=",
                   system_info(e.system).key, g.P, g.N, g.fc.size(), e.prns.size(), e.group,
                   g.nc > 0 ? "fused16368-noncoherent"
                   : g.fp16 ? "fused16368-fp16"
                   : g.fused ? "fused16368"
-                  : g.big ? (g.big_r0 == 4 ? "split4x16368" : "split10x16368")
+                  : g.big ? (g.big_r0 == 4 ? (g.big16 ? "split4x16368-fp16" : "split4x16368")
+                                           : (g.big16 ? "split10x16368-fp16" : "split10x16368"))
                           : (g.folded ? "generic-folded" : "generic-unfolded"),
-                  g.rows_cfg.variant, g.rows_cfg.groups, g.tune_ms[0], g.tune_ms[1]);
+                  g.rows_cfg.variant, g.rows_cfg.groups);
     s += buf;
+    // every candidate the autotuner timed: v<variant>g<groups>:<ms>
+    for (size_t k = 0; k < g.tune.size(); ++k) {
+      std::snprintf(buf, sizeof buf, "%sv%dg%d:%.3f", k ? "," : "", g.tune[k].first.variant,
+                    g.tune[k].first.groups, g.tune[k].second);
+      s += buf;
+    }
+    s += g.tune.empty() ? "-\n" : "\n";
   }
   s += !graphable         ? "graph=off (Bluestein transform)"
        : !graph_enabled   ? "graph=off"
@@ -673,9 +707,13 @@ const char* tga_plan_kernel_name(const tga_plan* plan, int32_t kernel) {
   const bool fused = p && !p->groups.empty() && p->groups[0].fused;
   const bool fp16 = fused && p->groups[0].fp16;
   const bool big = p && !p->groups.empty() && p->groups[0].big;
+  const bool big16 = big && p->groups[0].big16;
   switch (kernel) {
-    case 0: return fp16 ? "k_rows16" : fused ? "k_rows" : big ? "k_big_pre+k_rows<0,R0>"
-                                                              : "k_mul_rows+k_pass+k_row_reduce";
+    case 0: return fp16    ? "k_rows16"
+                   : fused ? "k_rows"
+                   : big16 ? "k_big_pre+k_rows16<R0>"
+                   : big   ? "k_big_pre+k_rows<0,R0>"
+                           : "k_mul_rows+k_pass+k_row_reduce";
     case 1: return fused ? "k_forward" : "k_fold_obs+k_pass";
     case 2: return "k_select";
   }

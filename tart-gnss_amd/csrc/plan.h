@@ -3,6 +3,7 @@
 
 #include <array>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "internal.h"
@@ -19,6 +20,7 @@ struct Group {
   int nc = 0;        // non-coherent form: one-period epochs summed (0 = reference form)
   bool big = false;  // P = big_r0 x 16368: sub-row split through the fused LDS kernel
   int big_r0 = 0;
+  bool big16 = false;  // FP16 form of the split: f16 sub-rows, matrix-core rows kernel
   std::vector<std::vector<double>> codes;  // replicas, plan creation only
   std::vector<PeakSlot> slots;
   int n_prn = 0, n_bin = 0;
@@ -31,7 +33,7 @@ struct Group {
   DevBuf d_Y[2], d_rowkey, d_rownan, d_twb, d_win, d_Zw, d_Zws;  // long-period split
   float2* X = nullptr;  // where the forward spectra of the last search landed
   FusedRowsConfig rows_cfg;  // rows-kernel shape picked by the autotuner
-  double tune_ms[4] = {0, 0, 0, 0};
+  std::vector<std::pair<FusedRowsConfig, double>> tune;  // candidates timed, ms per launch
   double row_bytes_alg = 0, fwd_bytes_alg = 0;
 };
 

@@ -60,8 +60,11 @@ void launch_fused_rows_nc(const float2* X, const float2* rep, const float2* tw, 
 // FP16 form of the fused path: X and the replica spectra are stored as f16 (re, im) pairs of
 // the value times kF16Scale (so |conj(X) R| stays far inside the f16 range for 1- and 2-bit
 // input), the radix passes run as f16 GEMMs on the matrix cores with f32 accumulation, and the
-// row powers are those of the transform times kF16Scale^2.
+// row values are those of the transform times kF16Scale^2 (powers times kF16Scale^4).
 constexpr float kF16Scale = 1.0f / 256.0f;
+// Sub-rows of long rows (P = 4 or 10 x 16368) after the first split stage reach about
+// sqrt(R0) |X| |R| ~ 1e6 for 1-bit input; this brings them near 1 in f16.
+constexpr float kSplitF16Scale = 1.0f / 1048576.0f;  // 2^-20
 void launch_fused_forward16(const uint8_t* packed, size_t row_bytes, int fmt,
                             const double* means, const double* fc, int n_ant, int n_bin,
                             double phase_const, const float2* tau, const float2* tw,
@@ -75,13 +78,15 @@ std::vector<float2> fused_twiddles();
 
 // ---- generic path -------------------------------------------------------------------------
 // folded input rows (S / twisted E) for any P, N = 2P: out[(ant*n_bin + bin)*2 + h][P]
+// phase64: carrier phase in f64 throughout (the FP16 precision form) instead of the reference's
+// phi = f32(pp * fc)
 void launch_fold_obs(const uint8_t* packed, size_t row_bytes, int fmt, const double* means,
                      const double* fc, int n_ant, int n_bin, int P, double phase_const,
-                     const float2* tau, float2* out, hipStream_t st);
+                     const float2* tau, float2* out, bool phase64, hipStream_t st);
 // unfolded wiped rows for N-point correlation: out[ant*n_bin + bin][N]
 void launch_wipe_obs(const uint8_t* packed, size_t row_bytes, int fmt, const double* means,
                      const double* fc, int n_ant, int n_bin, int N, double phase_const,
-                     float2* out, hipStream_t st);
+                     float2* out, bool phase64, hipStream_t st);
 // level 1: out[bin][N] from f32 samples and f64 phase points
 void launch_wipe_signal(const float* sig, const double* pp, const double* fc, int n_bin, int N,
                         float2* out, hipStream_t st);
@@ -117,12 +122,15 @@ std::vector<float2> big_twiddles(size_t P);
 // launch_big_sub transforms them and writes the rows' RowOut.
 // hs = 1: every (bin, h) row; hs = 2: unfolded rows, h = 0 only (the [c,-c] hypothesis equals
 // [c,c] when N <= P, correlate.rs:53-56, and gets a row that never wins the selection).
+// f16: Y holds f16 (re, im) pairs scaled by kSplitF16Scale (uint32 per value) and the sub-rows
+// run through the matrix-core rows kernel, whose powers then carry kSplitF16Scale^2; otherwise
+// Y holds float2 and the f32 rows kernel transforms them.
 void launch_big_pre(const float2* X, const float2* rep, const float2* twb, int R0, int n_prn,
-                    int n_bin, int hs, long long r0, long long r1, float2* Y,
+                    int n_bin, int hs, long long r0, long long r1, void* Y, bool f16,
                     unsigned long long* rowkey, int* rownan, hipStream_t st);
 void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int hs, long long r0,
-                    long long r1, const float2* Y, unsigned long long* rowkey, int* rownan,
-                    RowOut* rows, hipStream_t st);
+                    long long r1, const void* Y, bool f16, unsigned long long* rowkey,
+                    int* rownan, RowOut* rows, hipStream_t st);
 // second peaks of the winning rows; Z, Zs hold n_ant * n_prn * M complex values
 void launch_big_second(const float2* X, const float2* rep, const PeakSlot* slots, int M, int P,
                        int cw, int n_ant, int n_prn, int n_bin, bool folded, const int* win,

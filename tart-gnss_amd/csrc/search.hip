@@ -477,14 +477,25 @@ struct RowRed {
   int bad[NW_];
 };
 
-// End of one correlation row: block argmax (last maximum), the masked second maximum around
-// it, and the NaN / overflow flag, with two workgroup barriers.  Waves reduce with DPP moves;
-// across waves, lane w reads wave w's slot and the same 16-lane reduction combines them, so no
-// thread walks the slots one by one.  Safe to call once per row in a loop: every LDS slot is
-// read before the barrier that precedes its next write.  Called by every thread of the block.
+// A row whose result waits for its masked maximum: thread 0 writes it in row_flush.
+struct RowPending {
+  RowOut* out = nullptr;
+  float p1 = 0.0f;
+  int i1 = 0;
+  int nan = 0;
+};
+
+// End of one correlation row, first half: block argmax (last maximum), the NaN / overflow flag
+// and each wave's masked maximum around the argmax, with one workgroup barrier.  Waves reduce
+// with DPP moves; across waves, lane w reads wave w's slot and the same 16-lane reduction
+// combines them, so no thread walks the slots one by one.  The result waits in `pend` until
+// row_flush, which runs after the kernel's next workgroup barrier and before the second one
+// (so each LDS slot is read before the barrier that precedes its next write).  Called by every
+// thread of the block in uniform control flow.
 template <int NT>
-__device__ __forceinline__ void row_finish(const Top2& main_set, const Top2& extra_set,
-                                           bool bad, int cw, RowRed<NT / 64>& red, RowOut* out) {
+__device__ __forceinline__ void row_finish_begin(const Top2& main_set, const Top2& extra_set,
+                                                 bool bad, int cw, RowRed<NT / 64>& red,
+                                                 RowOut* out, RowPending& pend) {
   constexpr int NWR = NT / 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   float bp = main_set.best();
@@ -507,19 +518,41 @@ __device__ __forceinline__ void row_finish(const Top2& main_set, const Top2& ext
   const unsigned int mw =
       wave_umax(ford(fmaxf(main_set.masked(bi, cw), extra_set.masked(bi, cw))));
   if (lane == 0) red.m[wave] = mw;
-  lds_barrier();
-  if (wave == 0) {
-    const unsigned int m2 = static_cast<unsigned int>(
-        __builtin_amdgcn_readlane(static_cast<int>(row16_umax(mine ? red.m[lane] : 0u)), 0));
+  pend.out = out;
+  pend.p1 = bp;
+  pend.i1 = bi;
+  pend.nan = any_bad ? 1 : 0;
+}
+
+// Second half: the block's masked maximum and the row's RowOut (wave 0), once a workgroup
+// barrier separates it from every wave's row_finish_begin.
+template <int NT>
+__device__ __forceinline__ void row_flush(const RowRed<NT / 64>& red, RowPending& pend) {
+  constexpr int NWR = NT / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (wave == 0 && pend.out != nullptr) {
+    const unsigned int m2 = static_cast<unsigned int>(__builtin_amdgcn_readlane(
+        static_cast<int>(row16_umax(lane < NWR ? red.m[lane] : 0u)), 0));
     if (lane == 0) {
       RowOut o;
-      o.p1 = bp;
+      o.p1 = pend.p1;
       o.p2 = funord(m2);
-      o.i1 = bi;
-      o.nan = any_bad ? 1 : 0;
-      *out = o;
+      o.i1 = pend.i1;
+      o.nan = pend.nan;
+      *pend.out = o;
     }
   }
+  pend.out = nullptr;
+}
+
+// Both halves with their own second barrier.
+template <int NT>
+__device__ __forceinline__ void row_finish(const Top2& main_set, const Top2& extra_set,
+                                           bool bad, int cw, RowRed<NT / 64>& red, RowOut* out) {
+  RowPending pend;
+  row_finish_begin<NT>(main_set, extra_set, bad, cw, red, out, pend);
+  lds_barrier();
+  row_flush<NT>(red, pend);
 }
 
 // End of one sub-row of a long row: block argmax, folded into the row's 64-bit key (power
@@ -866,6 +899,8 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
             accn += p;
             (slot < 31 ? ms : es).add(p, idx);
           });
+      // the two-barrier finish: deferring its second half to the next row's first barrier
+      // measured 6% slower here (wave 0 then carries the flush into the next pass)
       row_finish<T>(ms, es, accn != accn, a.cw, red,
                     a.rows + ((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) * 2 + h);
     }
@@ -916,6 +951,13 @@ struct Row16Args {
   int per_group;
   int cw;
   RowOut* rows;
+  // sub-row form (SUB > 0, long rows, see k_big_pre): the pass-A operands are the f16 words
+  // of sub-row s of the chunk, the results go to the row keys like k_rows<0, SUB>
+  const uint32_t* Y = nullptr;  // [sub-row][P] f16 pairs
+  long long n_sub = 0, r0 = 0;
+  unsigned long long* rowkey = nullptr;
+  int* rownan = nullptr;
+  int hs = 1;
 };
 
 // conj(x) * r on f16 pairs (two packed instructions)
@@ -969,11 +1011,20 @@ __device__ __forceinline__ void load_tile_a(TileA& d, __amdgpu_buffer_rsrc_t xv,
   }
 }
 
+// The same lane's words of one sub-row (already conj(X) . R, twiddled, in f16).
+__device__ __forceinline__ void load_tile_y(uint32_t (&y)[9], __amdgpu_buffer_rsrc_t yv, int vo) {
+  constexpr int NBA = P / 33;
+#pragma unroll
+  for (int u = 0; u < 9; ++u)
+    y[u] = __builtin_amdgcn_raw_buffer_load_b32(yv, vo, NBA * (16 * (u >> 2) + (u & 3)) * 4, 0);
+}
+
 // 1024 threads (16 waves, four per SIMD): the matrix-core form keeps few values per lane, so
 // more waves can hide the LDS and matrix-core latencies.
 constexpr int T16 = 1024;
 constexpr int NW16 = T16 / 64;
 
+template <int SUB>
 __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
   constexpr int NBA = P / 33;  // 496
   constexpr int NBB = P / 16;  // 1023
@@ -1001,18 +1052,26 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
   for (int t = 0; t < 4; ++t)
 #pragma unroll
     for (int kb = 0; kb < 2; ++kb) fc[t][kb] = dft_frag<31>(stw, t, kb, tid & 63);
-  const int g = blockIdx.x % a.groups;
-  const int xrow = blockIdx.x / a.groups;
+  // SUB == 0: one (antenna, bin, hypothesis) row of X per block and the PRNs of its group, dealt
+  // round-robin to the XCDs (blockIdx % groups); SUB > 0: sub-rows, grid-stride
+  const int g = SUB == 0 ? blockIdx.x % a.groups : 0;
+  const int xrow = SUB == 0 ? blockIdx.x / a.groups : 0;
   const int h = xrow & 1;
-  const int bin = (xrow >> 1) % a.n_bin;
-  const int ant = (xrow >> 1) / a.n_bin;
-  const int p0 = g * a.per_group;
-  const int p1 = min(p0 + a.per_group, a.n_prn);
-  const __amdgpu_buffer_rsrc_t xv = buffer_view(a.X + static_cast<size_t>(xrow) * P, P * 4);
+  const int bin = SUB == 0 ? (xrow >> 1) % a.n_bin : 0;
+  const int ant = SUB == 0 ? (xrow >> 1) / a.n_bin : 0;
+  const long long it0 = SUB == 0 ? g * a.per_group : blockIdx.x;
+  const long long it1 = SUB == 0 ? min(g * a.per_group + a.per_group, a.n_prn) : a.n_sub;
+  const long long step = SUB == 0 ? 1 : gridDim.x;
+  const __amdgpu_buffer_rsrc_t xv =
+      buffer_view(SUB == 0 ? a.X + static_cast<size_t>(xrow) * P : a.Y, P * 4);
   __syncthreads();
-  for (int prn = p0; prn < p1; ++prn) {
+  RowPending pend;  // the previous row, written after this row's first barrier
+  for (long long it = it0; it < it1; it += step) {
+    const int prn = static_cast<int>(it);
+    // SUB == 0: replica row; SUB > 0: the sub-row itself
     const __amdgpu_buffer_rsrc_t rv =
-        buffer_view(a.rep + (static_cast<size_t>(prn) * 2 + h) * P, P * 4);
+        SUB == 0 ? buffer_view(a.rep + (static_cast<size_t>(prn) * 2 + h) * P, P * 4)
+                 : buffer_view(a.Y + it * P, P * 4);
     // Lane coordinates are re-derived per PRN through an opaque copy: otherwise the compiler
     // hoists every lane-dependent twiddle and address out of the PRN loop and spills them.
     int tid2 = tid;
@@ -1024,18 +1083,33 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
       const int tile = wave + 16 * s;
       if (tile < NBA / 16) {
         const int j = 16 * tile + ln16;
-        TileA cur;
-        load_tile_a(cur, xv, rv, (j + NBA * 4 * grp) * 4);
         halfx8 b[3];
+        if constexpr (SUB == 0) {
+          TileA cur;
+          load_tile_a(cur, xv, rv, (j + NBA * 4 * grp) * 4);
 #pragma unroll
-        for (int kb = 0; kb < 3; ++kb) {
-          u32x4 w;
+          for (int kb = 0; kb < 3; ++kb) {
+            u32x4 w;
 #pragma unroll
-          for (int i2 = 0; i2 < 4; ++i2) {
-            const int u = 4 * kb + i2;
-            w[i2] = u < 9 ? conjmul16(cur.x[u], cur.r[u]) : 0u;
+            for (int i2 = 0; i2 < 4; ++i2) {
+              const int u = 4 * kb + i2;
+              w[i2] = u < 9 ? conjmul16(cur.x[u], cur.r[u]) : 0u;
+            }
+            b[kb] = __builtin_bit_cast(halfx8, w);
           }
-          b[kb] = __builtin_bit_cast(halfx8, w);
+        } else {
+          uint32_t y[9];
+          load_tile_y(y, rv, (j + NBA * 4 * grp) * 4);
+#pragma unroll
+          for (int kb = 0; kb < 3; ++kb) {
+            u32x4 w;
+#pragma unroll
+            for (int i2 = 0; i2 < 4; ++i2) {
+              const int u = 4 * kb + i2;
+              w[i2] = u < 9 ? y[u] : 0u;
+            }
+            b[kb] = __builtin_bit_cast(halfx8, w);
+          }
         }
         // twiddle W_P^{q j} for q = 8t + 2 grp (+1); output q of butterfly j is pass-B
         // input r = j / 31 of butterfly 33 (j mod 31) + q
@@ -1060,6 +1134,7 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
       }
     }
     lds_barrier();
+    if constexpr (SUB == 0) row_flush<T16>(red, pend);
     // ---- pass B: radix 16, 64 tiles (the last holds 15 butterflies): wave + 16 s
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
@@ -1164,8 +1239,22 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
         }
       }
     }
-    row_finish<T16>(main_set, extra_set, !(accn <= 3.0e38f), a.cw, red,
-                    a.rows + ((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) * 2 + h);
+    if constexpr (SUB == 0) {
+      row_finish_begin<T16>(
+          main_set, extra_set, !(accn <= 3.0e38f), a.cw, red,
+          a.rows + ((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) * 2 + h, pend);
+    } else {
+      // sub-row it of the chunk: row r0 + it / SUB, output samples k0 + SUB m (k0 = it mod SUB)
+      const long long rq = it / SUB;
+      const int k0 = static_cast<int>(it - rq * SUB);
+      const long long r = big_row(a.r0 + rq, a.n_prn, a.n_bin, a.hs);
+      row_finish_key<T16>(main_set, extra_set, !(accn <= 3.0e38f), red, k0, SUB, a.rowkey + r,
+                          a.rownan + r);
+    }
+  }
+  if constexpr (SUB == 0) {
+    lds_barrier();
+    row_flush<T16>(red, pend);
   }
 }
 
@@ -1179,6 +1268,8 @@ __global__ void k_to_f16(const float2* __restrict__ in, uint32_t* __restrict__ o
 }  // namespace fused
 
 // ================================ generic path ===========================================
+// PHASE64: the FP16 precision form's carrier phase, f64 throughout (no f32 rounding of phi).
+template <bool PHASE64>
 __global__ void k_fold_obs(const uint8_t* __restrict__ packed, size_t row_bytes, int fmt,
                            const double* __restrict__ means, const double* __restrict__ fc,
                            int n_bin, int P, long long total, double pc,
@@ -1191,11 +1282,15 @@ __global__ void k_fold_obs(const uint8_t* __restrict__ packed, size_t row_bytes,
   const int bin = static_cast<int>((xrow >> 1) % n_bin);
   const int ant = static_cast<int>((xrow >> 1) / n_bin);
   const uint8_t* row = packed + static_cast<size_t>(ant) * row_bytes;
-  const float2 x0 = wipe_obs(row, fmt, n, means[ant], pc, fc[bin]);
-  const float2 x1 = wipe_obs(row, fmt, static_cast<long long>(n) + P, means[ant], pc, fc[bin]);
+  const long long n2 = static_cast<long long>(n) + P;
+  const float2 x0 = PHASE64 ? wipe_obs_f64(row, fmt, n, means[ant], pc, fc[bin])
+                            : wipe_obs(row, fmt, n, means[ant], pc, fc[bin]);
+  const float2 x1 = PHASE64 ? wipe_obs_f64(row, fmt, n2, means[ant], pc, fc[bin])
+                            : wipe_obs(row, fmt, n2, means[ant], pc, fc[bin]);
   out[gid] = h == 0 ? cadd(x0, x1) : cmul(csub(x0, x1), tau[n]);
 }
 
+template <bool PHASE64>
 __global__ void k_wipe_obs(const uint8_t* __restrict__ packed, size_t row_bytes, int fmt,
                            const double* __restrict__ means, const double* __restrict__ fc,
                            int n_bin, int N, long long total, double pc,
@@ -1207,7 +1302,8 @@ __global__ void k_wipe_obs(const uint8_t* __restrict__ packed, size_t row_bytes,
   const int bin = static_cast<int>(xrow % n_bin);
   const int ant = static_cast<int>(xrow / n_bin);
   const uint8_t* row = packed + static_cast<size_t>(ant) * row_bytes;
-  out[gid] = wipe_obs(row, fmt, n, means[ant], pc, fc[bin]);
+  out[gid] = PHASE64 ? wipe_obs_f64(row, fmt, n, means[ant], pc, fc[bin])
+                    : wipe_obs(row, fmt, n, means[ant], pc, fc[bin]);
 }
 
 __global__ void k_wipe_signal(const float* __restrict__ sig, const double* __restrict__ pp,
@@ -1280,13 +1376,15 @@ __global__ __launch_bounds__(kRedT) void k_row_reduce(const float2* __restrict__
 // With n = n1 + 16368 n0 and k = k0 + R0 k1, the P-point transform of z = conj(X) R is
 //   Z[k0 + R0 k1] = FFT_16368( W_P^{n1 k0} sum_{n0} z[n1 + 16368 n0] W_R0^{n0 k0} )[k1]:
 // k_big_pre forms the R0 twiddled sub-rows y_k0 of each row (one R0-point DFT per n1), and the
-// fused rows kernel in sub-row mode transforms each sub-row in LDS.
-template <int R0>
+// fused rows kernel in sub-row mode transforms each sub-row in LDS.  F16: the sub-rows are
+// stored as f16 (re, im) pairs scaled by kSplitF16Scale for the matrix-core rows kernel (the
+// products and the R0-point DFT stay f32).
+template <int R0, bool F16>
 __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
                                                  const float2* __restrict__ rep,
                                                  const float2* __restrict__ twb, int n_prn,
                                                  int n_bin, int hs, long long r0, long long total,
-                                                 float2* __restrict__ Y,
+                                                 void* __restrict__ Yv,
                                                  unsigned long long* __restrict__ rowkey,
                                                  int* __restrict__ rownan) {
   constexpr int P1 = kFusedP;
@@ -1305,12 +1403,20 @@ __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
 #pragma unroll
   for (int n0 = 0; n0 < R0; ++n0) v[n0] = cconjmul(x[n1 + P1 * n0], c[n1 + P1 * n0]);
   Dft<R0>::run(v);
-  float2* y = Y + rr * P;
-  y[n1] = v[0];
 #pragma unroll
   for (int k0 = 1; k0 < R0; ++k0) {
     const int e = n1 * k0;  // < P: W_P^e from two 512-entry levels
-    y[static_cast<long long>(k0) * P1 + n1] = cmul(v[k0], cmul(twb[e & 511], twb[512 + (e >> 9)]));
+    v[k0] = cmul(v[k0], cmul(twb[e & 511], twb[512 + (e >> 9)]));
+  }
+  if constexpr (F16) {
+    uint32_t* y = static_cast<uint32_t*>(Yv) + rr * P;
+#pragma unroll
+    for (int k0 = 0; k0 < R0; ++k0)
+      y[static_cast<long long>(k0) * P1 + n1] = fused::pack_f16(v[k0], kSplitF16Scale);
+  } else {
+    float2* y = static_cast<float2*>(Yv) + rr * P;
+#pragma unroll
+    for (int k0 = 0; k0 < R0; ++k0) y[static_cast<long long>(k0) * P1 + n1] = v[k0];
   }
   if (n1 == 0) {
     const long long r = big_row(I, n_prn, n_bin, hs);
@@ -1520,8 +1626,8 @@ void launch_fused_rows16(const uint32_t* X16, const uint32_t* rep16, const float
   const int per = (n_prn + groups - 1) / groups;
   fused::Row16Args a{X16, rep16, tw, n_bin, n_prn, groups, per, cw, rows};
   const long long blocks = static_cast<long long>(n_ant) * n_bin * 2 * groups;
-  hipLaunchKernelGGL(fused::k_rows16, dim3(static_cast<unsigned>(blocks)), dim3(fused::T16), 0,
-                     st, a);
+  hipLaunchKernelGGL(fused::k_rows16<0>, dim3(static_cast<unsigned>(blocks)),
+                     dim3(fused::T16), 0, st, a);
   TGA_HIP(hipGetLastError());
 }
 
@@ -1533,19 +1639,27 @@ void launch_to_f16(const float2* in, uint32_t* out, long long n, float scale, hi
 
 void launch_fold_obs(const uint8_t* packed, size_t row_bytes, int fmt, const double* means,
                      const double* fc, int n_ant, int n_bin, int P, double phase_const,
-                     const float2* tau, float2* out, hipStream_t st) {
+                     const float2* tau, float2* out, bool phase64, hipStream_t st) {
   const long long total = static_cast<long long>(n_ant) * n_bin * 2 * P;
-  hipLaunchKernelGGL(k_fold_obs, dim3(grid_for(total, 256)), dim3(256), 0, st, packed,
-                     row_bytes, fmt, means, fc, n_bin, P, total, phase_const, tau, out);
+  if (phase64)
+    hipLaunchKernelGGL(k_fold_obs<true>, dim3(grid_for(total, 256)), dim3(256), 0, st, packed,
+                       row_bytes, fmt, means, fc, n_bin, P, total, phase_const, tau, out);
+  else
+    hipLaunchKernelGGL(k_fold_obs<false>, dim3(grid_for(total, 256)), dim3(256), 0, st, packed,
+                       row_bytes, fmt, means, fc, n_bin, P, total, phase_const, tau, out);
   TGA_HIP(hipGetLastError());
 }
 
 void launch_wipe_obs(const uint8_t* packed, size_t row_bytes, int fmt, const double* means,
                      const double* fc, int n_ant, int n_bin, int N, double phase_const,
-                     float2* out, hipStream_t st) {
+                     float2* out, bool phase64, hipStream_t st) {
   const long long total = static_cast<long long>(n_ant) * n_bin * N;
-  hipLaunchKernelGGL(k_wipe_obs, dim3(grid_for(total, 256)), dim3(256), 0, st, packed,
-                     row_bytes, fmt, means, fc, n_bin, N, total, phase_const, out);
+  if (phase64)
+    hipLaunchKernelGGL(k_wipe_obs<true>, dim3(grid_for(total, 256)), dim3(256), 0, st, packed,
+                       row_bytes, fmt, means, fc, n_bin, N, total, phase_const, out);
+  else
+    hipLaunchKernelGGL(k_wipe_obs<false>, dim3(grid_for(total, 256)), dim3(256), 0, st, packed,
+                       row_bytes, fmt, means, fc, n_bin, N, total, phase_const, out);
   TGA_HIP(hipGetLastError());
 }
 
@@ -1598,41 +1712,68 @@ std::vector<float2> big_twiddles(size_t P) {
 }
 
 void launch_big_pre(const float2* X, const float2* rep, const float2* twb, int R0, int n_prn,
-                    int n_bin, int hs, long long r0, long long r1, float2* Y,
+                    int n_bin, int hs, long long r0, long long r1, void* Y, bool f16,
                     unsigned long long* rowkey, int* rownan, hipStream_t st) {
   const long long nr = r1 - r0;
   const long long total = nr * kFusedP;
-  if (R0 == 4)
-    hipLaunchKernelGGL(k_big_pre<4>, dim3(grid_for(total, 256)), dim3(256), 0, st, X, rep, twb,
-                       n_prn, n_bin, hs, r0, total, Y, rowkey, rownan);
+  const dim3 grid(grid_for(total, 256));
+  if (R0 == 4 && !f16)
+    hipLaunchKernelGGL((k_big_pre<4, false>), grid, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
+                       hs, r0, total, Y, rowkey, rownan);
+  else if (R0 == 4)
+    hipLaunchKernelGGL((k_big_pre<4, true>), grid, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
+                       hs, r0, total, Y, rowkey, rownan);
+  else if (R0 == 10 && !f16)
+    hipLaunchKernelGGL((k_big_pre<10, false>), grid, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
+                       hs, r0, total, Y, rowkey, rownan);
   else if (R0 == 10)
-    hipLaunchKernelGGL(k_big_pre<10>, dim3(grid_for(total, 256)), dim3(256), 0, st, X, rep, twb,
-                       n_prn, n_bin, hs, r0, total, Y, rowkey, rownan);
+    hipLaunchKernelGGL((k_big_pre<10, true>), grid, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
+                       hs, r0, total, Y, rowkey, rownan);
   else
     fail(TGA_ERR_UNSUPPORTED, "long-period split supports 4 and 10 sub-rows");
   TGA_HIP(hipGetLastError());
 }
 
 void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int hs, long long r0,
-                    long long r1, const float2* Y, unsigned long long* rowkey, int* rownan,
-                    RowOut* rows, hipStream_t st) {
+                    long long r1, const void* Y, bool f16, unsigned long long* rowkey,
+                    int* rownan, RowOut* rows, hipStream_t st) {
   const long long nr = r1 - r0;
-  fused::RowArgs a{};
-  a.hs = hs;
-  a.tw = tw;
-  a.cw = 0;
-  a.n_prn = n_prn;
-  a.n_bin = n_bin;
-  a.Y = Y;
-  a.n_sub = nr * R0;
-  a.r0 = r0;
-  a.rowkey = rowkey;
-  a.rownan = rownan;
-  const unsigned blocks = static_cast<unsigned>(a.n_sub < 2048 ? a.n_sub : 2048);
-  if (R0 == 4)
-    hipLaunchKernelGGL((fused::k_rows<0, 4>), dim3(blocks), dim3(fused::T), 0, st, a);
-  else
-    hipLaunchKernelGGL((fused::k_rows<0, 10>), dim3(blocks), dim3(fused::T), 0, st, a);
+  const long long n_sub = nr * R0;
+  const unsigned blocks = static_cast<unsigned>(n_sub < 2048 ? n_sub : 2048);
+  if (f16) {
+    fused::Row16Args a{};
+    a.tw = tw;
+    a.n_prn = n_prn;
+    a.n_bin = n_bin;
+    a.groups = 1;
+    a.per_group = 1;
+    a.Y = static_cast<const uint32_t*>(Y);
+    a.n_sub = n_sub;
+    a.r0 = r0;
+    a.rowkey = rowkey;
+    a.rownan = rownan;
+    a.hs = hs;
+    if (R0 == 4)
+      hipLaunchKernelGGL((fused::k_rows16<4>), dim3(blocks), dim3(fused::T16), 0, st, a);
+    else
+      hipLaunchKernelGGL((fused::k_rows16<10>), dim3(blocks), dim3(fused::T16), 0, st, a);
+  } else {
+    fused::RowArgs a{};
+    a.hs = hs;
+    a.tw = tw;
+    a.cw = 0;
+    a.n_prn = n_prn;
+    a.n_bin = n_bin;
+    a.Y = static_cast<const float2*>(Y);
+    a.n_sub = n_sub;
+    a.r0 = r0;
+    a.rowkey = rowkey;
+    a.rownan = rownan;
+    if (R0 == 4)
+      hipLaunchKernelGGL((fused::k_rows<0, 4>), dim3(blocks), dim3(fused::T), 0, st, a);
+    else
+      hipLaunchKernelGGL((fused::k_rows<0, 10>), dim3(blocks), dim3(fused::T), 0, st, a);
+  }
   TGA_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_big_rowout, dim3(grid_for(nr, 256)), dim3(256), 0, st, rowkey, rownan,
                      r0, nr, n_prn, n_bin, hs, rows);
