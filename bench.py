@@ -14,6 +14,10 @@ Other workloads (same JSON line):
   pilots24 Galileo E1-C (P = 65472), BeiDou B1C and GPS L1C (P = 163680), 24 antennas, 1-bit,
            40 ms: 6 signals per system at 40-48 dB-Hz; E1-C 50 SVs x 81 bins (125 Hz), B1C and
            L1C 63 SVs x 201 bins (50 Hz), N = 2P.
+  stream64 64 independent 24-antenna observations (2-bit, 10 ms) resident in HBM, each a random
+           4-12 SV mix from all six systems at 35-50 dB-Hz plus 3 CW tones per antenna at
+           -10..+10 dB; step k searches observation k mod 64 (all systems, 21 bins), computes its
+           ACR C/N0 and its RFI report; extra.rfi counts the tones found as report lines.
 
 A step = one tga_search_device call on an observation already resident in HBM: decode +
 de-mean, forward spectra, all correlation rows, selection, copy of the peaks to the host, and
@@ -86,6 +90,27 @@ def workload(name, rng):
                     systems=[("galileo", list(PRNS["galileo"]), grid(5000.0, 125.0)),
                              ("beidou", list(PRNS["beidou"]), grid(5000.0, 50.0)),
                              ("l1c", list(PRNS["l1c"]), grid(5000.0, 50.0))])
+    if name == "stream64":
+        # 64 independent observations, each a random 4-12 SV mix from every system at 35-50
+        # dB-Hz and 3 CW tones per antenna at -10..+10 dB against the noise
+        pool = [(s, p) for s in ("gps", "sbas", "qzss", "galileo", "beidou", "l1c")
+                for p in PRNS[s]]
+        stream = []
+        for _ in range(64):
+            pick = rng.choice(len(pool), int(rng.integers(4, 13)), replace=False)
+            sig = [sv(rng, pool[i][0], pool[i][1], 35, 50, 5000) for i in pick]
+            tones = [[(float(rng.uniform(0.2e6, 8.0e6)),
+                       math.sqrt(2.0 * 10.0 ** (rng.uniform(-10.0, 10.0) / 10.0)))
+                      for _ in range(3)] for _ in range(24)]
+            stream.append((sig, tones))
+        fc = grid(5000.0, 500.0)
+        return dict(desc="stream of 64 observations: all six systems (C/A N=2P, E1-C N=2P, "
+                         "B1C/L1C N=P), 24 antennas, 2-bit, 16.368 MHz, 10 ms, 4-12 SVs at "
+                         "35-50 dB-Hz and 3 CW tones per antenna; 21 bins (+-5 kHz, 500 Hz), "
+                         "FP32 + ACR + RFI report per observation",
+                    n_ant=24, L=163680, fmt=2, signals=stream[0][0], stream=stream,
+                    systems=[(s, list(PRNS[s]), fc)
+                             for s in ("gps", "sbas", "qzss", "galileo", "beidou", "l1c")])
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -134,7 +159,8 @@ def load_traffic(path, kernel):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="ca24", choices=["ca24", "ca1", "pilots24"])
+    ap.add_argument("--workload", default="ca24",
+                    choices=["ca24", "ca1", "pilots24", "stream64"])
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -164,9 +190,17 @@ def main():
     rng = np.random.default_rng(args.seed)
     w = workload(args.workload, rng)
     n_ant, L, fmt, signals = w["n_ant"], w["L"], w["fmt"], w["signals"]
+    stream = w.get("stream")
     t0 = time.perf_counter()
-    packed = synth.synth_observation(n_ant, L, FS, signals, fmt=fmt, seed=args.seed)
+    if stream:  # the observations one after another: [n_obs * n_ant, row_bytes]
+        packed = np.concatenate([synth.synth_observation(n_ant, L, FS, sig, fmt=fmt,
+                                                         seed=args.seed + k, tones=tones)
+                                 for k, (sig, tones) in enumerate(stream)])
+    else:
+        packed = synth.synth_observation(n_ant, L, FS, signals, fmt=fmt, seed=args.seed)
     t_synth = time.perf_counter() - t0
+    n_obs = len(stream) if stream else 1
+    obs_bytes = n_ant * packed.shape[1]
     systems = [tga.SystemSearch(s, fc, prns) for s, prns, fc in w["systems"]]
     t0 = time.perf_counter()
     plan = tga.Plan(FS, n_ant, L, systems, input_format=fmt)
@@ -187,14 +221,23 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    state = {"k": 0, "rfi": []}
+
     def step(pl):
-        peaks = pl.search_device(ptr)
+        k = state["k"] % n_obs  # stream: the next observation in HBM
+        state["k"] += 1
+        p = ptr + k * obs_bytes
+        peaks = pl.search_device(p)
         # ACR C/N0 per (antenna, PRN) in f64 on the host (acquisition.rs:341-357), natively
         cn0 = pl.cn0(peaks)
+        if stream:  # the narrowband RFI report of the same observation (rfi.rs:142-184)
+            state["rfi"].append((k, tga.rfi_device(p, n_ant, packed.shape[1], L, fmt, FS)))
         return peaks, int(sum(np.count_nonzero(~np.isnan(c)) for c in cn0))
 
     def timed(pl):
         """Warm-up, then exactly args.steps timed steps; returns (seconds, last peaks, cn0)."""
+        state["k"] = 0
+        state["rfi"] = []
         for _ in range(args.warmup):
             step(pl)
         pl.set_timing(True)
@@ -207,6 +250,17 @@ def main():
         return dt, pk, n_ok
 
     dt, peaks, n_cn0 = timed(plan)
+    rfi = None
+    if stream:  # injected CW tones found as report lines (within 10 kHz), over the timed steps
+        hit = tot = dead = 0
+        for k, chans in state["rfi"][-args.steps:]:
+            for a, ch in enumerate(chans):
+                dead += ch["dead"]
+                for f, _ in stream[k][1][a]:
+                    tot += 1
+                    hit += any(abs(ln["frequency"] - f) <= 10e3 for ln in ch["lines"])
+        rfi = {"tones_found": f"{hit}/{tot}", "dead_channels": dead,
+               "observations": len(state["rfi"][-args.steps:])}
     ms_step = dt * 1000.0 / args.steps
     value = cells * args.steps / dt
 
@@ -246,6 +300,8 @@ def main():
     # detections: every injected SV found at its code phase on most antennas
     found = 0
     sysidx = {s: i for i, (s, _, _) in enumerate(w["systems"])}
+    if stream:  # the observation of the last timed step
+        signals = stream[(state["k"] - 1) % n_obs][0]
     for s in signals:
         si = sysidx[s["system"]]
         P = periods[si]
@@ -292,7 +348,8 @@ def main():
         "cpu_baseline": cpu,
         "extra": {"forward_ms": fwd_ms / max(fwd_n, 1), "select_ms": sel_ms / max(sel_n, 1),
                   "plan_create_s": t_plan, "synth_s": t_synth, "cn0_valid": n_cn0,
-                  "injected_found": f"{found}/{len(signals)}",
+                  "injected_found": f"{found}/{len(signals)}", "rfi": rfi,
+                  "observations_per_s": args.steps / dt if stream else None,
                   "graph": plan.describe().strip().splitlines()[-1], "fp16_mfma": fp16},
     }
     print(json.dumps(out))

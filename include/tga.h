@@ -261,6 +261,10 @@ typedef struct tga_rfi_channel {
  * must exceed 16 (the largest ACF lag). */
 int tga_rfi(const uint8_t* packed, size_t n_ant, size_t row_bytes, int64_t record_samples,
             int32_t input_format, double sampling_freq, tga_rfi_channel* out);
+/* The same report for an observation already in device memory (d_packed: n_ant * row_bytes
+ * bytes of device memory), e.g. the buffer a tga_search_device call searches. */
+int tga_rfi_device(const void* d_packed, size_t n_ant, size_t row_bytes, int64_t record_samples,
+                   int32_t input_format, double sampling_freq, tga_rfi_channel* out);
 
 /* ---- host orchestration and JSON (acquire_all_* + lib.rs CombinedOutput) ---------------- */
 

@@ -284,10 +284,14 @@ void Plan::build(const tga_plan_desc& d) {
       g.d_Xs.alloc(g.n_xrow * M * sizeof(float2));
       upload(g.d_tw, fused_twiddles(), stream);
       upload(g.d_twb, big_twiddles(g.P), stream);
-      // sub-rows of at most ~2 GiB of rows at a time
-      const long long cap = std::max<long long>(1, (256LL << 20) / static_cast<long long>(M));
-      g.chunk_rows = std::min<long long>(cap, g.n_rows);
+      // sub-rows of a chunk of rows per pre-pass / transform pair, two chunks in flight;
+      // TGA_SPLIT_CHUNK_MB sets the bytes of one chunk's sub-rows (default 2048)
       const size_t y_elem = g.big16 ? sizeof(uint32_t) : sizeof(float2);
+      const char* env_mb = std::getenv("TGA_SPLIT_CHUNK_MB");
+      const long long chunk_bytes = (env_mb ? std::max(1, std::atoi(env_mb)) : 2048) * (1LL << 20);
+      const long long cap = std::max<long long>(
+          1, chunk_bytes / (static_cast<long long>(M) * static_cast<long long>(y_elem)));
+      g.chunk_rows = std::min<long long>(cap, g.n_rows);
       for (auto& y : g.d_Y) y.alloc(static_cast<size_t>(g.chunk_rows) * M * y_elem);
       if (!side) {
         TGA_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));

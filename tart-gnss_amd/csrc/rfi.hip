@@ -172,8 +172,9 @@ inline unsigned blocks_for(long long total, int b) {
 
 }  // namespace
 
-void rfi_run(const uint8_t* packed_host, size_t n_ant, size_t row_bytes, long long n, int fmt,
-             double fs, tga_rfi_channel* out) {
+// packed: host memory (copied in on the call's stream) or, with on_device, device memory
+void rfi_run(const uint8_t* packed, bool on_device, size_t n_ant, size_t row_bytes,
+             long long n, int fmt, double fs, tga_rfi_channel* out) {
   const int dev = current_device_checked();
   (void)dev;
   hipStream_t st;
@@ -182,11 +183,13 @@ void rfi_run(const uint8_t* packed_host, size_t n_ant, size_t row_bytes, long lo
     hipStream_t s;
     ~StreamGuard() { (void)hipStreamDestroy(s); }
   } sg{st};
-  DevBuf din(n_ant * row_bytes);
-  TGA_HIP(hipMemcpyAsync(din.p, packed_host, n_ant * row_bytes, hipMemcpyHostToDevice, st));
+  DevBuf din(on_device ? 0 : n_ant * row_bytes);
+  if (!on_device)
+    TGA_HIP(hipMemcpyAsync(din.p, packed, n_ant * row_bytes, hipMemcpyHostToDevice, st));
+  const uint8_t* d_in = on_device ? packed : din.as<uint8_t>();
   DevBuf dmom(n_ant * sizeof(Moments));
   hipLaunchKernelGGL(k_moments, dim3(static_cast<unsigned>(n_ant)), dim3(256), 0, st,
-                     din.as<uint8_t>(), row_bytes, n, fmt, dmom.as<Moments>());
+                     d_in, row_bytes, n, fmt, dmom.as<Moments>());
   TGA_HIP(hipGetLastError());
   std::vector<Moments> mom(n_ant);
   TGA_HIP(hipMemcpyAsync(mom.data(), dmom.p, n_ant * sizeof(Moments), hipMemcpyDeviceToHost, st));
@@ -215,7 +218,7 @@ void rfi_run(const uint8_t* packed_host, size_t n_ant, size_t row_bytes, long lo
     DevBuf z(static_cast<size_t>(total) * sizeof(double2));
     DevBuf zs(static_cast<size_t>(total) * sizeof(double2));
     hipLaunchKernelGGL(k_demean, dim3(blocks_for(total, 256)), dim3(256), 0, st,
-                       din.as<uint8_t>(), row_bytes, n, fmt, dmeans.as<double>(), total,
+                       d_in, row_bytes, n, fmt, dmeans.as<double>(), total,
                        z.as<double2>());
     TGA_HIP(hipGetLastError());
     double2* X = fft_forward_d(fft_plan_d(static_cast<size_t>(n)), z.as<double2>(),
@@ -325,20 +328,39 @@ void rfi_run(const uint8_t* packed_host, size_t n_ant, size_t row_bytes, long lo
 
 }  // namespace tga
 
+namespace tga {
+namespace {
+void rfi_checked(const uint8_t* packed, bool on_device, size_t n_ant, size_t row_bytes,
+                 int64_t record_samples, int32_t input_format, double sampling_freq,
+                 tga_rfi_channel* out) {
+  if (!packed || !out) fail(TGA_ERR_INVALID_ARGUMENT, "null argument");
+  if (n_ant == 0 || record_samples <= 16)
+    fail(TGA_ERR_INVALID_ARGUMENT, "need at least one antenna and more than 16 samples");
+  if (input_format != TGA_INPUT_1BIT && input_format != TGA_INPUT_2BIT)
+    fail(TGA_ERR_INVALID_ARGUMENT, "input_format must be TGA_INPUT_1BIT or TGA_INPUT_2BIT");
+  const int bits = input_format == TGA_INPUT_1BIT ? 1 : 2;
+  if (row_bytes * 8 < static_cast<size_t>(record_samples) * bits)
+    fail(TGA_ERR_INVALID_ARGUMENT, "row_bytes too small for record_samples");
+  if (!(sampling_freq > 0)) fail(TGA_ERR_INVALID_ARGUMENT, "sampling_freq must be positive");
+  rfi_run(packed, on_device, n_ant, row_bytes, record_samples, input_format, sampling_freq, out);
+}
+}  // namespace
+}  // namespace tga
+
 extern "C" int tga_rfi(const uint8_t* packed, size_t n_ant, size_t row_bytes,
                        int64_t record_samples, int32_t input_format, double sampling_freq,
                        tga_rfi_channel* out) {
-  using namespace tga;
-  return plan_guard([&] {
-    if (!packed || !out) fail(TGA_ERR_INVALID_ARGUMENT, "null argument");
-    if (n_ant == 0 || record_samples <= 16)
-      fail(TGA_ERR_INVALID_ARGUMENT, "need at least one antenna and more than 16 samples");
-    if (input_format != TGA_INPUT_1BIT && input_format != TGA_INPUT_2BIT)
-      fail(TGA_ERR_INVALID_ARGUMENT, "input_format must be TGA_INPUT_1BIT or TGA_INPUT_2BIT");
-    const int bits = input_format == TGA_INPUT_1BIT ? 1 : 2;
-    if (row_bytes * 8 < static_cast<size_t>(record_samples) * bits)
-      fail(TGA_ERR_INVALID_ARGUMENT, "row_bytes too small for record_samples");
-    if (!(sampling_freq > 0)) fail(TGA_ERR_INVALID_ARGUMENT, "sampling_freq must be positive");
-    rfi_run(packed, n_ant, row_bytes, record_samples, input_format, sampling_freq, out);
+  return tga::plan_guard([&] {
+    tga::rfi_checked(packed, false, n_ant, row_bytes, record_samples, input_format,
+                     sampling_freq, out);
+  });
+}
+
+extern "C" int tga_rfi_device(const void* d_packed, size_t n_ant, size_t row_bytes,
+                              int64_t record_samples, int32_t input_format,
+                              double sampling_freq, tga_rfi_channel* out) {
+  return tga::plan_guard([&] {
+    tga::rfi_checked(static_cast<const uint8_t*>(d_packed), true, n_ant, row_bytes,
+                     record_samples, input_format, sampling_freq, out);
   });
 }

@@ -876,7 +876,9 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
     }
   } else if constexpr (SUB == 0) {
     // Blocks are dealt round-robin to the 8 XCDs: the PRN group follows blockIdx % groups so
-    // the replica spectra a group touches stay in one XCD's L2.
+    // the replica spectra a group touches stay in one XCD's L2.  One block per (xrow, group):
+    // an outer loop over several items per block (the persistent form of the FP16 kernel)
+    // measured 45% slower here, even when it ran once per block.
     const int g = blockIdx.x % a.groups;
     const int xrow = blockIdx.x / a.groups;
     const int h = xrow & 1;
@@ -958,6 +960,7 @@ struct Row16Args {
   unsigned long long* rowkey = nullptr;
   int* rownan = nullptr;
   int hs = 1;
+  long long n_items = 0;  // SUB == 0: (xrow, PRN group) items, n_xrow * groups
 };
 
 // conj(x) * r on f16 pairs (two packed instructions)
@@ -1052,10 +1055,17 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
   for (int t = 0; t < 4; ++t)
 #pragma unroll
     for (int kb = 0; kb < 2; ++kb) fc[t][kb] = dft_frag<31>(stw, t, kb, tid & 63);
-  // SUB == 0: one (antenna, bin, hypothesis) row of X per block and the PRNs of its group, dealt
-  // round-robin to the XCDs (blockIdx % groups); SUB > 0: sub-rows, grid-stride
-  const int g = SUB == 0 ? blockIdx.x % a.groups : 0;
-  const int xrow = SUB == 0 ? blockIdx.x / a.groups : 0;
+  __syncthreads();
+  RowPending pend;  // the previous row, written after this row's first barrier
+  // SUB == 0: persistent blocks over the items (xrow, PRN group) b, b + gridDim, ... with
+  // item = xrow * groups + g; gridDim is a multiple of groups, so a block keeps g = b mod groups
+  // and, blocks being dealt round-robin to the 8 XCDs, a group's replicas stay in one XCD's L2.
+  // SUB > 0: one pass over the sub-rows, grid-stride.
+  const long long n_items = SUB == 0 ? a.n_items : 1;
+  for (long long item = SUB == 0 ? blockIdx.x : 0; item < n_items;
+       item += SUB == 0 ? gridDim.x : 1) {
+  const int g = SUB == 0 ? static_cast<int>(item % a.groups) : 0;
+  const int xrow = SUB == 0 ? static_cast<int>(item / a.groups) : 0;
   const int h = xrow & 1;
   const int bin = SUB == 0 ? (xrow >> 1) % a.n_bin : 0;
   const int ant = SUB == 0 ? (xrow >> 1) / a.n_bin : 0;
@@ -1064,8 +1074,6 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
   const long long step = SUB == 0 ? 1 : gridDim.x;
   const __amdgpu_buffer_rsrc_t xv =
       buffer_view(SUB == 0 ? a.X + static_cast<size_t>(xrow) * P : a.Y, P * 4);
-  __syncthreads();
-  RowPending pend;  // the previous row, written after this row's first barrier
   for (long long it = it0; it < it1; it += step) {
     const int prn = static_cast<int>(it);
     // SUB == 0: replica row; SUB > 0: the sub-row itself
@@ -1251,6 +1259,7 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
       row_finish_key<T16>(main_set, extra_set, !(accn <= 3.0e38f), red, k0, SUB, a.rowkey + r,
                           a.rownan + r);
     }
+  }
   }
   if constexpr (SUB == 0) {
     lds_barrier();
@@ -1569,6 +1578,20 @@ void launch_fused_forward(const uint8_t* packed, size_t row_bytes, int fmt, cons
   TGA_HIP(hipGetLastError());
 }
 
+// Grid of a persistent rows kernel (one workgroup per CU: the LDS row takes the CU): the
+// device's CU count rounded down to a multiple of `groups`, or one block per item when fewer.
+long long persistent_blocks(long long n_items, int groups) {
+  static thread_local int cus = 0, dev = -1;
+  int d = 0;
+  TGA_HIP(hipGetDevice(&d));
+  if (d != dev) {
+    TGA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d));
+    dev = d;
+  }
+  const long long per_wave = std::max<long long>(groups, (cus / groups) * groups);
+  return std::min(n_items, per_wave);
+}
+
 void launch_fused_rows(const float2* X, const float2* rep, const float2* tw, int n_ant,
                        int n_bin, int n_prn, int cw, RowOut* rows, hipStream_t st,
                        const FusedRowsConfig& cfg) {
@@ -1625,7 +1648,8 @@ void launch_fused_rows16(const uint32_t* X16, const uint32_t* rep16, const float
   groups = n_prn < groups ? n_prn : groups;
   const int per = (n_prn + groups - 1) / groups;
   fused::Row16Args a{X16, rep16, tw, n_bin, n_prn, groups, per, cw, rows};
-  const long long blocks = static_cast<long long>(n_ant) * n_bin * 2 * groups;
+  a.n_items = static_cast<long long>(n_ant) * n_bin * 2 * groups;
+  const long long blocks = persistent_blocks(a.n_items, groups);
   hipLaunchKernelGGL(fused::k_rows16<0>, dim3(static_cast<unsigned>(blocks)),
                      dim3(fused::T16), 0, st, a);
   TGA_HIP(hipGetLastError());

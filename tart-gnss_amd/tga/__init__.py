@@ -28,6 +28,6 @@ from .api import (AcquisitionResult, CorrelationPeak, Plan, SystemSearch,  # noq
                   b1c_code_resampled, code_chips, code_length, code_resampled, code_spectra,
                   correlate_code, device_count, e1c_code_resampled, estimate_cn0, format_f64,
                   gold_code,
-                  l1c_code_resampled, qzss_gold_code, rfi, sbas_gold_code,
+                  l1c_code_resampled, qzss_gold_code, rfi, rfi_device, sbas_gold_code,
                   search_frequencies)
 from .observation import Observation, read_observation  # noqa: F401,E402

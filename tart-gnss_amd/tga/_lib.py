@@ -99,6 +99,8 @@ PROTOTYPES = {
     "tga_acr_cn0": (C.c_int, [C.c_double, C.c_double, C.c_int32, _dp, C.POINTER(C.c_int32)]),
     "tga_rfi": (C.c_int, [_u8p, C.c_size_t, C.c_size_t, C.c_int64, C.c_int32, C.c_double,
                           C.POINTER(RfiChannel)]),
+    "tga_rfi_device": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, C.c_int64, C.c_int32,
+                                 C.c_double, C.POINTER(RfiChannel)]),
     "tga_acquire_all_json": (C.c_int, [C.c_void_p, _u8p, C.c_size_t, C.c_size_t, _dp,
                                        C.POINTER(C.c_int64), C.c_int32, C.c_double,
                                        C.c_double, C.POINTER(C.c_void_p)]),

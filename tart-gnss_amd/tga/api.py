@@ -446,6 +446,17 @@ def format_f64(x):
     return buf.value.decode()
 
 
+def _rfi_channels(out):
+    def opt(x):
+        return None if np.isnan(x) else float(x)
+
+    return [{"antenna": ch.antenna, "dead": bool(ch.dead), "acf_lag1": opt(ch.acf_lag1),
+             "acf_lag3": opt(ch.acf_lag3), "acf_lag16": opt(ch.acf_lag16),
+             "period_samples": opt(ch.period_samples),
+             "lines": [{"frequency": ch.lines[i].frequency, "excess_db": ch.lines[i].excess_db}
+                       for i in range(ch.n_lines)]} for ch in out]
+
+
 def rfi(packed, record_samples, input_format, sampling_freq):
     """Narrowband RFI report per antenna (rfi.rs:142-184) on the GPU; a list of dicts shaped
     like the reference's RfiChannel (None where the reference has None)."""
@@ -455,12 +466,13 @@ def rfi(packed, record_samples, input_format, sampling_freq):
     out = (L.RfiChannel * pk.shape[0])()
     check(lib().tga_rfi(_ptr(pk, C.c_uint8), pk.shape[0], pk.shape[1], int(record_samples),
                         int(input_format), float(sampling_freq), out))
+    return _rfi_channels(out)
 
-    def opt(x):
-        return None if np.isnan(x) else float(x)
 
-    return [{"antenna": ch.antenna, "dead": bool(ch.dead), "acf_lag1": opt(ch.acf_lag1),
-             "acf_lag3": opt(ch.acf_lag3), "acf_lag16": opt(ch.acf_lag16),
-             "period_samples": opt(ch.period_samples),
-             "lines": [{"frequency": ch.lines[i].frequency, "excess_db": ch.lines[i].excess_db}
-                       for i in range(ch.n_lines)]} for ch in out]
+def rfi_device(dev_ptr, n_ant, row_bytes, record_samples, input_format, sampling_freq):
+    """rfi() for an observation already in device memory (an integer device address)."""
+    out = (L.RfiChannel * int(n_ant))()
+    check(lib().tga_rfi_device(C.c_void_p(int(dev_ptr)), int(n_ant), int(row_bytes),
+                               int(record_samples), int(input_format), float(sampling_freq),
+                               out))
+    return _rfi_channels(out)

@@ -223,3 +223,13 @@ def test_rfi_errors(gpu):
         tga.rfi(pk, 64, 3, FS)
     with pytest.raises(tga.TgaError):
         tga.rfi(pk, 64, 1, 0.0)
+
+
+def test_rfi_device_input_matches_host_input(rfi_1bit):
+    # tga_rfi_device on the same packed rows already in device memory (torch only allocates)
+    import torch
+    packed, host = rfi_1bit
+    dev = torch.from_numpy(np.ascontiguousarray(packed)).cuda()
+    torch.cuda.synchronize()
+    got = tga.rfi_device(dev.data_ptr(), packed.shape[0], packed.shape[1], N_RFI, 1, FS)
+    assert got == host
