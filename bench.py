@@ -14,10 +14,13 @@ Other workloads (same JSON line):
   pilots24 Galileo E1-C (P = 65472), BeiDou B1C and GPS L1C (P = 163680), 24 antennas, 1-bit,
            40 ms: 6 signals per system at 40-48 dB-Hz; E1-C 50 SVs x 81 bins (125 Hz), B1C and
            L1C 63 SVs x 201 bins (50 Hz), N = 2P.
+  all24    all six systems, 24 antennas, 2-bit, 40 ms: the C/A signals and grid of ca24 and the
+           pilot signals and grids of pilots24 in one plan (1.0746e11 cells).
   stream64 64 independent 24-antenna observations (2-bit, 10 ms) resident in HBM, each a random
            4-12 SV mix from all six systems at 35-50 dB-Hz plus 3 CW tones per antenna at
-           -10..+10 dB; step k searches observation k mod 64 (all systems, 21 bins), computes its
-           ACR C/N0 and its RFI report; extra.rfi counts the tones found as report lines.
+           -10..+10 dB; step k searches observation k mod 64 (all systems, the grids of all24),
+           computes its ACR C/N0 and its RFI report; extra.rfi counts the tones found as report
+           lines.  About 1 s per step: use --steps 16 for a short run.
 
 A step = one tga_search_device call on an observation already resident in HBM: decode +
 de-mean, forward spectra, all correlation rows, selection, copy of the peaks to the host, and
@@ -103,15 +106,37 @@ def workload(name, rng):
                        math.sqrt(2.0 * 10.0 ** (rng.uniform(-10.0, 10.0) / 10.0)))
                       for _ in range(3)] for _ in range(24)]
             stream.append((sig, tones))
-        fc = grid(5000.0, 500.0)
         return dict(desc="stream of 64 observations: all six systems (C/A N=2P, E1-C N=2P, "
                          "B1C/L1C N=P), 24 antennas, 2-bit, 16.368 MHz, 10 ms, 4-12 SVs at "
-                         "35-50 dB-Hz and 3 CW tones per antenna; 21 bins (+-5 kHz, 500 Hz), "
-                         "FP32 + ACR + RFI report per observation",
+                         "35-50 dB-Hz and 3 CW tones per antenna; C/A +-10 kHz / 500 Hz, E1-C "
+                         "+-5 kHz / 125 Hz, B1C+L1C +-5 kHz / 50 Hz, FP32 + ACR + RFI report per "
+                         "observation",
                     n_ant=24, L=163680, fmt=2, signals=stream[0][0], stream=stream,
-                    systems=[(s, list(PRNS[s]), fc)
-                             for s in ("gps", "sbas", "qzss", "galileo", "beidou", "l1c")])
+                    systems=all_systems())
+    if name == "all24":
+        sig = [sv(rng, "gps", p, 38, 50, 5000) for p in rng.choice(range(1, 33), 8, replace=False)]
+        sig += [sv(rng, "sbas", p, 38, 50, 5000)
+                for p in rng.choice(range(120, 159), 2, replace=False)]
+        sig += [sv(rng, "qzss", p, 38, 50, 5000)
+                for p in rng.choice(range(184, 207), 1, replace=False)]
+        for s, hi in (("galileo", 51), ("beidou", 64), ("l1c", 64)):
+            sig += [sv(rng, s, p, 40, 48, 5000) for p in rng.choice(range(1, hi), 6, replace=False)]
+        return dict(desc="all six systems, 24 antennas, 2-bit, 16.368 MHz, 40 ms; C/A 100 PRNs x "
+                         "41 bins (+-10 kHz, 500 Hz), E1-C 50 x 81 (125 Hz), B1C + L1C 2 x 63 x 201 "
+                         "(50 Hz), N=2P, FP32 + ACR (FP16 form beside it)",
+                    n_ant=24, L=654720, fmt=2, signals=sig, systems=all_systems())
     raise SystemExit(f"unknown workload {name}")
+
+
+def all_systems():
+    """C/A grid of the 24-antenna C/A workload, pilot grids of the pilots workload: bins as fine
+    as each coherent length needs (first sinc null 500 / 125 / 100 Hz off)."""
+    step = {"gps": 500.0, "sbas": 500.0, "qzss": 500.0, "galileo": 125.0, "beidou": 50.0,
+            "l1c": 50.0}
+    half = {"gps": 10000.0, "sbas": 10000.0, "qzss": 10000.0, "galileo": 5000.0,
+            "beidou": 5000.0, "l1c": 5000.0}
+    return [(s, list(PRNS[s]), grid(half[s], step[s]))
+            for s in ("gps", "sbas", "qzss", "galileo", "beidou", "l1c")]
 
 
 class DeviceInput:
@@ -160,7 +185,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="ca24",
-                    choices=["ca24", "ca1", "pilots24", "stream64"])
+                    choices=["ca24", "ca1", "pilots24", "all24", "stream64"])
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -221,7 +246,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    state = {"k": 0, "rfi": []}
+    state = {"k": 0, "rfi": [], "peaks": []}
 
     def step(pl):
         k = state["k"] % n_obs  # stream: the next observation in HBM
@@ -232,12 +257,14 @@ def main():
         cn0 = pl.cn0(peaks)
         if stream:  # the narrowband RFI report of the same observation (rfi.rs:142-184)
             state["rfi"].append((k, tga.rfi_device(p, n_ant, packed.shape[1], L, fmt, FS)))
+            state["peaks"].append((k, peaks))
         return peaks, int(sum(np.count_nonzero(~np.isnan(c)) for c in cn0))
 
     def timed(pl):
         """Warm-up, then exactly args.steps timed steps; returns (seconds, last peaks, cn0)."""
         state["k"] = 0
         state["rfi"] = []
+        state["peaks"] = []
         for _ in range(args.warmup):
             step(pl)
         pl.set_timing(True)
@@ -250,6 +277,7 @@ def main():
         return dt, pk, n_ok
 
     dt, peaks, n_cn0 = timed(plan)
+    stream_peaks = state["peaks"][-args.steps:]
     rfi = None
     if stream:  # injected CW tones found as report lines (within 10 kHz), over the timed steps
         hit = tot = dead = 0
@@ -298,17 +326,24 @@ def main():
                 "plan": plan16.describe().splitlines()[0]}
 
     # detections: every injected SV found at its code phase on most antennas
-    found = 0
     sysidx = {s: i for i, (s, _, _) in enumerate(w["systems"])}
-    if stream:  # the observation of the last timed step
-        signals = stream[(state["k"] - 1) % n_obs][0]
-    for s in signals:
-        si = sysidx[s["system"]]
-        P = periods[si]
-        j = list(plan.prns[si]).index(s["prn"])
-        cp = (P - s["delay"]) % P
-        hits = sum(abs(int(peaks[si][a, j]["best_codephase"]) - cp) <= 1 for a in range(n_ant))
-        found += hits >= n_ant // 2
+
+    def detections(sigs, pk):
+        n = 0
+        for s in sigs:
+            si = sysidx[s["system"]]
+            P = periods[si]
+            j = list(plan.prns[si]).index(s["prn"])
+            cp = (P - s["delay"]) % P
+            hits = sum(abs(int(pk[si][a, j]["best_codephase"]) - cp) <= 1 for a in range(n_ant))
+            n += hits >= n_ant // 2
+        return n
+
+    if stream:  # every observation of the timed steps
+        found = sum(detections(stream[k][0], pk) for k, pk in stream_peaks)
+        signals = [s for k, _ in stream_peaks for s in stream[k][0]]
+    else:
+        found = detections(signals, peaks)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -327,10 +362,24 @@ def main():
             n_ants += 1
             if t_s >= args.cpu_budget:
                 break
+        # the same port on one thread, over the first system of antenna 0, for a few seconds
+        c1, t1, d1 = cpu_port.time_sample(levels[0], FS, w["systems"][0][0], w["systems"][0][1],
+                                          w["systems"][0][2], 1, budget_s=args.cpu_budget / 4)
+        model = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next((ln.split(":", 1)[1].strip() for ln in f
+                              if ln.startswith("model name")), "")
+        except OSError:
+            pass
         cpu = {"value": t_cells / t_s, "unit": "cells/s", "cores": workers, "kind": "port",
                "sample": f"{n_done} (antenna, PRN) searches over {n_ants} antenna(s), N=2P, "
                          f"complex64 pocketfft over {workers} threads, reference work structure "
-                         f"(2 replica + 3 FFTs per bin per PRN), {t_s:.1f} s"}
+                         f"(2 replica + 3 FFTs per bin per PRN), {t_s:.1f} s",
+               "value_1_core": c1 / t1 if t1 > 0 else None,
+               "sample_1_core": f"{d1} PRN searches of {w['systems'][0][0]} on antenna 0, "
+                                f"{t1:.1f} s",
+               "host": {"nproc": os.cpu_count(), "model": model}}
 
     out = {
         "metric": METRIC, "value": value, "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
