@@ -213,6 +213,7 @@ void Plan::build(const tga_plan_desc& d) {
 
   // ---- device state per group -------------------------------------------------------------
   d_means.alloc(static_cast<size_t>(n_ant) * sizeof(double));
+  d_partial.alloc(static_cast<size_t>(n_ant) * kAntSplit * sizeof(long long));
   d_input.alloc(static_cast<size_t>(n_ant) * row_bytes);
   d_peaks.alloc(n_peaks * sizeof(tga_peak));
   d_nan.alloc(sizeof(int));
@@ -442,7 +443,8 @@ void Plan::time_collect() {
 
 void Plan::search(const uint8_t* d_in, hipStream_t st) {
   if (st == nullptr) st = stream;
-  launch_ant_sums(d_in, row_bytes, n_ant, L, fmt, d_means.as<double>(), d_nan.as<int>(), st);
+  launch_ant_sums(d_in, row_bytes, n_ant, L, fmt, d_partial.as<long long>(),
+                  d_means.as<double>(), d_nan.as<int>(), st);
   double rows_bytes = 0.0, fwd_bytes = 0.0;
   for (auto& g : groups) {
     rows_bytes += g.row_bytes_alg;

@@ -60,7 +60,7 @@ struct Plan {
   size_t row_bytes = 0, n_peaks = 0;
   std::vector<SysEntry> systems;
   std::vector<Group> groups;
-  DevBuf d_means, d_input, d_peaks, d_nan;
+  DevBuf d_means, d_partial, d_input, d_peaks, d_nan;
   hipStream_t stream = nullptr;
   // long-period split: the sub-row pre-pass of chunk c + 1 runs on `side` while the rows
   // kernel transforms chunk c; ev_pre / ev_rows hand the two scratch buffers over

@@ -25,9 +25,11 @@ struct PeakSlot {
 };
 
 // ---- decoding / de-meaning --------------------------------------------------------------
-// Per-antenna means of the decoded levels; also zeroes *nan_flag (when non-null).
+// Per-antenna means of the decoded levels (exact integer sums over kAntSplit slices per
+// antenna, partial: n_ant * kAntSplit scratch); also zeroes *nan_flag (when non-null).
+constexpr int kAntSplit = 64;
 void launch_ant_sums(const uint8_t* packed, size_t row_bytes, int n_ant, int64_t L, int fmt,
-                     double* means, int* nan_flag, hipStream_t st);
+                     long long* partial, double* means, int* nan_flag, hipStream_t st);
 
 // ---- fused path, code period P = 16368 (16.368 MHz C/A) --------------------------------
 constexpr int kFusedP = 16368;

@@ -140,25 +140,36 @@ __device__ __forceinline__ int ring_dist(int a, int b, int P) {
 }
 
 // ---- antenna sums ---------------------------------------------------------------------------
-// Also clears the search's NaN flag (block 0), so a search has no memset between kernels and
-// a captured graph of it holds kernel nodes only.
-__global__ __launch_bounds__(256) void k_ant_sums(const uint8_t* __restrict__ packed,
-                                                  size_t row_bytes, long long L, int fmt,
-                                                  double* __restrict__ means,
-                                                  int* __restrict__ nan_flag) {
+// Exact integer sums of each antenna's decoded levels, in kAntSplit slices per antenna so the
+// grid fills the GPU even for one antenna: partial[ant * kAntSplit + slice].
+__global__ __launch_bounds__(256) void k_ant_partials(const uint8_t* __restrict__ packed,
+                                                      size_t row_bytes, long long L, int fmt,
+                                                      long long* __restrict__ partial) {
   __shared__ long long red[4];
-  if (nan_flag != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *nan_flag = 0;
-  const uint8_t* row = packed + static_cast<size_t>(blockIdx.x) * row_bytes;
+  const int ant = blockIdx.x / kAntSplit;
+  const int slice = blockIdx.x - ant * kAntSplit;
+  const long long chunk = (L + kAntSplit - 1) / kAntSplit;
+  const long long n0 = slice * chunk;
+  const long long n1 = n0 + chunk < L ? n0 + chunk : L;
+  const uint8_t* row = packed + static_cast<size_t>(ant) * row_bytes;
   long long acc = 0;
-  for (long long n = threadIdx.x; n < L; n += 256) acc += decode_level(row, fmt, n);
-  // reduce 64-bit sums
+  for (long long n = n0 + threadIdx.x; n < n1; n += 256) acc += decode_level(row, fmt, n);
   for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const long long s = red[0] + red[1] + red[2] + red[3];
-    means[blockIdx.x] = static_cast<double>(s) / static_cast<double>(L);
-  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// Per-antenna mean: the slices' exact sum over one wave, then one f64 division.  Also clears
+// the search's NaN flag (block 0), so a search has no memset between kernels and a captured
+// graph of it holds kernel nodes only.
+__global__ __launch_bounds__(kAntSplit) void k_ant_means(const long long* __restrict__ partial,
+                                                         long long L, double* __restrict__ means,
+                                                         int* __restrict__ nan_flag) {
+  if (nan_flag != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *nan_flag = 0;
+  long long v = partial[static_cast<size_t>(blockIdx.x) * kAntSplit + threadIdx.x];
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  if (threadIdx.x == 0) means[blockIdx.x] = static_cast<double>(v) / static_cast<double>(L);
 }
 
 // ================================ fused P = 16368 path ===================================
@@ -1546,9 +1557,12 @@ inline unsigned grid_for(long long total, int block) {
 
 // ---- launchers -------------------------------------------------------------------------------
 void launch_ant_sums(const uint8_t* packed, size_t row_bytes, int n_ant, int64_t L, int fmt,
-                     double* means, int* nan_flag, hipStream_t st) {
-  hipLaunchKernelGGL(k_ant_sums, dim3(n_ant), dim3(256), 0, st, packed, row_bytes,
-                     static_cast<long long>(L), fmt, means, nan_flag);
+                     long long* partial, double* means, int* nan_flag, hipStream_t st) {
+  hipLaunchKernelGGL(k_ant_partials, dim3(n_ant * kAntSplit), dim3(256), 0, st, packed,
+                     row_bytes, static_cast<long long>(L), fmt, partial);
+  TGA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_ant_means, dim3(n_ant), dim3(kAntSplit), 0, st, partial,
+                     static_cast<long long>(L), means, nan_flag);
   TGA_HIP(hipGetLastError());
 }
 
