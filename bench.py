@@ -11,6 +11,7 @@ observation and is reported under extra.fp16_mfma with its difference to the FP3
 Other workloads (same JSON line):
   ca1      GPS L1 C/A, 1 antenna, 1-bit, 10 ms: 4 signals at 45 dB-Hz, PRN 1-32 x 21 bins
            (+-5 kHz, 500 Hz).
+  ca1nc    ca1 with 1 ms coherent x 10 non-coherent epochs (the non-coherent form; same cells).
   pilots24 Galileo E1-C (P = 65472), BeiDou B1C and GPS L1C (P = 163680), 24 antennas, 1-bit,
            40 ms: 6 signals per system at 40-48 dB-Hz; E1-C 50 SVs x 81 bins (125 Hz), B1C and
            L1C 63 SVs x 201 bins (50 Hz), N = 2P.
@@ -82,6 +83,12 @@ def workload(name, rng):
                          "(+-5 kHz, 500 Hz), N=2P=32736, FP32 + ACR",
                     n_ant=1, L=163680, fmt=1, signals=sig,
                     systems=[("gps", list(range(1, 33)), grid(5000.0, 500.0))])
+    if name == "ca1nc":
+        w = workload("ca1", rng)
+        w["desc"] = ("GPS L1 C/A, 1 antenna, 1-bit, 16.368 MHz, 10 ms; PRN 1-32 x 21 bins "
+                     "(+-5 kHz, 500 Hz), 1 ms coherent x 10 non-coherent, FP32")
+        w["nc_epochs"] = 10
+        return w
     if name == "pilots24":
         sig = []
         for s, hi in (("galileo", 51), ("beidou", 64), ("l1c", 64)):
@@ -170,12 +177,15 @@ class DeviceInput:
             raise RuntimeError(f"{what} failed with HIP error {rc}")
 
 
-def load_traffic(path, kernel):
+def load_traffic(path, kernel, workload_name):
     """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
-    from the committed summary; None if absent."""
+    from the committed summary; None if absent or measured on another workload (the bytes of a
+    launch depend on its rows)."""
     try:
         with open(path) as f:
             d = json.load(f)
+        if d.get("workload") != workload_name:
+            return None
         return float(d["kernels"][kernel]["hbm_bytes_per_launch"])
     except (OSError, KeyError, ValueError):
         return None
@@ -185,7 +195,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="ca24",
-                    choices=["ca24", "ca1", "pilots24", "all24", "stream64"])
+                    choices=["ca24", "ca1", "ca1nc", "pilots24", "all24", "stream64"])
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -226,7 +236,8 @@ def main():
     t_synth = time.perf_counter() - t0
     n_obs = len(stream) if stream else 1
     obs_bytes = n_ant * packed.shape[1]
-    systems = [tga.SystemSearch(s, fc, prns) for s, prns, fc in w["systems"]]
+    nc = w.get("nc_epochs", 0)
+    systems = [tga.SystemSearch(s, fc, prns, nc_epochs=nc) for s, prns, fc in w["systems"]]
     t0 = time.perf_counter()
     plan = tga.Plan(FS, n_ant, L, systems, input_format=fmt)
     t_plan = time.perf_counter() - t0
@@ -297,11 +308,11 @@ def main():
     sel_ms, sel_n, _ = plan.timing(2)
     avg_rows_s = rows_ms / 1000.0 / max(rows_n, 1)
     achieved = rows_bytes / avg_rows_s / 1e9
-    traffic = load_traffic(args.traffic, plan.kernel_name(0))
+    traffic = load_traffic(args.traffic, plan.kernel_name(0), args.workload)
 
     fp16 = None
     desc = plan.describe()
-    if not args.no_fp16 and ("fused16368" in desc or "split" in desc):
+    if not args.no_fp16 and not nc and ("fused16368" in desc or "split" in desc):
         # the FP16 matrix-core form on the same observation, compared with the FP32 peaks
         plan16 = tga.Plan(FS, n_ant, L, systems, input_format=fmt, precision="fp16")
         dt16, peaks16, _ = timed(plan16)
@@ -355,7 +366,8 @@ def main():
         for a in range(n_ant):  # whole antennas, then whole systems, until the budget is spent
             for s, prns, fc in w["systems"]:
                 c, t, d = cpu_port.time_sample(levels[a], FS, s, prns, fc, workers,
-                                               budget_s=max(args.cpu_budget - t_s, 0.0))
+                                               budget_s=max(args.cpu_budget - t_s, 0.0),
+                                               epochs=nc)
                 t_cells += c
                 t_s += t
                 n_done += d
@@ -364,7 +376,8 @@ def main():
                 break
         # the same port on one thread, over the first system of antenna 0, for a few seconds
         c1, t1, d1 = cpu_port.time_sample(levels[0], FS, w["systems"][0][0], w["systems"][0][1],
-                                          w["systems"][0][2], 1, budget_s=args.cpu_budget / 4)
+                                          w["systems"][0][2], 1, budget_s=args.cpu_budget / 4,
+                                          epochs=nc)
         model = ""
         try:
             with open("/proc/cpuinfo") as f:
@@ -373,9 +386,12 @@ def main():
         except OSError:
             pass
         cpu = {"value": t_cells / t_s, "unit": "cells/s", "cores": workers, "kind": "port",
-               "sample": f"{n_done} (antenna, PRN) searches over {n_ants} antenna(s), N=2P, "
-                         f"complex64 pocketfft over {workers} threads, reference work structure "
-                         f"(2 replica + 3 FFTs per bin per PRN), {t_s:.1f} s",
+               "sample": (f"{n_done} (antenna, PRN) searches over {n_ants} antenna(s), "
+                          + (f"{nc} one-period epochs per bin (1 replica FFT per PRN, "
+                             f"{nc} forward + {nc} inverse FFTs per bin)" if nc else
+                             "N=2P, reference work structure (2 replica + 3 FFTs per bin per "
+                             "PRN)")
+                          + f", complex64 pocketfft over {workers} threads, {t_s:.1f} s"),
                "value_1_core": c1 / t1 if t1 > 0 else None,
                "sample_1_core": f"{d1} PRN searches of {w['systems'][0][0]} on antenna 0, "
                                 f"{t1:.1f} s",
@@ -388,7 +404,8 @@ def main():
         "config": {"workload": w["desc"], "name": args.workload, "n_ant": n_ant,
                    "n_prn": n_prn, "n_bins": [len(fc) for _, _, fc in w["systems"]],
                    "code_periods": periods, "cells": cells,
-                   "form": "reference (dual hypothesis, folded P-point rows)",
+                   "form": (f"non-coherent ({nc} one-period epochs, powers summed)" if nc
+                            else "reference (dual hypothesis, folded P-point rows)"),
                    "plan": plan.describe().strip().splitlines()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -39,19 +39,41 @@ def search_prn(sig32, pp, code, fc, N, P, workers):
     return best
 
 
-def time_sample(levels, fs, system, prns, fc, workers, budget_s=10.0):
+def search_prn_nc(sig32, pp, code, fc, P, epochs, workers):
+    """One (antenna, PRN) non-coherent search in the same complex64 work structure: one P-point
+    replica FFT, then per bin one wipe-off of E periods, E forward and E inverse P-point FFTs,
+    |y|^2 summed over the epochs and the argmax.  Returns (best_power, best_bin, codephase)."""
+    cc = np.conj(sfft.fft(code[:P].astype(np.complex64), workers=workers))
+    ph = np.multiply.outer(np.asarray(fc), pp[:epochs * P]).astype(np.float32)
+    s = sig32[:epochs * P]
+    iq = (np.cos(ph) * s + 1j * (np.sin(ph) * s)).astype(np.complex64)
+    X = sfft.fft(iq.reshape(len(fc), epochs, P), axis=-1, workers=workers)
+    y = sfft.ifft(X * cc, axis=-1, workers=workers, norm="forward")
+    pw = (y.real * y.real + y.imag * y.imag).sum(axis=1)
+    idx = pw.argmax(axis=-1)
+    val = pw[np.arange(len(fc)), idx]
+    b = int(val.argmax())
+    return float(val[b]), b, int(idx[b])
+
+
+def time_sample(levels, fs, system, prns, fc, workers, budget_s=10.0, epochs=0):
     """Run whole PRNs on one antenna until ``budget_s`` elapses; returns
-    (cells searched, seconds, PRNs done)."""
+    (cells searched, seconds, PRNs done).  epochs > 0: the non-coherent form over that many
+    one-period epochs."""
     P = codes.samples_per_period(system, fs)
-    N = min(2 * P, len(levels))
+    N = epochs * P if epochs > 0 else min(2 * P, len(levels))
     lv = np.asarray(levels, dtype=np.float64)
     sig32 = (lv - lv.sum() / len(lv))[:N].astype(np.float32)
     pp = (2.0 * math.pi / fs) * np.arange(N, dtype=np.float64)
     done = 0
     t0 = time.perf_counter()
     for prn in prns:
-        code = codes.code_resampled(system, prn, float(P), N)
-        search_prn(sig32, pp, code, fc, N, P, workers)
+        if epochs > 0:
+            code = codes.code_resampled(system, prn, float(P), P)
+            search_prn_nc(sig32, pp, code, fc, P, epochs, workers)
+        else:
+            code = codes.code_resampled(system, prn, float(P), N)
+            search_prn(sig32, pp, code, fc, N, P, workers)
         done += 1
         if time.perf_counter() - t0 >= budget_s:
             break

@@ -276,6 +276,7 @@ void Plan::build(const tga_plan_desc& d) {
     } else if (g.nc > 0) {
       g.d_X.alloc(g.n_xrow * M * sizeof(float2));
       upload(g.d_tw, fused_twiddles(), stream);
+      autotune(g);
     } else if (g.fused) {
       g.d_X.alloc(g.n_xrow * M * sizeof(float2));
       upload(g.d_tw, fused_twiddles(), stream);
@@ -354,8 +355,10 @@ void Plan::autotune(Group& g) {
       if (gr == 8 || gr / 2 < g.n_prn) grs.push_back(gr);
   }
   std::vector<FusedRowsConfig> cands;
-  for (int v = 0; v < (g.fp16 ? 1 : 2); ++v) {
-    if (pin_v && !g.fp16 && std::atoi(pin_v) != v) continue;
+  // the FP16 and non-coherent kernels have one radix-31 implementation
+  const bool one_variant = g.fp16 || g.nc > 0;
+  for (int v = 0; v < (one_variant ? 1 : 2); ++v) {
+    if (pin_v && !one_variant && std::atoi(pin_v) != v) continue;
     for (int gr : grs) {
       FusedRowsConfig c;
       c.variant = v;
@@ -369,7 +372,10 @@ void Plan::autotune(Group& g) {
   DevBuf& xs = g.fp16 ? g.d_X16 : g.d_X;
   TGA_HIP(hipMemsetAsync(xs.p, 0, xs.bytes, stream));
   auto launch = [&](const FusedRowsConfig& c) {
-    if (g.fp16)
+    if (g.nc > 0)
+      launch_fused_rows_nc(g.d_X.as<float2>(), g.d_rep.as<float2>(), g.d_tw.as<float2>(), n_ant,
+                           g.n_bin, g.n_prn, g.nc, g.cw, g.d_rows.as<RowOut>(), c.groups, stream);
+    else if (g.fp16)
       launch_fused_rows16(g.d_X16.as<uint32_t>(), g.d_rep16.as<uint32_t>(), g.d_tw.as<float2>(),
                           n_ant, g.n_bin, g.n_prn, g.cw, g.d_rows.as<RowOut>(), c.groups, stream);
     else
@@ -486,7 +492,7 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
     const int M = static_cast<int>(g.M);
     if (g.nc > 0) {
       launch_fused_rows_nc(g.X, g.d_rep.as<float2>(), g.d_tw.as<float2>(), n_ant, g.n_bin,
-                           g.n_prn, g.nc, g.cw, g.d_rows.as<RowOut>(), 8, st);
+                           g.n_prn, g.nc, g.cw, g.d_rows.as<RowOut>(), g.rows_cfg.groups, st);
     } else if (g.fp16) {
       launch_fused_rows16(g.d_X16.as<uint32_t>(), g.d_rep16.as<uint32_t>(), g.d_tw.as<float2>(),
                           n_ant, g.n_bin, g.n_prn, g.cw, g.d_rows.as<RowOut>(),
@@ -714,8 +720,10 @@ const char* tga_plan_kernel_name(const tga_plan* plan, int32_t kernel) {
   const bool fp16 = fused && p->groups[0].fp16;
   const bool big = p && !p->groups.empty() && p->groups[0].big;
   const bool big16 = big && p->groups[0].big16;
+  const bool nc = fused && p->groups[0].nc > 0;
   switch (kernel) {
     case 0: return fp16    ? "k_rows16"
+                   : nc    ? "k_rows<NC>"
                    : fused ? "k_rows"
                    : big16 ? "k_big_pre+k_rows16<R0>"
                    : big   ? "k_big_pre+k_rows<0,R0>"

@@ -180,7 +180,7 @@ def test_split_b1c_l1c_at_16368khz(gpu, system, prn):
     assert int(peaks[0][0, 0]["best_codephase"]) == (163680 - 99999) % 163680
 
 
-def check_nc_against_oracle(plan, peaks, packed, fs, L, fmt, systems, epochs):
+def check_nc_against_oracle(plan, peaks, packed, fs, L, fmt, systems, epochs, prn_idx=None):
     from oracle import correlate as ocor
     levels = synth.unpack_levels(packed, L, fmt)
     for si, (name, prns, fc) in enumerate(systems):
@@ -190,6 +190,8 @@ def check_nc_against_oracle(plan, peaks, packed, fs, L, fmt, systems, epochs):
             sig = (lv - lv.sum() / len(lv)).astype(np.float32)
             pp = ocor.phasepoints(fs, epochs * P)
             for j, prn in enumerate(plan.prns[si]):
+                if prn_idx is not None and j not in prn_idx:
+                    continue
                 code = ocodes.code_resampled(name, prn, float(P), P)
                 ref = ocor.noncoherent(sig, pp, code, fc, P, epochs, fs)
                 got = peaks[si][a, j]
@@ -222,6 +224,28 @@ def test_noncoherent_plan_matches_oracle(gpu, fmt):
     for a in range(2):
         assert int(peaks[0][a, 0]["best_codephase"]) == (16368 - 7000) % 16368
         assert int(peaks[1][a, 0]["best_codephase"]) == (16368 - 300) % 16368
+
+
+@pytest.mark.parametrize("groups", ["8", "16", "32"])
+def test_noncoherent_groups_match(gpu, monkeypatch, groups):
+    # every PRN grouping of the non-coherent rows kernel (the autotuner's candidates) computes
+    # the same rows: bit-identical peaks to the plan's own choice, and the oracle on two PRNs
+    L, E = 163680, 10
+    signals = [dict(system="gps", prn=9, cn0=42.0, doppler=-1000.0, delay=5000)]
+    fc = grid(IF, 1500.0, 500.0)
+    prns = list(range(1, 21))
+    packed = synth.synth_observation(1, L, FS, signals, fmt=1, seed=77)
+    monkeypatch.delenv("TGA_ROWS_GROUPS", raising=False)
+    base = tga.Plan(FS, 1, L, [tga.SystemSearch("gps", fc, prns, nc_epochs=E)]).search(packed)
+    monkeypatch.setenv("TGA_ROWS_GROUPS", groups)
+    plan = tga.Plan(FS, 1, L, [tga.SystemSearch("gps", fc, prns, nc_epochs=E)])
+    assert f"groups={groups}" in plan.describe()
+    peaks = plan.search(packed)
+    for f in ("best_peak", "second_peak", "best_codephase", "best_freq_idx"):
+        np.testing.assert_array_equal(peaks[0][f], base[0][f], err_msg=f)
+    check_nc_against_oracle(plan, peaks, packed, FS, L, 1, [("gps", prns, fc)], E,
+                            prn_idx=[8, 9])
+    assert int(peaks[0][0, 8]["best_codephase"]) == (16368 - 5000) % 16368
 
 
 def test_noncoherent_plan_errors(gpu):
