@@ -62,7 +62,10 @@ void launch_fused_rows_nc(const float2* X, const float2* rep, const float2* tw, 
 // FP16 form of the fused path: X and the replica spectra are stored as f16 (re, im) pairs of
 // the value times kF16Scale (so |conj(X) R| stays far inside the f16 range for 1- and 2-bit
 // input), the radix passes run as f16 GEMMs on the matrix cores with f32 accumulation, and the
-// row values are those of the transform times kF16Scale^2 (powers times kF16Scale^4).
+// row values are those of the transform times kF16Scale^2 (powers times kF16Scale^4).  X16 and
+// rep16 rows are stored in the prime-factor (Good-Thomas) order of 16368 = 33 x 16 x 31 (position
+// 496 n1 + 31 n2 + n3 holds natural index (496 n1 + 1023 n2 + 528 n3) mod 16368), for which the
+// three radix passes need no twiddle factors.
 constexpr float kF16Scale = 1.0f / 256.0f;
 // Sub-rows of long rows (P = 4 or 10 x 16368) after the first split stage reach about
 // sqrt(R0) |X| |R| ~ 1e6 for 1-bit input; this brings them near 1 in f16.
@@ -74,6 +77,7 @@ void launch_fused_forward16(const uint8_t* packed, size_t row_bytes, int fmt,
 void launch_fused_rows16(const uint32_t* X16, const uint32_t* rep16, const float2* tw,
                          int n_ant, int n_bin, int n_prn, int cw, RowOut* rows, int groups,
                          hipStream_t st);
+// n values = n / 16368 natural-order rows -> f16 pairs times scale in the prime-factor order
 void launch_to_f16(const float2* in, uint32_t* out, long long n, float scale, hipStream_t st);
 // Twiddle table of the fused kernels: W_P^i (i < 128) then W_P^{128 i} (i < 128).
 std::vector<float2> fused_twiddles();

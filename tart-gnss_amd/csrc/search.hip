@@ -189,6 +189,25 @@ __device__ __forceinline__ float2 tw_lds(const float2* stw, int e) {
   return cmul(stw[e & 127], stw[128 + (e >> 7)]);
 }
 
+// Prime-factor (Good-Thomas) order of the FP16 form's rows.  16368 = 33 x 16 x 31 with pairwise
+// coprime factors: with the input map n = (496 n1 + 1023 n2 + 528 n3) mod P and the output map
+// k = (496 k1 + 15345 k2 + 528 k3) mod P (the CRT number with k mod 33 = k1, k mod 16 = k2,
+// k mod 31 = k3), the P-point DFT is a plain 33 x 16 x 31 DFT with no twiddle factors between
+// the passes.  Storage position s = 496 n1 + 31 n2 + n3 holds the value of natural index
+// pfa_nat(s).
+__device__ __forceinline__ int pfa_nat(int s) {
+  const int n1 = s / 496, b = s - 496 * n1, n2 = b / 31, n3 = b - 31 * n2;
+  const int n = 496 * n1 + 1023 * n2 + 528 * n3;  // < 3 P
+  return n >= 2 * P ? n - 2 * P : n >= P ? n - P : n;
+}
+// natural index of output k3 of the 31-point pass for column m = k1 + 33 k2: base(m) + 528 k3
+// reduced mod P, base(m) = (496 k1 + 15345 k2) mod P
+__device__ __forceinline__ int pfa_base(int m) {
+  const int k2 = m / 33, k1 = m - 33 * k2;
+  return (496 * k1 + 15345 * k2) % P;
+}
+__device__ __forceinline__ int pfa_wrap(int k) { return k >= P ? k - P : k; }
+
 // In-LDS Stockham pass of radix R after NS points of sub-transforms (in place: every thread
 // holds its butterflies in registers across the barrier).
 template <int R, int NS>
@@ -339,15 +358,22 @@ __global__ __launch_bounds__(T) void k_forward(FwdArgs a) {
       for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], tw_lds(stw, r * j));
       Dft<16>::run(v);
       if constexpr (F16) {
-        uint32_t* out = a.X16 + static_cast<size_t>(xrow) * P;
+        // back into LDS in natural order: slots j + r NB4 are read and written by this thread
+        // only, so no barrier is needed before the writes
 #pragma unroll
-        for (int r = 0; r < 16; ++r) out[j + r * NB4] = pack_f16(v[r], kF16Scale);
+        for (int r = 0; r < 16; ++r) buf[j + r * NB4] = v[r];
       } else {
         float2* out = a.X + static_cast<size_t>(xrow) * P;
 #pragma unroll
         for (int r = 0; r < 16; ++r) out[j + r * NB4] = v[r];
       }
     }
+  }
+  if constexpr (F16) {
+    // the FP16 form stores X in the prime-factor order of the rows kernel, written coalesced
+    __syncthreads();
+    uint32_t* out = a.X16 + static_cast<size_t>(xrow) * P;
+    for (int s = tid; s < P; s += T) out[s] = pack_f16(buf[pfa_nat(s)], kF16Scale);
   }
 }
 
@@ -1130,13 +1156,17 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
             b[kb] = __builtin_bit_cast(halfx8, w);
           }
         }
-        // twiddle W_P^{q j} for q = 8t + 2 grp (+1); output q of butterfly j is pass-B
-        // input r = j / 31 of butterfly 33 (j mod 31) + q
-        const float2 s1 = tw_lds(stw, j);
-        const float2 s8 = tw_lds(stw, 8 * j);
-        float2 w0 = tw_lds(stw, 2 * grp * j);
+        // output q = 8t + 2 grp (+1) of butterfly j is pass-B input r = j / 31 of butterfly
+        // 33 (j mod 31) + q.  Prime-factor order (SUB == 0): j = 31 n2 + n3 and no twiddle;
+        // Stockham order (sub-rows): twiddle W_P^{q j}.
         const int jd = j / 31;
         uint32_t* dst = buf0 + 33 * (j - 31 * jd) + SB * jd + 2 * grp;
+        float2 s1, s8, w0;
+        if constexpr (SUB > 0) {
+          s1 = tw_lds(stw, j);
+          s8 = tw_lds(stw, 8 * j);
+          w0 = tw_lds(stw, 2 * grp * j);
+        }
 #pragma unroll
         for (int t = 0; t < 5; ++t) {
           floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
@@ -1145,10 +1175,15 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
             acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(
                 __builtin_bit_cast(halfx8, fa_lds[(t * 3 + kb) * 64 + lane]), b[kb], acc, 0, 0,
                 0);
-          if (t > 0) w0 = cmul(w0, s8);
-          const float2 w1 = cmul(w0, s1);
-          if (t < 4 || grp == 0) dst[8 * t] = pack_f16(cmulf(acc[0], acc[1], w0), 1.0f);
-          if (t < 4) dst[8 * t + 1] = pack_f16(cmulf(acc[2], acc[3], w1), 1.0f);
+          if constexpr (SUB == 0) {
+            if (t < 4 || grp == 0) dst[8 * t] = pack_f16(make_float2(acc[0], acc[1]), 1.0f);
+            if (t < 4) dst[8 * t + 1] = pack_f16(make_float2(acc[2], acc[3]), 1.0f);
+          } else {
+            if (t > 0) w0 = cmul(w0, s8);
+            const float2 w1 = cmul(w0, s1);
+            if (t < 4 || grp == 0) dst[8 * t] = pack_f16(cmulf(acc[0], acc[1], w0), 1.0f);
+            if (t < 4) dst[8 * t + 1] = pack_f16(cmulf(acc[2], acc[3], w1), 1.0f);
+          }
         }
       }
     }
@@ -1167,22 +1202,32 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
       floatx4 acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(fb[1], b, floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
       if (ok) {
         // output q of butterfly j (a = j / 33, k = j mod 33) is pass-C input r = a of
-        // butterfly k + 33 q
+        // butterfly k + 33 q; the Stockham order twiddles it by W_496^{a q}
         const int aa = j / 33;
         const int k = j - 33 * aa;
         uint32_t* dst = buf1 + aa * SC + k + 66 * grp;
-        const float2* tq = t16 + aa * 16 + 2 * grp;
-        dst[0] = pack_f16(cmulf(acc0[0], acc0[1], tq[0]), 1.0f);
-        dst[33] = pack_f16(cmulf(acc0[2], acc0[3], tq[1]), 1.0f);
-        dst[264] = pack_f16(cmulf(acc1[0], acc1[1], tq[8]), 1.0f);
-        dst[297] = pack_f16(cmulf(acc1[2], acc1[3], tq[9]), 1.0f);
+        if constexpr (SUB == 0) {
+          dst[0] = pack_f16(make_float2(acc0[0], acc0[1]), 1.0f);
+          dst[33] = pack_f16(make_float2(acc0[2], acc0[3]), 1.0f);
+          dst[264] = pack_f16(make_float2(acc1[0], acc1[1]), 1.0f);
+          dst[297] = pack_f16(make_float2(acc1[2], acc1[3]), 1.0f);
+        } else {
+          const float2* tq = t16 + aa * 16 + 2 * grp;
+          dst[0] = pack_f16(cmulf(acc0[0], acc0[1], tq[0]), 1.0f);
+          dst[33] = pack_f16(cmulf(acc0[2], acc0[3], tq[1]), 1.0f);
+          dst[264] = pack_f16(cmulf(acc1[0], acc1[1], tq[8]), 1.0f);
+          dst[297] = pack_f16(cmulf(acc1[2], acc1[3], tq[9]), 1.0f);
+        }
       }
     }
     lds_barrier();
-    // ---- pass C: radix 31, 33 tiles: tiles wave and wave + 16 on every wave, and the four
-    // output tiles t of tile 32 on waves 0-3 (wave t takes output tile t).  All products are
-    // issued before the first power is used, and the 16 powers of the two main tiles are
-    // merged as a tree, so neither the matrix-core latency nor the top-2 chain serialises.
+    // ---- pass C: radix 31, 33 tiles: tiles t0 and t0 + 8 (t0 = wave, or wave + 8 for waves
+    // 8-15) on every wave, and the four output tiles t of tile 32 on waves 0-3 (wave t takes
+    // output tile t).  All products are issued before the first power is used, and the 16
+    // powers of the two main tiles are merged as a tree, so neither the matrix-core latency nor
+    // the top-2 chain serialises.  Output q of butterfly j is sample j + 528 q (Stockham order)
+    // or pfa_base(j) + 528 q mod P (prime-factor order); either way the 16 outputs of a lane
+    // are pairwise at least 128 samples apart (ring distance), so each Top2 set may hold them.
     Top2 main_set, extra_set;
     float accn = 0.0f;  // sum of powers: NaN / overflow detector
     {
@@ -1198,7 +1243,13 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
           b[kb] = __builtin_bit_cast(halfx8, w);
         }
       };
-      const int j0 = 16 * wave + ln16, j1 = j0 + 256;
+      const int j0 = 16 * (wave < 8 ? wave : wave + 8) + ln16, j1 = j0 + 128;
+      // sample index of output 2 grp of butterflies j0, j1 (add 528 q' and wrap for q')
+      const int base0 = SUB == 0 ? pfa_base(j0) + NBC * 2 * grp : j0 + NBC * 2 * grp;
+      const int base1 = SUB == 0 ? pfa_base(j1) + NBC * 2 * grp : j1 + NBC * 2 * grp;
+      auto sample = [&](int base, int dq) {
+        return SUB == 0 ? pfa_wrap(base + NBC * dq) : base + NBC * dq;
+      };
       halfx8 b0[2], b1[2];
       load_c(j0, b0);
       load_c(j1, b1);
@@ -1216,17 +1267,16 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
       Top2 lv[16];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        const int q = 8 * t + 2 * grp;
         const float pa0 = c0[t][0] * c0[t][0] + c0[t][1] * c0[t][1];
         const float pb0 = c0[t][2] * c0[t][2] + c0[t][3] * c0[t][3];
         const float pa1 = c1[t][0] * c1[t][0] + c1[t][1] * c1[t][1];
         const float pb1 = c1[t][2] * c1[t][2] + c1[t][3] * c1[t][3];
-        const bool okb = t < 3 || grp < 3;  // output q + 1 = 31 does not exist
+        const bool okb = t < 3 || grp < 3;  // output 8 t + 2 grp + 1 = 31 does not exist
         accn += (pa0 + pa1) + (okb ? pb0 + pb1 : 0.0f);
-        lv[4 * t + 0] = Top2::leaf(pa0, j0 + NBC * q);
-        lv[4 * t + 1] = Top2::leaf(pa1, j1 + NBC * q);
-        lv[4 * t + 2] = okb ? Top2::leaf(pb0, j0 + NBC * (q + 1)) : Top2();
-        lv[4 * t + 3] = okb ? Top2::leaf(pb1, j1 + NBC * (q + 1)) : Top2();
+        lv[4 * t + 0] = Top2::leaf(pa0, sample(base0, 8 * t));
+        lv[4 * t + 1] = Top2::leaf(pa1, sample(base1, 8 * t));
+        lv[4 * t + 2] = okb ? Top2::leaf(pb0, sample(base0, 8 * t + 1)) : Top2();
+        lv[4 * t + 3] = okb ? Top2::leaf(pb1, sample(base1, 8 * t + 1)) : Top2();
       }
 #pragma unroll
       for (int w = 8; w >= 1; w >>= 1)
@@ -1248,13 +1298,14 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
         floatx4 acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(f0, b[0], floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(f1, b[1], acc, 0, 0, 0);
         const int q = 8 * wave + 2 * grp;
+        const int be = SUB == 0 ? pfa_base(j) + NBC * q : j + NBC * q;
         const float pa = acc[0] * acc[0] + acc[1] * acc[1];
         accn += pa;
-        extra_set.add(pa, j + NBC * q);
+        extra_set.add(pa, sample(be, 0));
         if (wave < 3 || grp < 3) {
           const float pb = acc[2] * acc[2] + acc[3] * acc[3];
           accn += pb;
-          extra_set.add(pb, j + NBC * (q + 1));
+          extra_set.add(pb, sample(be, 1));
         }
       }
     }
@@ -1278,11 +1329,14 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
   }
 }
 
-// f32 pairs -> f16 pairs scaled by s
+// rows of P f32 pairs (natural order) -> f16 pairs scaled by s in the prime-factor order
 __global__ void k_to_f16(const float2* __restrict__ in, uint32_t* __restrict__ out, long long n,
                          float s) {
   const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-  if (i < n) out[i] = pack_f16(in[i], s);
+  if (i >= n) return;
+  const long long row = i / P;
+  const int pos = static_cast<int>(i - row * P);
+  out[i] = pack_f16(in[row * P + pfa_nat(pos)], s);
 }
 
 }  // namespace fused
