@@ -273,11 +273,8 @@ void Plan::build(const tga_plan_desc& d) {
       g.d_X16.alloc(g.n_xrow * M * sizeof(uint32_t));
       upload(g.d_tw, fused_twiddles(), stream);
       autotune(g);
-    } else if (g.nc > 0) {
-      g.d_X.alloc(g.n_xrow * M * sizeof(float2));
-      upload(g.d_tw, fused_twiddles(), stream);
-      autotune(g);
-    } else if (g.fused) {
+    } else if (g.fused) {  // FP32 fused rows, reference or non-coherent form
+      rep_to_pfa(g, static_cast<long long>(n_prn) * 2 * static_cast<long long>(M));
       g.d_X.alloc(g.n_xrow * M * sizeof(float2));
       upload(g.d_tw, fused_twiddles(), stream);
       autotune(g);
@@ -344,6 +341,14 @@ void Plan::build(const tga_plan_desc& d) {
 // PRN groups (8, 16 or 32: more groups give more, shorter workgroups, each over fewer PRNs).
 // TGA_FUSED_VARIANT=0|1 pins the radix-31 implementation (tests use it to cover both) and
 // TGA_ROWS_GROUPS=n the grouping.
+// The fused FP32 rows kernels read the replica spectra in the prime-factor order of 16368.
+void Plan::rep_to_pfa(Group& g, long long n) {
+  DevBuf t(static_cast<size_t>(n) * sizeof(float2));
+  launch_to_pfa(g.d_rep.as<float2>(), t.as<float2>(), n, stream);
+  TGA_HIP(hipStreamSynchronize(stream));
+  g.d_rep = std::move(t);
+}
+
 void Plan::autotune(Group& g) {
   const char* pin_v = std::getenv("TGA_FUSED_VARIANT");
   const char* pin_g = std::getenv("TGA_ROWS_GROUPS");

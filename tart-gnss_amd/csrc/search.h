@@ -38,6 +38,11 @@ bool fused_supported(size_t P, size_t N, int cw);
 void launch_fused_forward(const uint8_t* packed, size_t row_bytes, int fmt, const double* means,
                           const double* fc, int n_ant, int n_bin, double phase_const,
                           const float2* tau, const float2* tw, float2* X, hipStream_t st);
+// The fused FP32 kernels store X and read the replica spectra in the prime-factor (Good-Thomas)
+// order of 16368 = 33 x 16 x 31 (position 496 n1 + 31 n2 + n3 holds natural index
+// (496 n1 + 1023 n2 + 528 n3) mod 16368), for which the three radix passes of a correlation row
+// need no twiddle factors.  launch_to_pfa reorders n values (rows of 16368) into it.
+void launch_to_pfa(const float2* in, float2* out, long long n, hipStream_t st);
 // Tunable shape of the rows kernel: `variant` 0 runs the radix-31 pass on the VALU, 1 on the
 // matrix cores; `groups` PRN groups per (antenna, bin, hypothesis) (a multiple of the 8 XCDs
 // keeps each group's replica spectra within one XCD's L2).

@@ -357,23 +357,20 @@ __global__ __launch_bounds__(T) void k_forward(FwdArgs a) {
 #pragma unroll
       for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], tw_lds(stw, r * j));
       Dft<16>::run(v);
-      if constexpr (F16) {
-        // back into LDS in natural order: slots j + r NB4 are read and written by this thread
-        // only, so no barrier is needed before the writes
+      // back into LDS in natural order: slots j + r NB4 are read and written by this thread
+      // only, so no barrier is needed before the writes
 #pragma unroll
-        for (int r = 0; r < 16; ++r) buf[j + r * NB4] = v[r];
-      } else {
-        float2* out = a.X + static_cast<size_t>(xrow) * P;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) out[j + r * NB4] = v[r];
-      }
+      for (int r = 0; r < 16; ++r) buf[j + r * NB4] = v[r];
     }
   }
+  // X is stored in the prime-factor order of the rows kernels, written coalesced
+  __syncthreads();
   if constexpr (F16) {
-    // the FP16 form stores X in the prime-factor order of the rows kernel, written coalesced
-    __syncthreads();
     uint32_t* out = a.X16 + static_cast<size_t>(xrow) * P;
     for (int s = tid; s < P; s += T) out[s] = pack_f16(buf[pfa_nat(s)], kF16Scale);
+  } else {
+    float2* out = a.X + static_cast<size_t>(xrow) * P;
+    for (int s = tid; s < P; s += T) out[s] = buf[pfa_nat(s)];
   }
 }
 
@@ -626,11 +623,16 @@ __device__ __forceinline__ void row_finish_key(const Top2& main_set, const Top2&
 // Correlation rows.  Per (antenna, bin, hypothesis) block and per PRN of its group:
 //   pass A  radix 33 (NS = 1), 496 butterflies, operands conj(X) . R gathered straight from
 //           global memory (buffer loads) into registers
-//   pass B  radix 16 (NS = 33) through LDS, twiddles W_528^{rk} from a [r][k] LDS table
-//           (lanes read consecutive entries: no bank conflicts)
-//   pass C  radix 31 (NS = 528); outputs stay in registers as |z|^2.  Two implementations:
-//     VARIANT 0 (VALU): one butterfly per thread by the symmetric codelet with its twiddles
-//           W_P^{r j} in registers; the 16 butterflies beyond one per thread are split into
+//   pass B  radix 16 (NS = 33) through LDS
+//   pass C  radix 31 (NS = 528); outputs stay in registers as |z|^2.
+// Full rows (SUB == 0) read X and R in the prime-factor order of 16368 = 33 x 16 x 31, for
+// which the passes need no twiddle factors and output r of pass-C butterfly j is sample
+// out_idx(j, r), the CRT number of (j mod 33, j / 33, r).  Sub-rows of long rows (SUB > 0)
+// arrive in natural order and take the Stockham twiddles: W_528^{rk} in pass B from a [r][k]
+// LDS table (lanes read consecutive entries: no bank conflicts) and W_P^{rj} in pass C.
+// Pass C has two implementations:
+//     VARIANT 0 (VALU): one butterfly per thread by the symmetric codelet (its twiddles in
+//           registers); the 16 butterflies beyond one per thread are split into
 //           256 (butterfly, output pair) units, one per thread of waves 0-3.
 //     VARIANT 1 (matrix cores): with a_j = v_j + v_{31-j}, b_j = v_j - v_{31-j} (a_0 = v_0),
 //           Y_k = A_k - i B_k and Y_{31-k} = A_k + i B_k where A = C a, B = S b for the 16x16
@@ -667,15 +669,19 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
   float2 twl[8];
   int rr[8];
   if constexpr (VARIANT == 0) {
-    for (int i = tid; i < 31 * LEFT; i += T) tl[i] = tw_lds(stw, (i / LEFT) * (T + i % LEFT));
+    if constexpr (SUB > 0)
+      for (int i = tid; i < 31 * LEFT; i += T) tl[i] = tw_lds(stw, (i / LEFT) * (T + i % LEFT));
     if (tid < 31) {
       const float2 w = tw_lds(stw, tid * NBC);  // exp(-2 pi i m / 31)
       r31[tid] = make_float2(w.x, -w.y);
     }
+    if constexpr (SUB > 0) {
 #pragma unroll
-    for (int r = 1; r < 31; ++r) wc[r] = tw_lds(stw, r * tid);
+      for (int r = 1; r < 31; ++r) wc[r] = tw_lds(stw, r * tid);
+    }
   } else {
-    for (int i = tid; i < 1023; i += T) t1023[i] = tw_lds(stw, 16 * i);
+    if constexpr (SUB > 0)
+      for (int i = tid; i < 1023; i += T) t1023[i] = tw_lds(stw, 16 * i);
 #pragma unroll
     for (int st = 0; st < 4; ++st) {
       const int j = 4 * st + grp;
@@ -684,10 +690,21 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
       sA[st] = j == 0 ? 0.0f : -w.y;
       rr[2 * st] = j;
       rr[2 * st + 1] = j == 0 ? 0 : 31 - j;
-      twl[2 * st] = tw_lds(stw, j * ln16);
-      twl[2 * st + 1] = j == 0 ? make_float2(0.f, 0.f) : tw_lds(stw, (31 - j) * ln16);
+      if constexpr (SUB > 0) {
+        twl[2 * st] = tw_lds(stw, j * ln16);
+        twl[2 * st + 1] = j == 0 ? make_float2(0.f, 0.f) : tw_lds(stw, (31 - j) * ln16);
+      }
     }
   }
+  // Sample index of output r of pass-C butterfly j: Stockham order (sub-rows, natural-order
+  // input) j + 528 r; prime-factor order (X and R stored in it) the CRT number of
+  // (j mod 33, j / 33, r), and no twiddle factor between the passes.
+  auto out_idx = [](int j, int r) {
+    if constexpr (SUB == 0)
+      return pfa_wrap(pfa_base(j) + NBC * r);
+    else
+      return j + NBC * r;
+  };
   // One transform of a correlation row (or sub-row): `operand(ta, r)` gives pass A's input r of
   // butterfly ta; every output power goes to `sink(slot, p, idx)` with slot < 31 for the
   // thread's main set (outputs more than 32 samples apart) and slot 31 / 32 for its extra set.
@@ -724,8 +741,10 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
         if (b == 0 || j < NBB) {
           const int q33 = j / 33;
           const int k = j - q33 * 33;
+          if constexpr (SUB > 0) {
 #pragma unroll
-          for (int r = 1; r < 16; ++r) v[b][r] = cmul(v[b][r], tb[r * 33 + k]);
+            for (int r = 1; r < 16; ++r) v[b][r] = cmul(v[b][r], tb[r * 33 + k]);
+          }
           Dft<16>::run(v[b]);
           const int base = q33 * NBC + k;
 #pragma unroll
@@ -748,8 +767,10 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
         float2 w[31];
 #pragma unroll
         for (int r = 0; r < 31; ++r) w[r] = buf[q + r * NBC];
+        if constexpr (SUB > 0) {
 #pragma unroll
-        for (int r = 1; r < 31; ++r) w[r] = cmul(w[r], tl[r * LEFT + b]);
+          for (int r = 1; r < 31; ++r) w[r] = cmul(w[r], tl[r * LEFT + b]);
+        }
         float ar = w[0].x, ai = w[0].y, br = 0.f, bi = 0.f;
         int m = 0;
 #pragma unroll
@@ -767,21 +788,23 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
         // kp = 0: every root is 1 and the sine sums vanish, so Y_0 = ar + i ai
         const float y1r = ar + bi, y1i = ai - br;  // Y_kp
         const float p1 = y1r * y1r + y1i * y1i;
-        sink(31, p1, q + kp * NBC);
+        sink(31, p1, out_idx(q, kp));
         if (kp > 0) {
           const float y2r = ar - bi, y2i = ai + br;  // Y_{31-kp}
           const float p2 = y2r * y2r + y2i * y2i;
-          // the two outputs of a unit lie (31 - 2 kp) 528 > 32 samples apart
-          sink(32, p2, q + (31 - kp) * NBC);
+          // the two outputs of a unit lie a nonzero multiple of 528 samples apart
+          sink(32, p2, out_idx(q, 31 - kp));
         }
       }
+      if constexpr (SUB > 0) {
 #pragma unroll
-      for (int r = 1; r < 31; ++r) v[r] = cmul(v[r], wc[r]);
+        for (int r = 1; r < 31; ++r) v[r] = cmul(v[r], wc[r]);
+      }
       Dft<31>::run(v);
 #pragma unroll
       for (int r = 0; r < 31; ++r) {
         const float pv = v[r].x * v[r].x + v[r].y * v[r].y;
-        sink(r, pv, tc + r * NBC);  // outputs of one butterfly lie 528 apart
+        sink(r, pv, out_idx(tc, r));  // outputs of one butterfly lie multiples of 528 apart
       }
     } else {
       int gc = grp;
@@ -810,10 +833,14 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
               }
               int m2 = c31 - m1;
               if (m2 < 0) m2 += 1023;
-              const float2 w1 = cmul(twl[2 * st], t1023[m1]);
-              const float2 w2 = cmul(twl[2 * st + 1], t1023[m2]);
-              const float2 v1 = cmul(buf[q + r1 * NBC], w1);
-              const float2 v2 = cmul(buf[q + r2 * NBC], w2);
+              float2 v1 = buf[q + r1 * NBC];
+              float2 v2 = buf[q + r2 * NBC];
+              if constexpr (SUB > 0) {
+                v1 = cmul(v1, cmul(twl[2 * st], t1023[m1]));
+                v2 = cmul(v2, cmul(twl[2 * st + 1], t1023[m2]));  // 0 for r1 = 0
+              } else if (r1 == 0) {
+                v2 = make_float2(0.f, 0.f);  // a_0 = v_0 (no partner input)
+              }
               fr[u][st][0] = v1.x + v2.x;
               fr[u][st][1] = v1.y + v2.y;
               fr[u][st][2] = v1.x - v2.x;
@@ -843,7 +870,10 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
           const int ts = b0 + u;
           if (ts < ntile) {
             const int q = 16 * (wave + 8 * ts) + ln16;
-            const int slot = ts < 4 ? 0 : 31;
+            // butterflies q0 + 128 ts: in the prime-factor order the outputs of butterflies 256
+            // apart can lie 8 samples apart, so tiles 0, 1 (and 4) and tiles 2, 3 keep separate
+            // top-2 sets (within each, outputs lie at least 49 samples apart)
+            const int slot = (ts == 2 || ts == 3) ? 31 : 0;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
               const int kk = 4 * gc + e;  // output row of the GEMM tile
@@ -851,11 +881,11 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
               const float br = acc[u][2][e], bi = acc[u][3][e];
               const float y1r = ar + bi, y1i = ai - br;  // Y_kk
               const float p1 = y1r * y1r + y1i * y1i;
-              sink(slot, p1, q + kk * NBC);
+              sink(slot, p1, out_idx(q, kk));
               if (kk > 0) {
                 const float y2r = ar - bi, y2i = ai + br;  // Y_{31-kk}
                 const float p2 = y2r * y2r + y2i * y2i;
-                sink(slot, p2, q + (31 - kk) * NBC);
+                sink(slot, p2, out_idx(q, 31 - kk));
               }
             }
           }
@@ -896,15 +926,15 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
 #pragma unroll
       for (int r = 0; r < 31; ++r) {
         accn += acc[r];
-        ms.add(acc[r], tid + r * NBC);
+        ms.add(acc[r], out_idx(tid, r));
       }
       if (tid < 16 * LEFT) {
         const int b = tid & (LEFT - 1), kp = tid >> 4, q = T + b;
         accn += acc[31];
-        es.add(acc[31], q + kp * NBC);
+        es.add(acc[31], out_idx(q, kp));
         if (kp > 0) {
           accn += acc[32];
-          es.add(acc[32], q + (31 - kp) * NBC);
+          es.add(acc[32], out_idx(q, 31 - kp));
         }
       }
       RowOut* out = a.rows + ((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) * 2;
@@ -1329,6 +1359,15 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
   }
 }
 
+// rows of P f32 pairs (natural order) -> the same in the prime-factor order
+__global__ void k_to_pfa(const float2* __restrict__ in, float2* __restrict__ out, long long n) {
+  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  const long long row = i / P;
+  const int pos = static_cast<int>(i - row * P);
+  out[i] = in[row * P + pfa_nat(pos)];
+}
+
 // rows of P f32 pairs (natural order) -> f16 pairs scaled by s in the prime-factor order
 __global__ void k_to_f16(const float2* __restrict__ in, uint32_t* __restrict__ out, long long n,
                          float s) {
@@ -1720,6 +1759,11 @@ void launch_fused_rows16(const uint32_t* X16, const uint32_t* rep16, const float
   const long long blocks = persistent_blocks(a.n_items, groups);
   hipLaunchKernelGGL(fused::k_rows16<0>, dim3(static_cast<unsigned>(blocks)),
                      dim3(fused::T16), 0, st, a);
+  TGA_HIP(hipGetLastError());
+}
+
+void launch_to_pfa(const float2* in, float2* out, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(fused::k_to_pfa, dim3(grid_for(n, 256)), dim3(256), 0, st, in, out, n);
   TGA_HIP(hipGetLastError());
 }
 
