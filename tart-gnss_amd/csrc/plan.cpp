@@ -517,11 +517,11 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
         const int b = c & 1;
         if (c >= 2) TGA_HIP(hipStreamWaitEvent(side, ev_rows[b], 0));
         launch_big_pre(g.X, g.d_rep.as<float2>(), g.d_twb.as<float2>(), g.big_r0, g.n_prn,
-                       g.n_bin, hs, r0, r1, g.d_Y[b].p, g.big16,
+                       g.n_bin, n_ant, hs, r0, r1, g.d_Y[b].p, g.big16,
                        g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(), side);
         TGA_HIP(hipEventRecord(ev_pre[b], side));
         TGA_HIP(hipStreamWaitEvent(st, ev_pre[b], 0));
-        launch_big_sub(g.d_tw.as<float2>(), g.big_r0, g.n_prn, g.n_bin, hs, r0, r1,
+        launch_big_sub(g.d_tw.as<float2>(), g.big_r0, g.n_prn, g.n_bin, n_ant, hs, r0, r1,
                        g.d_Y[b].p, g.big16, g.d_rowkey.as<unsigned long long>(),
                        g.d_rownan.as<int>(), g.d_rows.as<RowOut>(), st);
         TGA_HIP(hipEventRecord(ev_rows[b], st));

@@ -127,8 +127,8 @@ void launch_select(const RowOut* rows, const PeakSlot* slots, int n_ant, int n_p
 bool big_supported(size_t P, size_t N, bool folded);
 // W_P^i (i < 512) then W_P^{512 i} (i < ceil(P / 512)).
 std::vector<float2> big_twiddles(size_t P);
-// Rows [r0, r1) of the xrow-major order I = xrow * n_prn + prn (rows sharing a forward spectrum
-// are consecutive); results land at the usual row index.  launch_big_pre writes the twiddled
+// Rows [r0, r1) of the tiled enumeration (tiles of 8 forward spectra x 8 replica spectra, whose
+// rows reuse them from the caches); results land at the usual row index.  launch_big_pre writes the twiddled
 // sub-rows of the rows to Y ((r1 - r0) * P complex values) and clears their keys;
 // launch_big_sub transforms them and writes the rows' RowOut.
 // hs = 1: every (bin, h) row; hs = 2: unfolded rows, h = 0 only (the [c,-c] hypothesis equals
@@ -137,11 +137,11 @@ std::vector<float2> big_twiddles(size_t P);
 // run through the matrix-core rows kernel, whose powers then carry kSplitF16Scale^2; otherwise
 // Y holds float2 and the f32 rows kernel transforms them.
 void launch_big_pre(const float2* X, const float2* rep, const float2* twb, int R0, int n_prn,
-                    int n_bin, int hs, long long r0, long long r1, void* Y, bool f16,
+                    int n_bin, int n_ant, int hs, long long r0, long long r1, void* Y, bool f16,
                     unsigned long long* rowkey, int* rownan, hipStream_t st);
-void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int hs, long long r0,
-                    long long r1, const void* Y, bool f16, unsigned long long* rowkey,
-                    int* rownan, RowOut* rows, hipStream_t st);
+void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, int hs,
+                    long long r0, long long r1, const void* Y, bool f16,
+                    unsigned long long* rowkey, int* rownan, RowOut* rows, hipStream_t st);
 // second peaks of the winning rows; Z, Zs hold n_ant * n_prn * M complex values
 void launch_big_second(const float2* X, const float2* rep, const PeakSlot* slots, int M, int P,
                        int cw, int n_ant, int n_prn, int n_bin, bool folded, const int* win,

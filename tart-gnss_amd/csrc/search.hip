@@ -122,16 +122,41 @@ __device__ __forceinline__ float block_max(float v, float* sp) {
   return v;
 }
 
-// Long rows are processed in xrow-major order, I = xrow * n_prn + prn with xrow =
-// (ant * n_bin + bin) * 2 + h, so consecutive rows share their forward spectrum in the caches;
-// this maps I to the row index ((ant * n_prn + prn) * n_bin + bin) * 2 + h.
-// hs = 2 enumerates h = 0 only (unfolded rows, whose two hypotheses coincide).
-__device__ __forceinline__ long long big_row(long long I, int n_prn, int n_bin, int hs = 1) {
-  const long long xrow = (I / n_prn) * hs;
-  const int prn = static_cast<int>(I - (I / n_prn) * n_prn);
-  const long long ant = xrow / (2 * n_bin);
-  const int bh = static_cast<int>(xrow - ant * 2 * n_bin);
-  return ((ant * n_prn + prn) * n_bin) * 2 + bh;
+// Long rows are enumerated in tiles of kTileX xrows x kTileP PRNs, xrow fastest inside a
+// tile, with xrow = (ant * n_bin + bin) * 2 + h; hs = 2 enumerates h = 0 only (unfolded rows,
+// whose two hypotheses coincide).  The rows of a tile share kTileX forward spectra and kTileP
+// replica spectra, which stay in the caches while the tile passes, so the pre-pass does not
+// fetch one replica spectrum from memory per row.  Tiles at the end of the xrow or PRN range
+// are narrower.
+constexpr int kTileX = 8, kTileP = 8;
+struct BigItem {
+  long long xrow;
+  int prn;
+};
+__device__ __forceinline__ BigItem big_item(long long I, int n_prn, int n_bin, int n_ant,
+                                            int hs) {
+  const long long n_xe = static_cast<long long>(n_ant) * n_bin * 2 / hs;  // enumerated xrows
+  const long long blk = static_cast<long long>(kTileX) * n_prn;           // rows per xrow block
+  const long long xb = I / blk;
+  long long rem = I - xb * blk;
+  const long long left = n_xe - xb * kTileX;
+  const int a = static_cast<int>(left < kTileX ? left : kTileX);  // xrows in this block
+  const long long tile = static_cast<long long>(a) * kTileP;
+  const int pt = static_cast<int>(rem / tile);
+  rem -= pt * tile;
+  const int q = static_cast<int>(rem / a);
+  BigItem it;
+  it.xrow = (xb * kTileX + (rem - static_cast<long long>(q) * a)) * hs;
+  it.prn = pt * kTileP + q;
+  return it;
+}
+// the row index ((ant * n_prn + prn) * n_bin + bin) * 2 + h of enumerated row I
+__device__ __forceinline__ long long big_row(long long I, int n_prn, int n_bin, int n_ant,
+                                             int hs) {
+  const BigItem it = big_item(I, n_prn, n_bin, n_ant, hs);
+  const long long ant = it.xrow / (2 * n_bin);
+  const int bh = static_cast<int>(it.xrow - ant * 2 * n_bin);
+  return ((ant * n_prn + it.prn) * n_bin) * 2 + bh;
 }
 
 __device__ __forceinline__ int ring_dist(int a, int b, int P) {
@@ -392,6 +417,7 @@ struct RowArgs {
   int* rownan;
   int epochs;  // non-coherent form: one-period epochs per (antenna, bin)
   int hs;      // sub-row mode: 2 when only h = 0 rows are enumerated (unfolded rows)
+  int n_ant;   // sub-row mode: antennas (for the tiled row order)
 };
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -986,7 +1012,7 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
                   accn += p;
                   (slot < 31 ? ms : es).add(p, idx);
                 });
-      const long long r = big_row(a.r0 + rq, a.n_prn, a.n_bin, a.hs);
+      const long long r = big_row(a.r0 + rq, a.n_prn, a.n_bin, a.n_ant, a.hs);
       row_finish_key<T>(ms, es, accn != accn, red, k0, SUB, a.rowkey + r, a.rownan + r);
     }
   }
@@ -1027,6 +1053,7 @@ struct Row16Args {
   unsigned long long* rowkey = nullptr;
   int* rownan = nullptr;
   int hs = 1;
+  int n_ant = 0;  // sub-row mode: antennas (for the tiled row order)
   long long n_items = 0;  // SUB == 0: (xrow, PRN group) items, n_xrow * groups
 };
 
@@ -1347,7 +1374,7 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
       // sub-row it of the chunk: row r0 + it / SUB, output samples k0 + SUB m (k0 = it mod SUB)
       const long long rq = it / SUB;
       const int k0 = static_cast<int>(it - rq * SUB);
-      const long long r = big_row(a.r0 + rq, a.n_prn, a.n_bin, a.hs);
+      const long long r = big_row(a.r0 + rq, a.n_prn, a.n_bin, a.n_ant, a.hs);
       row_finish_key<T16>(main_set, extra_set, !(accn <= 3.0e38f), red, k0, SUB, a.rowkey + r,
                           a.rownan + r);
     }
@@ -1496,7 +1523,8 @@ template <int R0, bool F16>
 __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
                                                  const float2* __restrict__ rep,
                                                  const float2* __restrict__ twb, int n_prn,
-                                                 int n_bin, int hs, long long r0, long long total,
+                                                 int n_bin, int n_ant, int hs, long long r0,
+                                                 long long total,
                                                  void* __restrict__ Yv,
                                                  unsigned long long* __restrict__ rowkey,
                                                  int* __restrict__ rownan) {
@@ -1506,9 +1534,10 @@ __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
   if (gid >= total) return;
   const long long rr = gid / P1;
   const int n1 = static_cast<int>(gid - rr * P1);
-  const long long I = r0 + rr;  // xrow-major order (big_row)
-  const long long xrow = (I / n_prn) * hs;
-  const int prn = static_cast<int>(I - (I / n_prn) * n_prn);
+  const long long I = r0 + rr;  // tiled order (big_item)
+  const BigItem it = big_item(I, n_prn, n_bin, n_ant, hs);
+  const long long xrow = it.xrow;
+  const int prn = it.prn;
   const int h = static_cast<int>(xrow & 1);
   const float2* x = X + (hs == 1 ? xrow : xrow >> 1) * P;  // unfolded: one spectrum per bin
   const float2* c = rep + (static_cast<long long>(prn) * 2 + h) * P;
@@ -1521,18 +1550,25 @@ __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
     const int e = n1 * k0;  // < P: W_P^e from two 512-entry levels
     v[k0] = cmul(v[k0], cmul(twb[e & 511], twb[512 + (e >> 9)]));
   }
+  // non-temporal stores: the sub-rows are read once, by the next kernel, and should not
+  // displace the tile's spectra from L2
   if constexpr (F16) {
     uint32_t* y = static_cast<uint32_t*>(Yv) + rr * P;
 #pragma unroll
     for (int k0 = 0; k0 < R0; ++k0)
-      y[static_cast<long long>(k0) * P1 + n1] = fused::pack_f16(v[k0], kSplitF16Scale);
+      __builtin_nontemporal_store(fused::pack_f16(v[k0], kSplitF16Scale),
+                                  y + static_cast<long long>(k0) * P1 + n1);
   } else {
     float2* y = static_cast<float2*>(Yv) + rr * P;
 #pragma unroll
-    for (int k0 = 0; k0 < R0; ++k0) y[static_cast<long long>(k0) * P1 + n1] = v[k0];
+    for (int k0 = 0; k0 < R0; ++k0) {
+      float* d = reinterpret_cast<float*>(y + static_cast<long long>(k0) * P1 + n1);
+      __builtin_nontemporal_store(v[k0].x, d);
+      __builtin_nontemporal_store(v[k0].y, d + 1);
+    }
   }
   if (n1 == 0) {
-    const long long r = big_row(I, n_prn, n_bin, hs);
+    const long long r = big_row(I, n_prn, n_bin, n_ant, hs);
     rowkey[r] = 0ull;
     rownan[r] = 0;
   }
@@ -1541,10 +1577,11 @@ __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
 // Row keys of long rows -> RowOut (p2 is filled later from the winning row only).
 __global__ void k_big_rowout(const unsigned long long* __restrict__ rowkey,
                              const int* __restrict__ rownan, long long r0, long long n_rows,
-                             int n_prn, int n_bin, int hs, RowOut* __restrict__ rows) {
+                             int n_prn, int n_bin, int n_ant, int hs,
+                             RowOut* __restrict__ rows) {
   const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
   if (i >= n_rows) return;
-  const long long r = big_row(r0 + i, n_prn, n_bin, hs);
+  const long long r = big_row(r0 + i, n_prn, n_bin, n_ant, hs);
   const unsigned long long k = rowkey[r];
   RowOut o;
   o.p1 = __uint_as_float(static_cast<unsigned int>(k >> 32));
@@ -1848,29 +1885,29 @@ std::vector<float2> big_twiddles(size_t P) {
 }
 
 void launch_big_pre(const float2* X, const float2* rep, const float2* twb, int R0, int n_prn,
-                    int n_bin, int hs, long long r0, long long r1, void* Y, bool f16,
+                    int n_bin, int n_ant, int hs, long long r0, long long r1, void* Y, bool f16,
                     unsigned long long* rowkey, int* rownan, hipStream_t st) {
   const long long nr = r1 - r0;
   const long long total = nr * kFusedP;
   const dim3 grid(grid_for(total, 256));
   if (R0 == 4 && !f16)
     hipLaunchKernelGGL((k_big_pre<4, false>), grid, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
-                       hs, r0, total, Y, rowkey, rownan);
+                       n_ant, hs, r0, total, Y, rowkey, rownan);
   else if (R0 == 4)
     hipLaunchKernelGGL((k_big_pre<4, true>), grid, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
-                       hs, r0, total, Y, rowkey, rownan);
+                       n_ant, hs, r0, total, Y, rowkey, rownan);
   else if (R0 == 10 && !f16)
     hipLaunchKernelGGL((k_big_pre<10, false>), grid, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
-                       hs, r0, total, Y, rowkey, rownan);
+                       n_ant, hs, r0, total, Y, rowkey, rownan);
   else if (R0 == 10)
     hipLaunchKernelGGL((k_big_pre<10, true>), grid, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
-                       hs, r0, total, Y, rowkey, rownan);
+                       n_ant, hs, r0, total, Y, rowkey, rownan);
   else
     fail(TGA_ERR_UNSUPPORTED, "long-period split supports 4 and 10 sub-rows");
   TGA_HIP(hipGetLastError());
 }
 
-void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int hs, long long r0,
+void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, int hs, long long r0,
                     long long r1, const void* Y, bool f16, unsigned long long* rowkey,
                     int* rownan, RowOut* rows, hipStream_t st) {
   const long long nr = r1 - r0;
@@ -1889,6 +1926,7 @@ void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int hs, long
     a.rowkey = rowkey;
     a.rownan = rownan;
     a.hs = hs;
+    a.n_ant = n_ant;
     if (R0 == 4)
       hipLaunchKernelGGL((fused::k_rows16<4>), dim3(blocks), dim3(fused::T16), 0, st, a);
     else
@@ -1896,6 +1934,7 @@ void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int hs, long
   } else {
     fused::RowArgs a{};
     a.hs = hs;
+    a.n_ant = n_ant;
     a.tw = tw;
     a.cw = 0;
     a.n_prn = n_prn;
@@ -1912,7 +1951,7 @@ void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int hs, long
   }
   TGA_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_big_rowout, dim3(grid_for(nr, 256)), dim3(256), 0, st, rowkey, rownan,
-                     r0, nr, n_prn, n_bin, hs, rows);
+                     r0, nr, n_prn, n_bin, n_ant, hs, rows);
   TGA_HIP(hipGetLastError());
 }
 
