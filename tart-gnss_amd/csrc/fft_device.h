@@ -190,15 +190,49 @@ struct DftOdd {
   }
 };
 
+// R = R1 * R2 with gcd(R1, R2) = 1 (Good-Thomas): input n = (R2 n1 + R1 n2) mod R and output
+// k = (E1 k1 + E2 k2) mod R with k mod R1 = k1, k mod R2 = k2 make the R-point DFT an R1 x R2
+// DFT with no twiddle factors between the stages; both index maps are register renamings.
+constexpr int inv_mod(int a, int m) {
+  for (int x = 1; x < m; ++x)
+    if ((a * x) % m == 1) return x;
+  return 0;
+}
+template <int R1, int R2>
+struct DftPfa {
+  static constexpr int R = R1 * R2;
+  static constexpr int E1 = R2 * inv_mod(R2 % R1, R1);
+  static constexpr int E2 = R1 * inv_mod(R1 % R2, R2);
+  static_assert(E1 % R1 == 1 && E1 % R2 == 0 && E2 % R2 == 1 && E2 % R1 == 0, "coprime radices");
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 u[R1][R2];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) {
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) u[n1][n2] = v[(R2 * n1 + R1 * n2) % R];
+      Dft<R2>::run(u[n1]);
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) {
+      float2 w[R1];
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) w[n1] = u[n1][k2];
+      Dft<R1>::run(w);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) v[(E1 * k1 + E2 * k2) % R] = w[k1];
+    }
+  }
+};
+
 template <> struct Dft<3> { __device__ __forceinline__ static void run(float2* v) { DftOdd<3>::run(v); } };
 template <> struct Dft<5> { __device__ __forceinline__ static void run(float2* v) { DftOdd<5>::run(v); } };
 template <> struct Dft<7> { __device__ __forceinline__ static void run(float2* v) { DftOdd<7>::run(v); } };
 template <> struct Dft<11> { __device__ __forceinline__ static void run(float2* v) { DftOdd<11>::run(v); } };
 template <> struct Dft<13> { __device__ __forceinline__ static void run(float2* v) { DftOdd<13>::run(v); } };
 template <> struct Dft<31> { __device__ __forceinline__ static void run(float2* v) { DftOdd<31>::run(v); } };
-// 33 = 3 x 11 in registers
-template <> struct Dft<33> { __device__ __forceinline__ static void run(float2* v) { DftSplit<3, 11>::run(v); } };
-template <> struct Dft<10> { __device__ __forceinline__ static void run(float2* v) { DftSplit<2, 5>::run(v); } };
+// 33 = 3 x 11 and 10 = 2 x 5 in registers, prime-factor form
+template <> struct Dft<33> { __device__ __forceinline__ static void run(float2* v) { DftPfa<3, 11>::run(v); } };
+template <> struct Dft<10> { __device__ __forceinline__ static void run(float2* v) { DftPfa<2, 5>::run(v); } };
 
 // Twiddle W_n^e from the two-level table: tw1[e % split] * tw2[e / split].
 __device__ __forceinline__ float2 twiddle2(const float2* tw1, const float2* tw2, int split,
