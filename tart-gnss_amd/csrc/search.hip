@@ -123,8 +123,8 @@ __device__ __forceinline__ float block_max(float v, float* sp) {
 }
 
 // Long rows are enumerated in tiles of kTileX xrows x kTileP PRNs, xrow fastest inside a
-// tile, with xrow = (ant * n_bin + bin) * 2 + h; hs = 2 enumerates h = 0 only (unfolded rows,
-// whose two hypotheses coincide).  The rows of a tile share kTileX forward spectra and kTileP
+// tile, with xrow = (ant * n_bin + bin) * 2 + h taken h = 0 first, then h = 1; hs = 2
+// enumerates h = 0 only (unfolded rows, whose two hypotheses coincide).  The rows of a tile share kTileX forward spectra and kTileP
 // replica spectra, which stay in the caches while the tile passes, so the pre-pass does not
 // fetch one replica spectrum from memory per row.  Tiles at the end of the xrow or PRN range
 // are narrower.
@@ -145,8 +145,12 @@ __device__ __forceinline__ BigItem big_item(long long I, int n_prn, int n_bin, i
   const int pt = static_cast<int>(rem / tile);
   rem -= pt * tile;
   const int q = static_cast<int>(rem / a);
+  // enumerated xrow xe -> xrow with h outermost (hs = 1), so a tile's xrows share h and with it
+  // the replica spectra (prn, h) of its rows
+  const long long xe = xb * kTileX + (rem - static_cast<long long>(q) * a);
+  const long long n_ab = static_cast<long long>(n_ant) * n_bin;
   BigItem it;
-  it.xrow = (xb * kTileX + (rem - static_cast<long long>(q) * a)) * hs;
+  it.xrow = hs == 1 ? (xe >= n_ab ? (xe - n_ab) * 2 + 1 : xe * 2) : xe * 2;
   it.prn = pt * kTileP + q;
   return it;
 }
