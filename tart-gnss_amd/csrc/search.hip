@@ -984,16 +984,20 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
     const int p0 = g * a.per_group;
     const int p1 = min(p0 + a.per_group, a.n_prn);
     const __amdgpu_buffer_rsrc_t xv = buffer_view(a.X + static_cast<size_t>(xrow) * P, P * 8);
+    // the block's forward spectrum is the same for every PRN: each thread keeps its 33 pass-A
+    // inputs in registers (66 VGPRs) instead of re-reading them per PRN through L2, where the
+    // 32 rows of an XCD do not fit beside the replicas
+    float2 xr[33];
+#pragma unroll
+    for (int r = 0; r < 33; ++r)
+      xr[r] = tid < NBA ? buffer_load_f2(xv, tid * 8, r * NBA * 8) : make_float2(0.f, 0.f);
     for (int prn = p0; prn < p1; ++prn) {
       const __amdgpu_buffer_rsrc_t rv =
           buffer_view(a.rep + (static_cast<size_t>(prn) * 2 + h) * P, P * 8);
       Top2 ms, es;
       float accn = 0.0f;  // sum of powers: NaN-propagating detector
       transform(
-          [&](int ta, int r) {
-            return cconjmul(buffer_load_f2(xv, ta * 8, r * NBA * 8),
-                            buffer_load_f2(rv, ta * 8, r * NBA * 8));
-          },
+          [&](int ta, int r) { return cconjmul(xr[r], buffer_load_f2(rv, ta * 8, r * NBA * 8)); },
           [&](int slot, float p, int idx) {
             accn += p;
             (slot < 31 ? ms : es).add(p, idx);
