@@ -984,20 +984,31 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
     const int p0 = g * a.per_group;
     const int p1 = min(p0 + a.per_group, a.n_prn);
     const __amdgpu_buffer_rsrc_t xv = buffer_view(a.X + static_cast<size_t>(xrow) * P, P * 8);
-    // the block's forward spectrum is the same for every PRN: each thread keeps its 33 pass-A
-    // inputs in registers (66 VGPRs) instead of re-reading them per PRN through L2, where the
-    // 32 rows of an XCD do not fit beside the replicas
-    float2 xr[33];
+    // The block's forward spectrum is the same for every PRN.  With the matrix-core radix-31
+    // (few VGPRs elsewhere) each thread keeps its 33 pass-A inputs in registers (66 VGPRs)
+    // instead of re-reading them per PRN through L2, where the 32 rows of an XCD do not fit
+    // beside the replicas; the VALU codelet needs those registers itself (255 VGPRs and 20%
+    // slower with them) and re-reads.
+    constexpr bool kKeepX = VARIANT == 1;
+    float2 xr[kKeepX ? 33 : 1];
+    if constexpr (kKeepX) {
 #pragma unroll
-    for (int r = 0; r < 33; ++r)
-      xr[r] = tid < NBA ? buffer_load_f2(xv, tid * 8, r * NBA * 8) : make_float2(0.f, 0.f);
+      for (int r = 0; r < 33; ++r)
+        xr[r] = tid < NBA ? buffer_load_f2(xv, tid * 8, r * NBA * 8) : make_float2(0.f, 0.f);
+    }
     for (int prn = p0; prn < p1; ++prn) {
       const __amdgpu_buffer_rsrc_t rv =
           buffer_view(a.rep + (static_cast<size_t>(prn) * 2 + h) * P, P * 8);
       Top2 ms, es;
       float accn = 0.0f;  // sum of powers: NaN-propagating detector
       transform(
-          [&](int ta, int r) { return cconjmul(xr[r], buffer_load_f2(rv, ta * 8, r * NBA * 8)); },
+          [&](int ta, int r) {
+            const float2 rr = buffer_load_f2(rv, ta * 8, r * NBA * 8);
+            if constexpr (kKeepX)
+              return cconjmul(xr[r], rr);
+            else
+              return cconjmul(buffer_load_f2(xv, ta * 8, r * NBA * 8), rr);
+          },
           [&](int slot, float p, int idx) {
             accn += p;
             (slot < 31 ? ms : es).add(p, idx);
