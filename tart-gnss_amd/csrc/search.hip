@@ -1152,6 +1152,7 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
   __shared__ uint32_t buf0[16 * SB];
   __shared__ uint32_t buf1[31 * SC];
   __shared__ u32x4 fa_lds[15 * 64];  // radix-33 fragments [t*3 + kb][lane]
+  __shared__ u32x4 fc_lds[8 * 64];   // radix-31 fragments [t*2 + kb][lane] (registers would spill)
   __shared__ float2 stw[256];
   __shared__ float2 t16[31 * 16];  // W_496^{a q}, [a][q]
   __shared__ RowRed<NW16> red;
@@ -1161,13 +1162,11 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
   for (int i = tid; i < 31 * 16; i += T16) t16[i] = tw_lds(stw, 33 * (i >> 4) * (i & 15));
   for (int i = tid; i < 15 * 64; i += T16)
     fa_lds[i] = __builtin_bit_cast(u32x4, dft_frag<33>(stw, (i >> 6) / 3, (i >> 6) % 3, i & 63));
-  halfx8 fb[2], fc[4][2];
+  for (int i = tid; i < 8 * 64; i += T16)
+    fc_lds[i] = __builtin_bit_cast(u32x4, dft_frag<31>(stw, (i >> 6) >> 1, (i >> 6) & 1, i & 63));
+  halfx8 fb[2];
 #pragma unroll
   for (int t = 0; t < 2; ++t) fb[t] = dft_frag<16>(stw, t, 0, tid & 63);
-#pragma unroll
-  for (int t = 0; t < 4; ++t)
-#pragma unroll
-    for (int kb = 0; kb < 2; ++kb) fc[t][kb] = dft_frag<31>(stw, t, kb, tid & 63);
   __syncthreads();
   RowPending pend;  // the previous row, written after this row's first barrier
   // SUB == 0: persistent blocks over the items (xrow, PRN group) b, b + gridDim, ... with
@@ -1329,16 +1328,21 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
       halfx8 b0[2], b1[2];
       load_c(j0, b0);
       load_c(j1, b1);
+      auto fcf = [&](int t, int kb) {
+        return __builtin_bit_cast(halfx8, fc_lds[(t * 2 + kb) * 64 + lane]);
+      };
       floatx4 c0[4], c1[4];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        c0[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fc[t][0], b0[0], floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-        c1[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fc[t][0], b1[0], floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        const halfx8 f = fcf(t, 0);
+        c0[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(f, b0[0], floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        c1[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(f, b1[0], floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
       }
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        c0[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fc[t][1], b0[1], c0[t], 0, 0, 0);
-        c1[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fc[t][1], b1[1], c1[t], 0, 0, 0);
+        const halfx8 f = fcf(t, 1);
+        c0[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(f, b0[1], c0[t], 0, 0, 0);
+        c1[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(f, b1[1], c1[t], 0, 0, 0);
       }
       Top2 lv[16];
 #pragma unroll
@@ -1363,14 +1367,8 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
         const int j = 512 + ln16;
         halfx8 b[2];
         load_c(j, b);
-        // output tile t = wave: select its fragments without dynamic register indexing
-        halfx8 f0 = fc[0][0], f1 = fc[0][1];
-#pragma unroll
-        for (int t = 1; t < 4; ++t)
-          if (wave == t) {
-            f0 = fc[t][0];
-            f1 = fc[t][1];
-          }
+        // output tile t = wave
+        const halfx8 f0 = fcf(wave, 0), f1 = fcf(wave, 1);
         floatx4 acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(f0, b[0], floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(f1, b[1], acc, 0, 0, 0);
         const int q = 8 * wave + 2 * grp;
