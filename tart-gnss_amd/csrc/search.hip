@@ -61,6 +61,12 @@ __device__ __forceinline__ float2 wipe_obs_f64(const uint8_t* row, int fmt, long
   sincos(__dmul_rn(__dmul_rn(pc, static_cast<double>(n)), f), &sd, &cd);
   return make_float2(__double2float_rn(cd) * s, __double2float_rn(sd) * s);
 }
+// (cos, sin) of an f64 phase rounded to f32 (the FP16 precision form's carrier)
+__device__ __forceinline__ float2 cossin_f64(double ph) {
+  double sd, cd;
+  sincos(ph, &sd, &cd);
+  return make_float2(__double2float_rn(cd), __double2float_rn(sd));
+}
 
 // ---- block reductions (wave64 shuffles, then LDS across waves) --------------------------------
 __device__ __forceinline__ void arg_combine(float& p, int& i, float q, int j) {
@@ -362,14 +368,33 @@ __global__ __launch_bounds__(T) void k_forward(FwdArgs a) {
   const uint8_t* row = a.packed + static_cast<size_t>(ant) * a.row_bytes;
   const double mean = a.means[ant];
   const double f = a.fc[bin];
+  // The packed bytes of the block's samples (2P, or one epoch of P) are staged in LDS by
+  // independent coalesced loads; decoding straight from global memory made every sample of the
+  // wipe loop wait for its own load.  The first sample e P is byte aligned (P = 8 x 2046).
+  __shared__ uint8_t raw[2 * P / 4];
+  const long long s0 = NC ? static_cast<long long>(e) * P : 0;
+  const int bits = a.fmt == TGA_INPUT_1BIT ? 1 : 2;
+  const int nbytes = ((NC ? P : 2 * P) * bits) >> 3;
+  const uint8_t* src = row + ((s0 * bits) >> 3);
+#pragma unroll 4
+  for (int i = tid; i < nbytes; i += T) raw[i] = src[i];
+  __syncthreads();
+  auto level = [&](int n) {  // f32(level - mean) of block-local sample n
+    return __double2float_rn(static_cast<double>(decode_level(raw, a.fmt, n)) - mean);
+  };
   for (int n = tid; n < P; n += T) {
     if constexpr (NC) {
-      buf[n] = wipe_obs(row, a.fmt, n + static_cast<long long>(e) * P, mean, a.pc, f);
+      buf[n] = wipe_value(level(n), __dmul_rn(a.pc, static_cast<double>(n + s0)), f);
+    } else if constexpr (F16) {
+      const float2 w0 = cossin_f64(__dmul_rn(__dmul_rn(a.pc, static_cast<double>(n)), f));
+      const float2 w1 = cossin_f64(__dmul_rn(__dmul_rn(a.pc, static_cast<double>(n + P)), f));
+      const float l0 = level(n), l1 = level(n + P);
+      const float2 x0 = make_float2(w0.x * l0, w0.y * l0);
+      const float2 x1 = make_float2(w1.x * l1, w1.y * l1);
+      buf[n] = h == 0 ? cadd(x0, x1) : cmul(csub(x0, x1), a.tau[n]);
     } else {
-      const float2 x0 = F16 ? wipe_obs_f64(row, a.fmt, n, mean, a.pc, f)
-                            : wipe_obs(row, a.fmt, n, mean, a.pc, f);
-      const float2 x1 = F16 ? wipe_obs_f64(row, a.fmt, n + P, mean, a.pc, f)
-                            : wipe_obs(row, a.fmt, n + P, mean, a.pc, f);
+      const float2 x0 = wipe_value(level(n), __dmul_rn(a.pc, static_cast<double>(n)), f);
+      const float2 x1 = wipe_value(level(n + P), __dmul_rn(a.pc, static_cast<double>(n + P)), f);
       buf[n] = h == 0 ? cadd(x0, x1) : cmul(csub(x0, x1), a.tau[n]);
     }
   }
