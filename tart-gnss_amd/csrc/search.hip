@@ -763,7 +763,8 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
   // One transform of a correlation row (or sub-row): `operand(ta, r)` gives pass A's input r of
   // butterfly ta; every output power goes to `sink(slot, p, idx)` with slot < 31 for the
   // thread's main set (outputs more than 32 samples apart) and slot 31 / 32 for its extra set.
-  auto transform = [&](auto operand, auto sink) {
+  // `after_a()` runs once pass A has consumed its operands (prefetches for the next row).
+  auto transform = [&](auto operand, auto sink, auto after_a) {
     // ---- pass A
     int ta = tid;
     asm volatile("" : "+v"(ta));
@@ -775,6 +776,7 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
 #pragma unroll
       for (int r = 0; r < 33; ++r) buf[ta * 33 + r] = v[r];
     }
+    after_a();
     lds_barrier();
     // ---- pass B
     {
@@ -972,7 +974,7 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
               return cconjmul(buffer_load_f2(xv, ta * 8, r * NBA * 8),
                               buffer_load_f2(rv, ta * 8, r * NBA * 8));
             },
-            [&](int slot, float p, int) { acc[slot] += p; });
+            [&](int slot, float p, int) { acc[slot] += p; }, [] {});
         lds_barrier();  // the next epoch's pass A overwrites the LDS row
       }
       // the outputs of each slot (same indices in every epoch, see pass C)
@@ -1021,22 +1023,47 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
       for (int r = 0; r < 33; ++r)
         xr[r] = tid < NBA ? buffer_load_f2(xv, tid * 8, r * NBA * 8) : make_float2(0.f, 0.f);
     }
+    // With X in registers the next PRN's replica inputs are loaded right after pass A, so
+    // their latency hides behind passes B and C.
+    auto rep_view = [&](int prn) {
+      return buffer_view(a.rep + (static_cast<size_t>(prn) * 2 + h) * P, P * 8);
+    };
+    float2 rn[kKeepX ? 33 : 1];
+    auto load_rep = [&](int prn) {
+      const __amdgpu_buffer_rsrc_t rv = rep_view(prn);
+#pragma unroll
+      for (int r = 0; r < 33; ++r)
+        rn[r] = tid < NBA ? buffer_load_f2(rv, tid * 8, r * NBA * 8) : make_float2(0.f, 0.f);
+    };
+    if constexpr (kKeepX) {
+      if (p0 < p1) load_rep(p0);
+    }
     for (int prn = p0; prn < p1; ++prn) {
-      const __amdgpu_buffer_rsrc_t rv =
-          buffer_view(a.rep + (static_cast<size_t>(prn) * 2 + h) * P, P * 8);
+      const __amdgpu_buffer_rsrc_t rv = rep_view(prn);
       Top2 ms, es;
       float accn = 0.0f;  // sum of powers: NaN-propagating detector
+      float2 rc[kKeepX ? 33 : 1];
+      if constexpr (kKeepX) {
+#pragma unroll
+        for (int r = 0; r < 33; ++r) rc[r] = rn[r];
+      }
       transform(
           [&](int ta, int r) {
-            const float2 rr = buffer_load_f2(rv, ta * 8, r * NBA * 8);
-            if constexpr (kKeepX)
-              return cconjmul(xr[r], rr);
-            else
+            if constexpr (kKeepX) {
+              return cconjmul(xr[r], rc[r]);
+            } else {
+              const float2 rr = buffer_load_f2(rv, ta * 8, r * NBA * 8);
               return cconjmul(buffer_load_f2(xv, ta * 8, r * NBA * 8), rr);
+            }
           },
           [&](int slot, float p, int idx) {
             accn += p;
             (slot < 31 ? ms : es).add(p, idx);
+          },
+          [&] {
+            if constexpr (kKeepX) {
+              if (prn + 1 < p1) load_rep(prn + 1);
+            }
           });
       // the two-barrier finish: deferring its second half to the next row's first barrier
       // measured 6% slower here (wave 0 then carries the flush into the next pass)
@@ -1055,7 +1082,8 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
                 [&](int slot, float p, int idx) {
                   accn += p;
                   (slot < 31 ? ms : es).add(p, idx);
-                });
+                },
+                [] {});
       const long long r = big_row(a.r0 + rq, a.n_prn, a.n_bin, a.n_ant, a.hs);
       row_finish_key<T>(ms, es, accn != accn, red, k0, SUB, a.rowkey + r, a.rownan + r);
     }
