@@ -166,6 +166,20 @@ def test_split_e1c_at_16368khz(gpu):
     assert int(peaks[0][1, 2]["best_codephase"]) == (65472 - 123) % 65472
 
 
+def test_split_row_tiles_ragged(gpu):
+    # long rows are enumerated in tiles of 8 forward spectra x 8 replica spectra, one hypothesis
+    # at a time: 10 PRNs and 2 x 5 (bin, h) spectra leave full and ragged tiles in both
+    # directions, and every row must still land at its own index
+    fs, L = FS, 2 * 65472
+    signals = [dict(system="galileo", prn=9, cn0=52.0, doppler=-1000.0, delay=31000)]
+    fc = grid(IF, 1000.0, 500.0)
+    systems = [("galileo", list(range(1, 11)), fc)]
+    plan, peaks, packed = run_case(fs, L, 1, 1, systems, signals, seed=23)
+    assert "split4x16368" in plan.describe()
+    assert check_against_oracle(plan, peaks, packed, fs, L, 1, systems) == 10
+    assert int(peaks[0][0, 8]["best_codephase"]) == (65472 - 31000) % 65472
+
+
 @pytest.mark.parametrize("system,prn", [("beidou", 19), ("l1c", 7)])
 def test_split_b1c_l1c_at_16368khz(gpu, system, prn):
     # B1C / L1C at 16.368 MHz: P = 163680 = 10 x 16368 (BOC(1,1) half-chips)
