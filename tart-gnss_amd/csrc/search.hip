@@ -1594,16 +1594,19 @@ __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
                                                  const float2* __restrict__ rep,
                                                  const float2* __restrict__ twb, int n_prn,
                                                  int n_bin, int n_ant, int hs, long long r0,
-                                                 long long total,
+                                                 long long n_rows,
                                                  void* __restrict__ Yv,
                                                  unsigned long long* __restrict__ rowkey,
                                                  int* __restrict__ rownan) {
   constexpr int P1 = kFusedP;
   constexpr int P = R0 * P1;
-  const long long gid = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-  if (gid >= total) return;
-  const long long rr = gid / P1;
-  const int n1 = static_cast<int>(gid - rr * P1);
+  // 64 blocks of 256 samples per row; blocks are dealt round-robin to the 8 XCDs, so block b
+  // takes chunk 8 (b mod 8) + (b / 8) mod 8 of row b / 64: each XCD then works on one
+  // contiguous eighth of every row and its L2 holds contiguous slices of the tile's spectra
+  const long long rr = blockIdx.x >> 6;
+  const int chunk = 8 * static_cast<int>(blockIdx.x & 7) + static_cast<int>((blockIdx.x >> 3) & 7);
+  const int n1 = 256 * chunk + static_cast<int>(threadIdx.x);
+  if (rr >= n_rows || n1 >= P1) return;
   const long long I = r0 + rr;  // tiled order (big_item)
   const BigItem it = big_item(I, n_prn, n_bin, n_ant, hs);
   const long long xrow = it.xrow;
@@ -1963,8 +1966,8 @@ void launch_big_pre(const float2* X, const float2* rep, const float2* twb, int R
                     int n_bin, int n_ant, int hs, long long r0, long long r1, void* Y, bool f16,
                     unsigned long long* rowkey, int* rownan, hipStream_t st) {
   const long long nr = r1 - r0;
-  const long long total = nr * kFusedP;
-  const dim3 grid(grid_for(total, 256));
+  const long long total = nr;  // rows of this chunk, 64 blocks each
+  const dim3 grid(static_cast<unsigned>(nr * 64));
   if (R0 == 4 && !f16)
     hipLaunchKernelGGL((k_big_pre<4, false>), grid, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
                        n_ant, hs, r0, total, Y, rowkey, rownan);
