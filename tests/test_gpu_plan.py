@@ -150,6 +150,30 @@ def test_plan_errors(gpu):
     assert e.value.name == "TGA_ERR_ANTENNA_RANGE"
 
 
+def test_masked_second_peak_many_code_phases(gpu, fused_variant):
+    # A 60 dB-Hz signal at 128 code phases (one per antenna): the main lobe's neighbours within
+    # cw = 16 samples are nearly as strong as the peak, so a top-2 set of a lane that held two
+    # outputs closer than 2 cw would report one of them as the second peak (a grouping with such
+    # pairs affects a few percent of the phases, so 128 phases find it with high probability).
+    # Both radix-31 implementations must give the oracle's masked second peak at every phase.
+    n_ant = 128
+    L = 2 * 16368
+    rows = []
+    delays = [int(d) for d in np.random.default_rng(31).integers(0, 16368, n_ant)]
+    for a, d in enumerate(delays):
+        sig = [dict(system="gps", prn=4, cn0=60.0, doppler=500.0, delay=d)]
+        rows.append(synth.synth_observation(1, L, FS, sig, fmt=1, seed=100 + a)[0])
+    packed = np.stack(rows)
+    fc = grid(IF + 500.0, 500.0, 500.0)
+    systems = [("gps", [4], fc)]
+    plan = tga.Plan(FS, n_ant, L, [tga.SystemSearch("gps", fc, [4])])
+    assert "fused16368" in plan.describe()
+    peaks = plan.search(packed)
+    assert check_against_oracle(plan, peaks, packed, FS, L, 1, systems) == n_ant
+    for a, d in enumerate(delays):
+        assert int(peaks[0][a, 0]["best_codephase"]) % 16368 == (16368 - d) % 16368
+
+
 def test_split_e1c_at_16368khz(gpu):
     # E1-C at 16.368 MHz: P = 65472 = 4 x 16368, rows transformed as 4 twiddled sub-rows by the
     # fused LDS kernel; the second peak comes from recomputing the winning row
