@@ -356,8 +356,8 @@ void Plan::autotune(Group& g) {
   if (pin_g) {
     grs.push_back(std::max(1, std::atoi(pin_g)));
   } else {
-    for (int gr : {8, 16, 32})  // groupings beyond the PRN count only add empty groups
-      if (gr == 8 || gr / 2 < g.n_prn) grs.push_back(gr);
+    for (int gr : {1, 2, 4, 8, 16, 32})  // groupings beyond the PRN count only add empty groups
+      if (gr <= 8 || gr / 2 < g.n_prn) grs.push_back(gr);
   }
   std::vector<FusedRowsConfig> cands;
   // the FP16 and non-coherent kernels have one radix-31 implementation

@@ -45,7 +45,8 @@ void launch_fused_forward(const uint8_t* packed, size_t row_bytes, int fmt, cons
 void launch_to_pfa(const float2* in, float2* out, long long n, hipStream_t st);
 // Tunable shape of the rows kernel: `variant` 0 runs the radix-31 pass on the VALU, 1 on the
 // matrix cores; `groups` PRN groups per (antenna, bin, hypothesis) (a multiple of the 8 XCDs
-// keeps each group's replica spectra within one XCD's L2).
+// keeps each group's replica spectra within one XCD's L2; fewer groups share each block's
+// forward spectrum among more PRNs).
 struct FusedRowsConfig {
   int variant = 0;
   int groups = 8;
