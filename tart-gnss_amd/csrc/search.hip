@@ -999,8 +999,10 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
       if (tid == 0) out[1] = RowOut{0.0f, -1.0f, 0, 0};  // single hypothesis: h = 1 never wins
     }
   } else if constexpr (SUB == 0) {
-    // Blocks are dealt round-robin to the 8 XCDs: the PRN group follows blockIdx % groups so
-    // the replica spectra a group touches stay in one XCD's L2.  One block per (xrow, group):
+    // Blocks are dealt round-robin to the 8 XCDs: the PRN group follows blockIdx % groups, so
+    // with 8 or more groups the replica spectra a group touches stay in one XCD's L2 (with
+    // fewer, a group spans 8 / groups XCDs and more PRNs share each block's forward spectrum;
+    // the autotuner picks the count).  One block per (xrow, group):
     // an outer loop over several items per block (the persistent form of the FP16 kernel)
     // measured 45% slower here, even when it ran once per block.
     const int g = blockIdx.x % a.groups;
@@ -1013,9 +1015,9 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
     const __amdgpu_buffer_rsrc_t xv = buffer_view(a.X + static_cast<size_t>(xrow) * P, P * 8);
     // The block's forward spectrum is the same for every PRN.  With the matrix-core radix-31
     // (few VGPRs elsewhere) each thread keeps its 33 pass-A inputs in registers (66 VGPRs)
-    // instead of re-reading them per PRN through L2, where the 32 rows of an XCD do not fit
-    // beside the replicas; the VALU codelet needs those registers itself (255 VGPRs and 20%
-    // slower with them) and re-reads.
+    // instead of re-reading them per PRN through L2, where the 32 X rows in flight on an XCD
+    // do not fit beside the replicas; the VALU codelet needs those registers itself (255 VGPRs
+    // and 20% slower with them) and re-reads.
     constexpr bool kKeepX = VARIANT == 1;
     float2 xr[kKeepX ? 33 : 1];
     if constexpr (kKeepX) {
