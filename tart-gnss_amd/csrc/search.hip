@@ -1241,6 +1241,16 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
   const long long step = SUB == 0 ? 1 : gridDim.x;
   const __amdgpu_buffer_rsrc_t xv =
       buffer_view(SUB == 0 ? a.X + static_cast<size_t>(xrow) * P : a.Y, P * 4);
+  // the X words of the lane's first pass-A tile are the same for every PRN of the item: kept
+  // in 9 VGPRs (the second tile's would not fit the 128-VGPR budget of 16 waves)
+  uint32_t xk[9];
+  if constexpr (SUB == 0) {
+    const int w0 = tid >> 6, l16 = tid & 15, gp = (tid & 63) >> 4;
+    const int vo = (16 * w0 + l16 + (P / 33) * 4 * gp) * 4;
+#pragma unroll
+    for (int u = 0; u < 9; ++u)
+      xk[u] = __builtin_amdgcn_raw_buffer_load_b32(xv, vo, (P / 33) * (16 * (u >> 2) + (u & 3)) * 4, 0);
+  }
   for (long long it = it0; it < it1; it += step) {
     const int prn = static_cast<int>(it);
     // SUB == 0: replica row; SUB > 0: the sub-row itself
@@ -1261,7 +1271,16 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
         halfx8 b[3];
         if constexpr (SUB == 0) {
           TileA cur;
-          load_tile_a(cur, xv, rv, (j + NBA * 4 * grp) * 4);
+          if (s == 0) {
+            const int vo = (j + NBA * 4 * grp) * 4;
+#pragma unroll
+            for (int u = 0; u < 9; ++u) {
+              cur.x[u] = xk[u];
+              cur.r[u] = __builtin_amdgcn_raw_buffer_load_b32(rv, vo, NBA * (16 * (u >> 2) + (u & 3)) * 4, 0);
+            }
+          } else {
+            load_tile_a(cur, xv, rv, (j + NBA * 4 * grp) * 4);
+          }
 #pragma unroll
           for (int kb = 0; kb < 3; ++kb) {
             u32x4 w;
