@@ -264,6 +264,14 @@ void Plan::build(const tga_plan_desc& d) {
     g.d_rows.alloc(static_cast<size_t>(g.n_rows) * sizeof(RowOut));
     g.n_prn = n_prn;
     g.n_bin = n_bin;
+    if (g.fused) {
+      // the forward kernels' carrier table: one (cos, sin) per (bin, sample) for all antennas
+      const long long clen = static_cast<long long>(g.nc > 0 ? g.nc : 2) * kFusedP;
+      g.d_carrier.alloc(static_cast<size_t>(n_bin) * clen * sizeof(float2));
+      launch_carrier(g.d_fc.as<double>(), n_bin, clen, phase_const, g.fp16,
+                     g.d_carrier.as<float2>(), stream);
+      TGA_HIP(hipStreamSynchronize(stream));
+    }
     if (g.fp16) {
       const long long nr = static_cast<long long>(n_prn) * 2 * static_cast<long long>(M);
       g.d_rep16.alloc(static_cast<size_t>(nr) * sizeof(uint32_t));
@@ -468,16 +476,17 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
     if (g.nc > 0) {
       launch_fused_forward_nc(d_in, row_bytes, fmt, d_means.as<double>(), g.d_fc.as<double>(),
                               n_ant, g.n_bin, g.nc, phase_const, g.d_tw.as<float2>(),
-                              g.d_X.as<float2>(), st);
+                              g.d_carrier.as<float2>(), g.d_X.as<float2>(), st);
       g.X = g.d_X.as<float2>();
     } else if (g.fp16) {
       launch_fused_forward16(d_in, row_bytes, fmt, d_means.as<double>(), g.d_fc.as<double>(),
                              n_ant, g.n_bin, phase_const, g.d_tau.as<float2>(),
-                             g.d_tw.as<float2>(), g.d_X16.as<uint32_t>(), st);
+                             g.d_tw.as<float2>(), g.d_carrier.as<float2>(),
+                             g.d_X16.as<uint32_t>(), st);
     } else if (g.fused) {
       launch_fused_forward(d_in, row_bytes, fmt, d_means.as<double>(), g.d_fc.as<double>(),
                            n_ant, g.n_bin, phase_const, g.d_tau.as<float2>(),
-                           g.d_tw.as<float2>(), g.d_X.as<float2>(), st);
+                           g.d_tw.as<float2>(), g.d_carrier.as<float2>(), g.d_X.as<float2>(), st);
       g.X = g.d_X.as<float2>();
     } else {
       if (g.folded)

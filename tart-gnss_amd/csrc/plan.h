@@ -29,6 +29,7 @@ struct Group {
   float scale = 1.f;
   int cw = 1;
   DevBuf d_fc, d_slots, d_tau, d_rep, d_X, d_Xs, d_rows, d_Z, d_Zs, d_tw;
+  DevBuf d_carrier;  // fused groups: carrier (cos, sin) per (bin, sample), built at plan creation
   DevBuf d_rep16, d_X16;  // FP16 form: f16 (re, im) pairs scaled by kF16Scale
   DevBuf d_Y[2], d_rowkey, d_rownan, d_twb, d_win, d_Zw, d_Zws;  // long-period split
   float2* X = nullptr;  // where the forward spectra of the last search landed

@@ -35,9 +35,16 @@ void launch_ant_sums(const uint8_t* packed, size_t row_bytes, int n_ant, int64_t
 constexpr int kFusedP = 16368;
 bool fused_supported(size_t P, size_t N, int cw);
 // X[(ant*n_bin + bin)*2 + h][P]: FFT of the folded sum (h=0) / twisted difference (h=1).
+// carrier: the plan's carrier table (launch_carrier) of 2 P samples per bin.
 void launch_fused_forward(const uint8_t* packed, size_t row_bytes, int fmt, const double* means,
                           const double* fc, int n_ant, int n_bin, double phase_const,
-                          const float2* tau, const float2* tw, float2* X, hipStream_t st);
+                          const float2* tau, const float2* tw, const float2* carrier, float2* X,
+                          hipStream_t st);
+// Carrier table out[bin][n] for n < clen: (cos, sin) of phi = f32(f64(pc * n) * fc[bin]) evaluated
+// in f64 and rounded to f32 (the reference form), or of the f64 phase (phase64, the FP16 form).
+// The wipe-off is the same for every antenna, so the forward kernels read it from this table.
+void launch_carrier(const double* fc, int n_bin, long long clen, double phase_const,
+                    bool phase64, float2* out, hipStream_t st);
 // The fused FP32 kernels store X and read the replica spectra in the prime-factor (Good-Thomas)
 // order of 16368 = 33 x 16 x 31 (position 496 n1 + 31 n2 + n3 holds natural index
 // (496 n1 + 1023 n2 + 528 n3) mod 16368), for which the three radix passes of a correlation row
@@ -60,8 +67,8 @@ void launch_fused_rows(const float2* X, const float2* rep, const float2* tw, int
 // over epochs (the h = 1 slot is a row that never wins the selection).
 void launch_fused_forward_nc(const uint8_t* packed, size_t row_bytes, int fmt,
                              const double* means, const double* fc, int n_ant, int n_bin,
-                             int epochs, double phase_const, const float2* tw, float2* X,
-                             hipStream_t st);
+                             int epochs, double phase_const, const float2* tw,
+                             const float2* carrier, float2* X, hipStream_t st);
 void launch_fused_rows_nc(const float2* X, const float2* rep, const float2* tw, int n_ant,
                           int n_bin, int n_prn, int epochs, int cw, RowOut* rows, int groups,
                           hipStream_t st);
@@ -79,7 +86,7 @@ constexpr float kSplitF16Scale = 1.0f / 1048576.0f;  // 2^-20
 void launch_fused_forward16(const uint8_t* packed, size_t row_bytes, int fmt,
                             const double* means, const double* fc, int n_ant, int n_bin,
                             double phase_const, const float2* tau, const float2* tw,
-                            uint32_t* X16, hipStream_t st);
+                            const float2* carrier, uint32_t* X16, hipStream_t st);
 void launch_fused_rows16(const uint32_t* X16, const uint32_t* rep16, const float2* tw,
                          int n_ant, int n_bin, int n_prn, int cw, RowOut* rows, int groups,
                          hipStream_t st);

@@ -338,6 +338,9 @@ struct FwdArgs {
   float2* X;     // FP32 form
   uint32_t* X16;  // FP16 form: f16 (re, im) pairs of X * kF16Scale
   int epochs;     // non-coherent form: one-period epochs per (antenna, bin)
+  // the carrier (cos, sin) of every (bin, sample), [n_bin][clen]: the same for all antennas
+  const float2* carrier = nullptr;
+  long long clen = 0;
 };
 
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
@@ -382,19 +385,20 @@ __global__ __launch_bounds__(T) void k_forward(FwdArgs a) {
   auto level = [&](int n) {  // f32(level - mean) of block-local sample n
     return __double2float_rn(static_cast<double>(decode_level(raw, a.fmt, n)) - mean);
   };
+  // the carrier comes from the plan's table (computed once with the same rounding as
+  // wipe_value / the f64 phase of the FP16 form), so no antenna repeats its cos / sin
+  const float2* car = a.carrier + static_cast<size_t>(bin) * a.clen + s0;
+  auto wipe = [&](int n) {
+    const float l = level(n);
+    const float2 c = car[n];
+    return make_float2(c.x * l, c.y * l);
+  };
+  (void)f;
   for (int n = tid; n < P; n += T) {
     if constexpr (NC) {
-      buf[n] = wipe_value(level(n), __dmul_rn(a.pc, static_cast<double>(n + s0)), f);
-    } else if constexpr (F16) {
-      const float2 w0 = cossin_f64(__dmul_rn(__dmul_rn(a.pc, static_cast<double>(n)), f));
-      const float2 w1 = cossin_f64(__dmul_rn(__dmul_rn(a.pc, static_cast<double>(n + P)), f));
-      const float l0 = level(n), l1 = level(n + P);
-      const float2 x0 = make_float2(w0.x * l0, w0.y * l0);
-      const float2 x1 = make_float2(w1.x * l1, w1.y * l1);
-      buf[n] = h == 0 ? cadd(x0, x1) : cmul(csub(x0, x1), a.tau[n]);
+      buf[n] = wipe(n);
     } else {
-      const float2 x0 = wipe_value(level(n), __dmul_rn(a.pc, static_cast<double>(n)), f);
-      const float2 x1 = wipe_value(level(n + P), __dmul_rn(a.pc, static_cast<double>(n + P)), f);
+      const float2 x0 = wipe(n), x1 = wipe(n + P);
       buf[n] = h == 0 ? cadd(x0, x1) : cmul(csub(x0, x1), a.tau[n]);
     }
   }
@@ -1486,6 +1490,23 @@ __global__ void k_to_pfa(const float2* __restrict__ in, float2* __restrict__ out
   out[i] = in[row * P + pfa_nat(pos)];
 }
 
+// carrier table [n_bin][clen]: (cos, sin) of phi = f32(f64(pc * n) * fc[bin]) in f64 rounded to
+// f32 (wipe_value), or of the f64 phase itself (PHASE64, the FP16 form)
+template <bool PHASE64>
+__global__ void k_carrier(const double* __restrict__ fc, double pc, long long clen,
+                          long long total, float2* __restrict__ out) {
+  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= total) return;
+  const long long bin = i / clen;
+  const long long n = i - bin * clen;
+  const double f = fc[bin];
+  if constexpr (PHASE64) {
+    out[i] = cossin_f64(__dmul_rn(__dmul_rn(pc, static_cast<double>(n)), f));
+  } else {
+    out[i] = wipe_value(1.0f, __dmul_rn(pc, static_cast<double>(n)), f);
+  }
+}
+
 // rows of P f32 pairs (natural order) -> f16 pairs scaled by s in the prime-factor order
 __global__ void k_to_f16(const float2* __restrict__ in, uint32_t* __restrict__ out, long long n,
                          float s) {
@@ -1815,8 +1836,11 @@ std::vector<float2> fused_twiddles() {
 
 void launch_fused_forward(const uint8_t* packed, size_t row_bytes, int fmt, const double* means,
                           const double* fc, int n_ant, int n_bin, double phase_const,
-                          const float2* tau, const float2* tw, float2* X, hipStream_t st) {
+                          const float2* tau, const float2* tw, const float2* carrier, float2* X,
+                          hipStream_t st) {
   fused::FwdArgs a{packed, row_bytes, fmt, means, fc, n_bin, phase_const, tau, tw, X, nullptr, 0};
+  a.carrier = carrier;
+  a.clen = 2LL * fused::P;
   hipLaunchKernelGGL(fused::k_forward<false>, dim3(n_ant * n_bin * 2), dim3(fused::T), 0, st, a);
   TGA_HIP(hipGetLastError());
 }
@@ -1853,10 +1877,12 @@ void launch_fused_rows(const float2* X, const float2* rep, const float2* tw, int
 
 void launch_fused_forward_nc(const uint8_t* packed, size_t row_bytes, int fmt,
                              const double* means, const double* fc, int n_ant, int n_bin,
-                             int epochs, double phase_const, const float2* tw, float2* X,
-                             hipStream_t st) {
+                             int epochs, double phase_const, const float2* tw,
+                             const float2* carrier, float2* X, hipStream_t st) {
   fused::FwdArgs a{packed, row_bytes, fmt, means, fc, n_bin, phase_const, nullptr, tw, X,
                    nullptr, epochs};
+  a.carrier = carrier;
+  a.clen = static_cast<long long>(epochs) * fused::P;
   hipLaunchKernelGGL((fused::k_forward<false, true>), dim3(n_ant * n_bin * epochs),
                      dim3(fused::T), 0, st, a);
   TGA_HIP(hipGetLastError());
@@ -1878,9 +1904,11 @@ void launch_fused_rows_nc(const float2* X, const float2* rep, const float2* tw, 
 void launch_fused_forward16(const uint8_t* packed, size_t row_bytes, int fmt,
                             const double* means, const double* fc, int n_ant, int n_bin,
                             double phase_const, const float2* tau, const float2* tw,
-                            uint32_t* X16, hipStream_t st) {
+                            const float2* carrier, uint32_t* X16, hipStream_t st) {
   fused::FwdArgs a{packed, row_bytes, fmt, means, fc, n_bin, phase_const, tau, tw, nullptr, X16,
                    0};
+  a.carrier = carrier;
+  a.clen = 2LL * fused::P;
   hipLaunchKernelGGL(fused::k_forward<true>, dim3(n_ant * n_bin * 2), dim3(fused::T), 0, st, a);
   TGA_HIP(hipGetLastError());
 }
@@ -1895,6 +1923,18 @@ void launch_fused_rows16(const uint32_t* X16, const uint32_t* rep16, const float
   const long long blocks = persistent_blocks(a.n_items, groups);
   hipLaunchKernelGGL(fused::k_rows16<0>, dim3(static_cast<unsigned>(blocks)),
                      dim3(fused::T16), 0, st, a);
+  TGA_HIP(hipGetLastError());
+}
+
+void launch_carrier(const double* fc, int n_bin, long long clen, double phase_const,
+                    bool phase64, float2* out, hipStream_t st) {
+  const long long total = static_cast<long long>(n_bin) * clen;
+  if (phase64)
+    hipLaunchKernelGGL(fused::k_carrier<true>, dim3(grid_for(total, 256)), dim3(256), 0, st, fc,
+                       phase_const, clen, total, out);
+  else
+    hipLaunchKernelGGL(fused::k_carrier<false>, dim3(grid_for(total, 256)), dim3(256), 0, st, fc,
+                       phase_const, clen, total, out);
   TGA_HIP(hipGetLastError());
 }
 
