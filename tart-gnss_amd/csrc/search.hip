@@ -370,7 +370,6 @@ __global__ __launch_bounds__(T) void k_forward(FwdArgs a) {
   const int ant = ab / a.n_bin;
   const uint8_t* row = a.packed + static_cast<size_t>(ant) * a.row_bytes;
   const double mean = a.means[ant];
-  const double f = a.fc[bin];
   // The packed bytes of the block's samples (2P, or one epoch of P) are staged in LDS by
   // independent coalesced loads; decoding straight from global memory made every sample of the
   // wipe loop wait for its own load.  The first sample e P is byte aligned (P = 8 x 2046).
@@ -393,7 +392,6 @@ __global__ __launch_bounds__(T) void k_forward(FwdArgs a) {
     const float2 c = car[n];
     return make_float2(c.x * l, c.y * l);
   };
-  (void)f;
   for (int n = tid; n < P; n += T) {
     if constexpr (NC) {
       buf[n] = wipe(n);
