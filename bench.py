@@ -1,22 +1,24 @@
 """Benchmark: batched code-phase/Doppler search throughput on one MI355X.
 
-Default workload (``--workload ca24``): GPS + SBAS + QZSS L1 C/A, 24 antennas, 2-bit real
-samples at 16.368 MHz, 10 ms records (163680 samples); seeded signals: 8 GPS, 2 SBAS and 1
-QZSS at C/N0 uniform in [38, 50] dB-Hz with per-SV Doppler and per-antenna carrier offsets
-from a 3 m disk; search 38 + 39 + 23 PRNs x 41 Doppler bins (+-10 kHz, 500 Hz step), N = 2P
-(reference form, two hypotheses), FP32, plus ACR C/N0 on the host.  The FP16 matrix-core form
-(f16 spectra or f16 sub-rows, radix passes as f16 GEMMs, f64 carrier phase) then runs the same
-observation and is reported under extra.fp16_mfma with its difference to the FP32 peaks.
+Default workload (``--workload all24``): all six systems in one plan, 24 antennas, 2-bit real
+samples at 16.368 MHz, 40 ms records (654720 samples); seeded signals: 8 GPS, 2 SBAS and 1 QZSS
+at C/N0 uniform in [38, 50] dB-Hz and 6 each of Galileo E1-C, BeiDou B1C and GPS L1C at
+[40, 48] dB-Hz, per-SV Doppler and per-antenna carrier offsets from a 3 m disk; search GPS +
+SBAS + QZSS L1 C/A 38 + 39 + 23 PRNs x 41 bins (+-10 kHz, 500 Hz), E1-C 50 SVs x 81 bins
+(+-5 kHz, 125 Hz), B1C and L1C 63 SVs x 201 bins each (+-5 kHz, 50 Hz), N = 2P (reference
+form, two hypotheses): 1.0746e11 cells per search, FP32, plus ACR C/N0 on the host.  The FP16
+matrix-core form (f16 spectra or f16 sub-rows, radix passes as f16 GEMMs, f64 carrier phase)
+then runs the same observation and is reported under extra.fp16_mfma with its difference to
+the FP32 peaks.
 
 Other workloads (same JSON line):
+  ca24     GPS + SBAS + QZSS L1 C/A only (the C/A part of all24 at 10 ms: 1.611e9 cells).
   ca1      GPS L1 C/A, 1 antenna, 1-bit, 10 ms: 4 signals at 45 dB-Hz, PRN 1-32 x 21 bins
            (+-5 kHz, 500 Hz).
   ca1nc    ca1 with 1 ms coherent x 10 non-coherent epochs (the non-coherent form; same cells).
   pilots24 Galileo E1-C (P = 65472), BeiDou B1C and GPS L1C (P = 163680), 24 antennas, 1-bit,
            40 ms: 6 signals per system at 40-48 dB-Hz; E1-C 50 SVs x 81 bins (125 Hz), B1C and
            L1C 63 SVs x 201 bins (50 Hz), N = 2P.
-  all24    all six systems, 24 antennas, 2-bit, 40 ms: the C/A signals and grid of ca24 and the
-           pilot signals and grids of pilots24 in one plan (1.0746e11 cells).
   stream64 64 independent 24-antenna observations (2-bit, 10 ms) resident in HBM, each a random
            4-12 SV mix from all six systems at 35-50 dB-Hz plus 3 CW tones per antenna at
            -10..+10 dB; step k searches observation k mod 64 (all systems, the grids of all24),
@@ -26,7 +28,10 @@ Other workloads (same JSON line):
 A step = one tga_search_device call on an observation already resident in HBM: decode +
 de-mean, forward spectra, all correlation rows, selection, copy of the peaks to the host, and
 the f64 ACR lookups.  After the warm-up every step replays the plan's search as one hipGraph
-whose timed stages carry event-record nodes (the kernel times of the roofline).  Metric: search cells/s = sum(n_ant * n_prn * n_bin * P) / t.
+whose timed stages carry event-record nodes (the kernel times of the roofline).  Metric: search
+cells/s = sum(n_ant * n_prn * n_bin * P) / t.  The roofline block covers the rows stage of the
+search (every correlation-row kernel of every group, HIP events around the stage on the plan's
+stream); its algorithmic bytes are 2 M S_c per row (M = P folded, S_c = 8 B).
 
 Prints one JSON line (see the repository's measurement contract).
 """
@@ -178,15 +183,13 @@ class DeviceInput:
 
 
 def load_traffic(path, kernel, workload_name):
-    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
-    from the committed summary; None if absent or measured on another workload (the bytes of a
-    launch depend on its rows)."""
+    """HBM bytes per launch (per search for a rows stage of several kernels) of `kernel`
+    measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE) on this workload, from the committed
+    summary; None if absent (the bytes of a launch depend on its rows)."""
     try:
         with open(path) as f:
             d = json.load(f)
-        if d.get("workload") != workload_name:
-            return None
-        return float(d["kernels"][kernel]["hbm_bytes_per_launch"])
+        return float(d["workloads"][workload_name]["kernels"][kernel]["hbm_bytes_per_launch"])
     except (OSError, KeyError, ValueError):
         return None
 
@@ -194,7 +197,7 @@ def load_traffic(path, kernel, workload_name):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="ca24",
+    ap.add_argument("--workload", default="all24",
                     choices=["ca24", "ca1", "ca1nc", "pilots24", "all24", "stream64"])
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -358,22 +361,23 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
+        # each system of the workload on antenna 0 for an equal share of the budget (whole
+        # PRNs, at least one), then the workload's own mix projected from the per-system rates:
+        # value = total cells / sum over systems of (system cells / system rate)
         from oracle import cpu_port
         workers = min(16, os.cpu_count() or 1)
-        levels = synth.unpack_levels(packed, L, fmt)
-        t_cells = t_s = 0.0
-        n_done = n_ants = 0
-        for a in range(n_ant):  # whole antennas, then whole systems, until the budget is spent
-            for s, prns, fc in w["systems"]:
-                c, t, d = cpu_port.time_sample(levels[a], FS, s, prns, fc, workers,
-                                               budget_s=max(args.cpu_budget - t_s, 0.0),
-                                               epochs=nc)
-                t_cells += c
-                t_s += t
-                n_done += d
-            n_ants += 1
-            if t_s >= args.cpu_budget:
-                break
+        levels = synth.unpack_levels(packed[:n_ant], L, fmt)
+        share = args.cpu_budget / len(w["systems"])
+        per_sys, t_all, cells_all, t_proj = [], 0.0, 0.0, 0.0
+        for s, prns, fc in w["systems"]:
+            c, t, d = cpu_port.time_sample(levels[0], FS, s, prns, fc, workers, budget_s=share,
+                                           epochs=nc)
+            rate = c / t
+            full = n_ant * len(prns) * len(fc) * int(FS * PERIOD_S[s])
+            per_sys.append(f"{s} {d} PRN {t:.1f} s {rate:.3g} cells/s")
+            t_all += t
+            cells_all += full
+            t_proj += full / rate
         # the same port on one thread, over the first system of antenna 0, for a few seconds
         c1, t1, d1 = cpu_port.time_sample(levels[0], FS, w["systems"][0][0], w["systems"][0][1],
                                           w["systems"][0][2], 1, budget_s=args.cpu_budget / 4,
@@ -385,13 +389,15 @@ def main():
                               if ln.startswith("model name")), "")
         except OSError:
             pass
-        cpu = {"value": t_cells / t_s, "unit": "cells/s", "cores": workers, "kind": "port",
-               "sample": (f"{n_done} (antenna, PRN) searches over {n_ants} antenna(s), "
+        cpu = {"value": cells_all / t_proj, "unit": "cells/s", "cores": workers, "kind": "port",
+               "sample": ("antenna 0, every system for an equal share of "
+                          f"{args.cpu_budget:.0f} s ({'; '.join(per_sys)}), "
                           + (f"{nc} one-period epochs per bin (1 replica FFT per PRN, "
                              f"{nc} forward + {nc} inverse FFTs per bin)" if nc else
                              "N=2P, reference work structure (2 replica + 3 FFTs per bin per "
                              "PRN)")
-                          + f", complex64 pocketfft over {workers} threads, {t_s:.1f} s"),
+                          + f", complex64 pocketfft over {workers} threads, {t_all:.1f} s; value "
+                          "= the workload's cells over the time the per-system rates project"),
                "value_1_core": c1 / t1 if t1 > 0 else None,
                "sample_1_core": f"{d1} PRN searches of {w['systems'][0][0]} on antenna 0, "
                                 f"{t1:.1f} s",

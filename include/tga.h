@@ -224,7 +224,9 @@ int tga_plan_set_graph(tga_plan* plan, int32_t enable);
 int tga_plan_set_timing(tga_plan* plan, int32_t enable);
 int tga_plan_get_timing(tga_plan* plan, int32_t kernel, double* total_ms, int64_t* launches,
                         double* bytes_per_launch);
-/* Short name of the kernel(s) behind a timing slot, as rocprof lists them. */
+/* Short names of the kernels behind a timing slot, as rocprof lists them, joined by '+' over
+ * the plan's execution groups (e.g. "k_rows+k_big_pre<4>+k_rows<0,4>" for a C/A group beside
+ * an E1-C split group); "" for a slot out of range. */
 const char* tga_plan_kernel_name(const tga_plan* plan, int32_t kernel);
 /* Describes the plan's execution path for each system ("fused16368", "generic", ...). */
 int tga_plan_describe(const tga_plan* plan, char* buf, size_t cap);
@@ -265,6 +267,17 @@ int tga_rfi(const uint8_t* packed, size_t n_ant, size_t row_bytes, int64_t recor
  * bytes of device memory), e.g. the buffer a tga_search_device call searches. */
 int tga_rfi_device(const void* d_packed, size_t n_ant, size_t row_bytes, int64_t record_samples,
                    int32_t input_format, double sampling_freq, tga_rfi_channel* out);
+
+/* The --rfi report as JSON (serde_json::to_string_pretty of RfiReport, rfi.rs:73-77, written
+ * at main.rs:468): {"antenna_numbers": [...], "channels": [{antenna, dead, acf_lag1,
+ * acf_lag3, acf_lag16, period_samples, lines: [{frequency, excess_db}]}]}, None (NaN in the
+ * struct) as null.  tga_rfi_report_json serialises channels a tga_rfi / tga_rfi_device call
+ * returned (host only, no GPU); tga_rfi_json runs tga_rfi on a packed observation first
+ * (replaces rfi::run + the serialisation, main.rs:439-478).  The string is library-owned:
+ * release it with tga_free_string. */
+int tga_rfi_report_json(const tga_rfi_channel* channels, size_t n_ant, char** json);
+int tga_rfi_json(const uint8_t* packed, size_t n_ant, size_t row_bytes, int64_t record_samples,
+                 int32_t input_format, double sampling_freq, char** json);
 
 /* ---- host orchestration and JSON (acquire_all_* + lib.rs CombinedOutput) ---------------- */
 

@@ -13,10 +13,16 @@ Pinning (what independently checks each function here):
   correlate  -- brute-force O(N^2) circular correlation, tie rules on constructed inputs,
                 the reference's delay-recovery identity, the two-period folding identity.
   acr        -- the reference's ACR known answers (r=4.1725 -> 45, 11.4497 -> 50 dB-Hz),
-                table round-trips, saturation and cut-off behaviour.
-  rfi        -- direct DFT on small inputs, the reference's square-wave / white-noise /
+                exact table points, saturation, cut-off, monotone interpolation, and the
+                log-space rule (r at the geometric mean of two brackets -> the arithmetic
+                mean of their C/N0).
+  rfi        -- closed forms: a tone on an FFT bin (|X_k|^2 = (A n / 2)^2 against a direct
+                DFT), the floor index sorted[half/2], the >/>= local-maximum rule on
+                plateaus, the strict 10 kHz clustering, the three-line cap and order, the
+                (n - lag) ACF normalisation on periodic sequences, the population-variance
+                dead-channel threshold; the reference's square-wave / white-noise /
                 dead-channel tests.
-  stats      -- the reference's median/MAD known answers.
+  stats      -- the reference's median/MAD known answers (stats.rs:32-61).
   jsonfmt    -- shortest round-trip float formatting checked against float() round trip
                 and hand-written serde_json/ryu expectations.
   antenna_test -- the eleven known-answer cases of antenna_test.rs's unit tests

@@ -20,13 +20,22 @@ def power_spectrum(z):
 
 
 def detect_lines(z, fs):
-    """rfi.rs:91-138."""
+    """rfi.rs:91-138: the line rules applied to |FFT_f64(z)|^2."""
     n = len(z)
     if n < 8:
         return []
+    return lines_from_powers(power_spectrum(z), n, fs)
+
+
+def lines_from_powers(p, n, fs):
+    """The tail of detect_lines (rfi.rs:105-137) on the powers p[k], k < n/2, of an n-point
+    record: floor = sorted[half/2]; local maxima p[k] > p[k-1] and p[k] >= p[k+1] for
+    1 <= k < half-1 more than 15 dB above the floor; strongest first (stable); a line is kept
+    only if it lies more than 10 kHz from every kept line; at most 3."""
     half = n // 2
     bin_hz = fs / n
-    p = power_spectrum(z)
+    p = np.asarray(p, dtype=np.float64)
+    assert len(p) == half
     floor = np.sort(p)[half // 2]
     lines = []
     k = np.arange(1, half - 1)
@@ -60,3 +69,15 @@ def analyze_channel(antenna, bipolar, fs):
             "acf_lag16": autocorrelation(z, 16),
             "period_samples": (fs / lines[0]["frequency"]) if lines else None,
             "lines": lines}
+
+
+def report(channels):
+    """RfiReport (rfi.rs:73-77, built by rfi::run at rfi.rs:186-194): antenna indices in order
+    and the channels, as dicts in the struct's field order."""
+    return {"antenna_numbers": [c["antenna"] for c in channels], "channels": list(channels)}
+
+
+def report_json(channels):
+    """serde_json::to_string_pretty of the report (main.rs:468)."""
+    from .jsonfmt import dumps_pretty
+    return dumps_pretty(report(channels))

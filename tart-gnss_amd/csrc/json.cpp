@@ -250,6 +250,51 @@ std::string combined_json(const std::vector<SystemOut>& outs, const int64_t* ant
   return w.out;
 }
 
+// RfiReport (rfi.rs:73-77) as serde_json::to_string_pretty writes it (main.rs:468): every
+// field of RfiChannel present in struct order (rfi.rs:54-70), Option::None as null (NaN in
+// tga_rfi_channel), lines as {frequency, excess_db} objects, antenna_numbers the channels'
+// antenna indices in order (rfi.rs:188-191).
+std::string rfi_report_json(const tga_rfi_channel* ch, size_t n) {
+  Writer w;
+  auto opt = [](double v) { return std::isnan(v) ? std::string("null") : json_f64(v); };
+  w.raw("{");
+  w.key("antenna_numbers", 1, true);
+  w.array(n, 1, [&](size_t i) { w.raw(std::to_string(ch[i].antenna)); });
+  w.key("channels", 1, false);
+  w.array(n, 1, [&](size_t i) {
+    const tga_rfi_channel& c = ch[i];
+    const int d = 3;
+    w.raw("{");
+    w.key("antenna", d, true);
+    w.raw(std::to_string(c.antenna));
+    w.key("dead", d, false);
+    w.raw(c.dead ? "true" : "false");
+    w.key("acf_lag1", d, false);
+    w.raw(opt(c.acf_lag1));
+    w.key("acf_lag3", d, false);
+    w.raw(opt(c.acf_lag3));
+    w.key("acf_lag16", d, false);
+    w.raw(opt(c.acf_lag16));
+    w.key("period_samples", d, false);
+    w.raw(opt(c.period_samples));
+    w.key("lines", d, false);
+    w.array(static_cast<size_t>(c.n_lines), d, [&](size_t q) {
+      w.raw("{");
+      w.key("frequency", d + 2, true);
+      w.raw(json_f64(c.lines[q].frequency));
+      w.key("excess_db", d + 2, false);
+      w.raw(json_f64(c.lines[q].excess_db));
+      w.nl(d + 1);
+      w.raw("}");
+    });
+    w.nl(d - 1);
+    w.raw("}");
+  });
+  w.nl(0);
+  w.raw("}");
+  return w.out;
+}
+
 std::string acquire_all_json(Plan& p, const tga_peak* peaks, const double* center,
                              const int64_t* antenna_numbers, bool cn0, double filt_phase,
                              double filt_freq) {
@@ -476,6 +521,31 @@ int tga_acquire_antenna_report_json(tga_plan* plan, const uint8_t* packed, size_
                                                filter_freq_mad, min_cn0, json);
         rc != TGA_OK)
       fail(rc, tga_last_error());
+  });
+}
+
+int tga_rfi_report_json(const tga_rfi_channel* channels, size_t n_ant, char** json) {
+  return plan_guard([&] {
+    if (!json || (n_ant && !channels)) fail(TGA_ERR_INVALID_ARGUMENT, "null argument");
+    *json = nullptr;
+    for (size_t i = 0; i < n_ant; ++i)
+      if (channels[i].n_lines < 0 || channels[i].n_lines > 3)
+        fail(TGA_ERR_INVALID_ARGUMENT, "n_lines must be in 0..3");
+    copy_out(rfi_report_json(channels, n_ant), json);
+  });
+}
+
+int tga_rfi_json(const uint8_t* packed, size_t n_ant, size_t row_bytes, int64_t record_samples,
+                 int32_t input_format, double sampling_freq, char** json) {
+  return plan_guard([&] {
+    if (!json) fail(TGA_ERR_INVALID_ARGUMENT, "null argument");
+    *json = nullptr;
+    std::vector<tga_rfi_channel> ch(n_ant);
+    if (int rc = tga_rfi(packed, n_ant, row_bytes, record_samples, input_format,
+                         sampling_freq, ch.data());
+        rc != TGA_OK)
+      fail(rc, tga_last_error());
+    copy_out(rfi_report_json(ch.data(), n_ant), json);
   });
 }
 

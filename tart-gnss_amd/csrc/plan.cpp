@@ -38,6 +38,24 @@ std::vector<float2> tau_table(size_t P) {
   return t;
 }
 
+// Kernels of one group behind a timing slot (0 rows, 1 forward spectra, 2 selection).
+std::string group_kernels(const Group& g, int slot) {
+  const std::string r0 = std::to_string(g.big_r0);
+  switch (slot) {
+    case 0:
+      return g.nc > 0  ? "k_rows<NC>"
+             : g.fp16  ? "k_rows16"
+             : g.fused ? "k_rows"
+             : g.big16 ? "k_big_pre<" + r0 + ",f16>+k_rows16<" + r0 + ">"
+             : g.big   ? "k_big_pre<" + r0 + ">+k_rows<0," + r0 + ">"
+                       : "k_mul_rows+k_pass+k_row_reduce";
+    case 1:
+      return g.fused ? "k_forward" : g.folded ? "k_fold_obs+k_pass" : "k_wipe_obs+k_pass";
+    default:
+      return g.big ? "k_select+k_second_win" : "k_select";
+  }
+}
+
 }  // namespace
 
 Plan::~Plan() {
@@ -45,10 +63,7 @@ Plan::~Plan() {
     if (t.start) (void)hipEventDestroy(t.start);
     if (t.stop) (void)hipEventDestroy(t.stop);
   }
-  for (hipEvent_t e : {ev_fwd, ev_pre[0], ev_pre[1], ev_rows[0], ev_rows[1]})
-    if (e) (void)hipEventDestroy(e);
   drop_graph();
-  if (side) (void)hipStreamDestroy(side);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -291,20 +306,17 @@ void Plan::build(const tga_plan_desc& d) {
       g.d_Xs.alloc(g.n_xrow * M * sizeof(float2));
       upload(g.d_tw, fused_twiddles(), stream);
       upload(g.d_twb, big_twiddles(g.P), stream);
-      // sub-rows of a chunk of rows per pre-pass / transform pair, two chunks in flight;
-      // TGA_SPLIT_CHUNK_MB sets the bytes of one chunk's sub-rows (default 2048)
+      // sub-rows of a chunk of rows per pre-pass / transform pair, on the plan's stream (a
+      // second stream for the pre-pass of the next chunk measured no faster: the two kernels
+      // then slow each other down); TGA_SPLIT_CHUNK_MB sets the bytes of one chunk's sub-rows
+      // (default 2048)
       const size_t y_elem = g.big16 ? sizeof(uint32_t) : sizeof(float2);
       const char* env_mb = std::getenv("TGA_SPLIT_CHUNK_MB");
       const long long chunk_bytes = (env_mb ? std::max(1, std::atoi(env_mb)) : 2048) * (1LL << 20);
       const long long cap = std::max<long long>(
           1, chunk_bytes / (static_cast<long long>(M) * static_cast<long long>(y_elem)));
       g.chunk_rows = std::min<long long>(cap, g.n_rows);
-      for (auto& y : g.d_Y) y.alloc(static_cast<size_t>(g.chunk_rows) * M * y_elem);
-      if (!side) {
-        TGA_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
-        for (hipEvent_t* e : {&ev_fwd, &ev_pre[0], &ev_pre[1], &ev_rows[0], &ev_rows[1]})
-          TGA_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
-      }
+      g.d_Y.alloc(static_cast<size_t>(g.chunk_rows) * M * y_elem);
       g.d_rowkey.alloc(static_cast<size_t>(g.n_rows) * sizeof(unsigned long long));
       g.d_rownan.alloc(static_cast<size_t>(g.n_rows) * sizeof(int));
       const size_t nw = static_cast<size_t>(n_ant) * n_prn;
@@ -332,6 +344,15 @@ void Plan::build(const tga_plan_desc& d) {
   }
   for (auto& g : groups)
     if (fft_plan_bluestein(fft_plan(g.M))) graphable = false;
+  // the kernels behind each timing slot, as rocprof lists them, joined over the groups
+  for (int slot = 0; slot < 3; ++slot) {
+    std::string names;
+    for (const auto& g : groups) {
+      const std::string n = group_kernels(g, slot);
+      if (names.find(n) == std::string::npos) names += (names.empty() ? "" : "+") + n;
+    }
+    kernel_names[slot] = names;
+  }
   TGA_HIP(hipStreamSynchronize(stream));
   for (auto& t : timers) {
     TGA_HIP(hipEventCreate(&t.start));
@@ -515,25 +536,16 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
       launch_fused_rows(g.X, g.d_rep.as<float2>(), g.d_tw.as<float2>(), n_ant, g.n_bin, g.n_prn,
                         g.cw, g.d_rows.as<RowOut>(), st, g.rows_cfg);
     } else if (g.big) {
-      // the pre-pass of chunk c + 1 (side stream) overlaps the transforms of chunk c
-      TGA_HIP(hipEventRecord(ev_fwd, st));
-      TGA_HIP(hipStreamWaitEvent(side, ev_fwd, 0));
-      int c = 0;
       const int hs = g.folded ? 1 : 2;
       const long long n_enum = g.n_rows / hs;
-      for (long long r0 = 0; r0 < n_enum; r0 += g.chunk_rows, ++c) {
+      for (long long r0 = 0; r0 < n_enum; r0 += g.chunk_rows) {
         const long long r1 = std::min(n_enum, r0 + g.chunk_rows);
-        const int b = c & 1;
-        if (c >= 2) TGA_HIP(hipStreamWaitEvent(side, ev_rows[b], 0));
         launch_big_pre(g.X, g.d_rep.as<float2>(), g.d_twb.as<float2>(), g.big_r0, g.n_prn,
-                       g.n_bin, n_ant, hs, r0, r1, g.d_Y[b].p, g.big16,
-                       g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(), side);
-        TGA_HIP(hipEventRecord(ev_pre[b], side));
-        TGA_HIP(hipStreamWaitEvent(st, ev_pre[b], 0));
+                       g.n_bin, n_ant, hs, r0, r1, g.d_Y.p, g.big16,
+                       g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(), st);
         launch_big_sub(g.d_tw.as<float2>(), g.big_r0, g.n_prn, g.n_bin, n_ant, hs, r0, r1,
-                       g.d_Y[b].p, g.big16, g.d_rowkey.as<unsigned long long>(),
+                       g.d_Y.p, g.big16, g.d_rowkey.as<unsigned long long>(),
                        g.d_rownan.as<int>(), g.d_rows.as<RowOut>(), st);
-        TGA_HIP(hipEventRecord(ev_rows[b], st));
       }
     } else {
       const FftPlan* fp = fft_plan(g.M);
@@ -730,22 +742,8 @@ int tga_plan_get_timing(tga_plan* plan, int32_t kernel, double* total_ms, int64_
 
 const char* tga_plan_kernel_name(const tga_plan* plan, int32_t kernel) {
   const Plan* p = reinterpret_cast<const Plan*>(plan);
-  const bool fused = p && !p->groups.empty() && p->groups[0].fused;
-  const bool fp16 = fused && p->groups[0].fp16;
-  const bool big = p && !p->groups.empty() && p->groups[0].big;
-  const bool big16 = big && p->groups[0].big16;
-  const bool nc = fused && p->groups[0].nc > 0;
-  switch (kernel) {
-    case 0: return fp16    ? "k_rows16"
-                   : nc    ? "k_rows<NC>"
-                   : fused ? "k_rows"
-                   : big16 ? "k_big_pre+k_rows16<R0>"
-                   : big   ? "k_big_pre+k_rows<0,R0>"
-                           : "k_mul_rows+k_pass+k_row_reduce";
-    case 1: return fused ? "k_forward" : "k_fold_obs+k_pass";
-    case 2: return "k_select";
-  }
-  return "";
+  if (!p || kernel < 0 || kernel > 2) return "";
+  return p->kernel_names[kernel].c_str();
 }
 
 int tga_plan_describe(const tga_plan* plan, char* buf, size_t cap) {

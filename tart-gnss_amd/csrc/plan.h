@@ -31,7 +31,7 @@ struct Group {
   DevBuf d_fc, d_slots, d_tau, d_rep, d_X, d_Xs, d_rows, d_Z, d_Zs, d_tw;
   DevBuf d_carrier;  // fused groups: carrier (cos, sin) per (bin, sample), built at plan creation
   DevBuf d_rep16, d_X16;  // FP16 form: f16 (re, im) pairs scaled by kF16Scale
-  DevBuf d_Y[2], d_rowkey, d_rownan, d_twb, d_win, d_Zw, d_Zws;  // long-period split
+  DevBuf d_Y, d_rowkey, d_rownan, d_twb, d_win, d_Zw, d_Zws;  // long-period split
   float2* X = nullptr;  // where the forward spectra of the last search landed
   FusedRowsConfig rows_cfg;  // rows-kernel shape picked by the autotuner
   std::vector<std::pair<FusedRowsConfig, double>> tune;  // candidates timed, ms per launch
@@ -63,11 +63,8 @@ struct Plan {
   std::vector<Group> groups;
   DevBuf d_means, d_partial, d_input, d_peaks, d_nan;
   hipStream_t stream = nullptr;
-  // long-period split: the sub-row pre-pass of chunk c + 1 runs on `side` while the rows
-  // kernel transforms chunk c; ev_pre / ev_rows hand the two scratch buffers over
-  hipStream_t side = nullptr;
-  hipEvent_t ev_fwd = nullptr, ev_pre[2] = {nullptr, nullptr}, ev_rows[2] = {nullptr, nullptr};
   bool timing = false;
+  std::array<std::string, 3> kernel_names;  // kernels behind each timing slot
   std::array<Timer, 3> timers;
   // hipGraph replay: the first search of a plan runs eagerly, the second captures the whole
   // search (reading d_input) into `gexec`, later ones replay it.  With timing on, the timer

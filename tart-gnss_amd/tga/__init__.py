@@ -17,6 +17,8 @@ no CPU fallback: if libtga.so or a gfx950 device is missing, calls raise.
   Plan.acquire_all_json / Plan.peaks_json / format_f64
                        CombinedOutput JSON (lib.rs:31-45, main.rs:242-255, 893)
   rfi                  rfi.rs:142-184
+  rfi_json / rfi_report_json
+                       --rfi JSON report (RfiReport, rfi.rs:73-77, main.rs:439-478)
   antenna_test_json / Plan.test_antennas_json
                        --test-antennas report (antenna_test.rs:105-222, main.rs:848-880)
   read_observation     Observation::from_hdf5 (observation.rs:48-92), host-side HDF5 reader
@@ -28,6 +30,7 @@ from .api import (AcquisitionResult, CorrelationPeak, Plan, SystemSearch,  # noq
                   b1c_code_resampled, code_chips, code_length, code_resampled, code_spectra,
                   correlate_code, device_count, e1c_code_resampled, estimate_cn0, format_f64,
                   gold_code,
-                  l1c_code_resampled, qzss_gold_code, rfi, rfi_device, sbas_gold_code,
+                  l1c_code_resampled, qzss_gold_code, rfi, rfi_device, rfi_json,
+                  rfi_report_json, sbas_gold_code,
                   search_frequencies)
 from .observation import Observation, read_observation  # noqa: F401,E402

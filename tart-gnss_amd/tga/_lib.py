@@ -117,6 +117,10 @@ PROTOTYPES = {
                                                   _dp, C.POINTER(C.c_int64), C.c_double,
                                                   C.c_double, C.c_double,
                                                   C.POINTER(C.c_void_p)]),
+    "tga_rfi_report_json": (C.c_int, [C.POINTER(RfiChannel), C.c_size_t,
+                                      C.POINTER(C.c_void_p)]),
+    "tga_rfi_json": (C.c_int, [_u8p, C.c_size_t, C.c_size_t, C.c_int64, C.c_int32, C.c_double,
+                               C.POINTER(C.c_void_p)]),
     "tga_free_string": (None, [C.c_void_p]),
     "tga_format_f64": (C.c_int, [C.c_double, C.c_char_p, C.c_size_t]),
 }

@@ -476,3 +476,43 @@ def rfi_device(dev_ptr, n_ant, row_bytes, record_samples, input_format, sampling
                                int(record_samples), int(input_format), float(sampling_freq),
                                out))
     return _rfi_channels(out)
+
+
+def _rfi_structs(channels):
+    out = (L.RfiChannel * len(channels))()
+    for c, d in zip(out, channels):
+        def val(x):
+            return float("nan") if x is None else float(x)
+        c.antenna = int(d["antenna"])
+        c.dead = 1 if d["dead"] else 0
+        c.acf_lag1, c.acf_lag3, c.acf_lag16 = (val(d["acf_lag1"]), val(d["acf_lag3"]),
+                                               val(d["acf_lag16"]))
+        c.period_samples = val(d["period_samples"])
+        if len(d["lines"]) > 3:
+            raise ValueError("at most 3 lines per channel")
+        c.n_lines = len(d["lines"])
+        for i, ln in enumerate(d["lines"]):
+            c.lines[i].frequency = float(ln["frequency"])
+            c.lines[i].excess_db = float(ln["excess_db"])
+    return out
+
+
+def rfi_report_json(channels):
+    """The --rfi JSON (serde_json::to_string_pretty of RfiReport, main.rs:468) of channels as
+    rfi() returns them; host only."""
+    st = _rfi_structs(channels)
+    out = C.c_void_p()
+    check(lib().tga_rfi_report_json(st, len(channels), C.byref(out)))
+    return _take_string(out)
+
+
+def rfi_json(packed, record_samples, input_format, sampling_freq):
+    """rfi() and its JSON report in one call (rfi::run + main.rs:439-478)."""
+    pk = np.ascontiguousarray(packed, dtype=np.uint8)
+    if pk.ndim != 2:
+        raise ValueError("packed must be [n_ant, row_bytes]")
+    out = C.c_void_p()
+    check(lib().tga_rfi_json(_ptr(pk, C.c_uint8), pk.shape[0], pk.shape[1],
+                             int(record_samples), int(input_format), float(sampling_freq),
+                             C.byref(out)))
+    return _take_string(out)

@@ -76,7 +76,8 @@ def test_fp16_split_matches_fp64_truth(gpu, case):
     res, packed = plans(L, n_ant, fmt, signals, systems, seed)
     plan16, p16 = res["fp16"]
     assert path in plan16.describe()
-    assert plan16.kernel_name(0) == "k_big_pre+k_rows16<R0>"
+    r0 = P // 16368
+    assert plan16.kernel_name(0) == f"k_big_pre<{r0},f16>+k_rows16<{r0}>"
     levels = synth.unpack_levels(packed, L, fmt)
     N = min(2 * P, L)
     worst = 0.0
