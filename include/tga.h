@@ -122,6 +122,25 @@ int tga_correlate_code(const float* signal, const double* phasepoints,
                        const double* fc, size_t n_fc, size_t num_samples,
                        size_t code_period_samples, double sampling_freq, tga_peak* out);
 
+/* tga_correlate_code without any device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory -- code under hipGraph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  Same semantics and kernels as tga_correlate_code,
+ * but every argument is DEVICE memory the caller owns (d_code_complex / d_code_complex_alt:
+ * num_samples complex (re, im) float pairs, already conjugated, as code_spectra returns them),
+ * the peak is written to d_out (device) and d_nan (device, may be NULL) receives 1 where
+ * tga_correlate_code would return TGA_ERR_NAN, else 0.  The call only enqueues work on
+ * `stream` (a hipStream_t, NULL for the null stream) and returns; it never synchronises.
+ * Working memory is a fixed 22 MiB device array inside the library, which each GPU the library
+ * runs on holds until the process ends; calls that overlap on one GPU share it and race (one
+ * call at a time per GPU).  Limits, else TGA_ERR_UNSUPPORTED: 2 <= num_samples <= 327360 with
+ * prime factors in {2, 3, 5, 7, 11, 13, 31} (every 16.368 MHz and 4.092 MHz reference length),
+ * n_fc <= 4096. */
+int tga_correlate_code_noalloc(const float* d_signal, const double* d_phasepoints,
+                               const float* d_code_complex, const float* d_code_complex_alt,
+                               const double* d_fc, size_t n_fc, size_t num_samples,
+                               size_t code_period_samples, double sampling_freq, tga_peak* d_out,
+                               int32_t* d_nan, void* stream);
+
 /* ---- level 2 --------------------------------------------------------------------------- */
 
 /* One PRN on one antenna (acquire_full and siblings): 300 Hz grid

@@ -132,14 +132,28 @@ unsigned grid_for(long long total) {
   return static_cast<unsigned>((total + kBlock - 1) / kBlock);
 }
 
-// Single precision pass order: odd radices first, then 16s and one 8/4/2 remainder.
+// Single precision pass order: odd radices first (ascending), then 16s and one 8/4/2
+// remainder; written to radix[0..cap), returns the pass count (-1 if more than cap).
+int factorize_into(size_t n, int* radix, int cap) {
+  int k = 0;
+  auto push = [&](size_t r) {
+    if (k < cap) radix[k] = static_cast<int>(r);
+    ++k;
+  };
+  size_t m = strip_pow2(n);
+  for (size_t p = 3; p * p <= m; p += 2)
+    while (m % p == 0) { push(p); m /= p; }
+  if (m > 1) push(m);
+  size_t q = n / strip_pow2(n);
+  while (q % 16 == 0) { push(16); q /= 16; }
+  if (q > 1) push(q);  // 8, 4 or 2
+  return k <= cap ? k : -1;
+}
+
 std::vector<int> factorize(size_t n) {
-  std::vector<int> out;
-  push_odd_primes(strip_pow2(n), out);
-  size_t m = n / strip_pow2(n);
-  while (m % 16 == 0) { out.push_back(16); m /= 16; }
-  if (m > 1) out.push_back(static_cast<int>(m));  // 8, 4 or 2
-  return out;
+  int buf[64];
+  const int k = factorize_into(n, buf, 64);  // at most log2(n) passes
+  return std::vector<int>(buf, buf + k);
 }
 
 // Double precision pass order: radix 4 (butterfly path), one 2, then odd primes (direct).
@@ -386,6 +400,79 @@ const FftPlan* fft_plan(size_t n) {
 size_t fft_plan_size(const FftPlan* plan) { return plan->n; }
 
 bool fft_plan_bluestein(const FftPlan* plan) { return plan->m > 0; }
+
+// ---- heap-free form: codelet radices only, twiddle table supplied by the caller ------------
+namespace {
+
+// The table of init_plan on the GPU: tw[m] = W_n^m for m < split, then W_n^{split m} for
+// m < n_hi, each from its exactly reduced exponent in f64 and rounded to f32 once.
+__global__ __launch_bounds__(kBlock) void k_fft_twiddles(long long n, int split, int n_hi,
+                                                         float2* __restrict__ tw) {
+  const int i = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
+  if (i >= split + n_hi) return;
+  const unsigned long long m = i < split ? i : i - split;
+  const unsigned long long step = i < split ? 1ull : static_cast<unsigned long long>(split);
+  const unsigned long long e = (m * step) % static_cast<unsigned long long>(n);
+  const double a = -2.0 * kPiD * static_cast<double>(e) / static_cast<double>(n);
+  tw[i] = make_float2(static_cast<float>(cos(a)), static_cast<float>(sin(a)));
+}
+
+}  // namespace
+
+int fft_codelet_passes(size_t n, int* radix, int cap) {
+  if (n < 2 || n > (1u << 30)) return -1;
+  const int k = factorize_into(n, radix, cap);
+  if (k < 0) return -1;
+  for (int i = 0; i < k; ++i)
+    if (!has_codelet(radix[i])) return -1;
+  return k;
+}
+
+int fft_twiddle_count(size_t n) {
+  const int split = std::max(1, static_cast<int>(std::ceil(std::sqrt(static_cast<double>(n)))));
+  return split + static_cast<int>((n + split - 1) / split + 1);
+}
+
+void launch_fft_twiddles(size_t n, float2* tw, hipStream_t st) {
+  const int split = std::max(1, static_cast<int>(std::ceil(std::sqrt(static_cast<double>(n)))));
+  const int n_hi = static_cast<int>((n + split - 1) / split + 1);
+  hipLaunchKernelGGL(k_fft_twiddles, dim3(grid_for(split + n_hi)), dim3(kBlock), 0, st,
+                     static_cast<long long>(n), split, n_hi, tw);
+  TGA_HIP(hipGetLastError());
+}
+
+float2* fft_forward_tables(size_t n_, const int* radix, int n_radix, const float2* tw,
+                           float2* data, float2* scratch, size_t batch, hipStream_t st) {
+  const int n = static_cast<int>(n_);
+  if (n == 1 || batch == 0) return data;
+  const int split = std::max(1, static_cast<int>(std::ceil(std::sqrt(static_cast<double>(n_)))));
+  const float2* tw1 = tw;
+  const float2* tw2 = tw + split;
+  float2* src = data;
+  float2* dst = scratch;
+  int ns = 1;
+  for (int pi = 0; pi < n_radix; ++pi) {
+    const int R = radix[pi];
+    const long long nbt = static_cast<long long>(n / R) * static_cast<long long>(batch);
+    switch (R) {
+      case 2: launch_pass<2>(src, dst, n, ns, nbt, tw1, tw2, split, st); break;
+      case 3: launch_pass<3>(src, dst, n, ns, nbt, tw1, tw2, split, st); break;
+      case 4: launch_pass<4>(src, dst, n, ns, nbt, tw1, tw2, split, st); break;
+      case 5: launch_pass<5>(src, dst, n, ns, nbt, tw1, tw2, split, st); break;
+      case 7: launch_pass<7>(src, dst, n, ns, nbt, tw1, tw2, split, st); break;
+      case 8: launch_pass<8>(src, dst, n, ns, nbt, tw1, tw2, split, st); break;
+      case 11: launch_pass<11>(src, dst, n, ns, nbt, tw1, tw2, split, st); break;
+      case 13: launch_pass<13>(src, dst, n, ns, nbt, tw1, tw2, split, st); break;
+      case 16: launch_pass<16>(src, dst, n, ns, nbt, tw1, tw2, split, st); break;
+      case 31: launch_pass<31>(src, dst, n, ns, nbt, tw1, tw2, split, st); break;
+      default: fail(TGA_ERR_UNSUPPORTED, "radix without a register codelet");
+    }
+    TGA_HIP(hipGetLastError());
+    ns *= R;
+    std::swap(src, dst);
+  }
+  return src;
+}
 
 TwiddleView fft_twiddles(const FftPlan* plan) {
   TwiddleView v;

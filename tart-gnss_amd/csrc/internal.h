@@ -84,6 +84,18 @@ struct TwiddleView {
 };
 TwiddleView fft_twiddles(const FftPlan* plan);
 
+// Heap-free form of fft_forward for lengths whose every factor has a register codelet (2, 3,
+// 4, 5, 7, 8, 11, 13, 16, 31): fft_codelet_passes writes the pass order of fft_plan(n) into
+// radix[0..cap) and returns the count (-1 for any other length); the twiddle table of
+// fft_twiddle_count(n) float2 entries is computed on the GPU by launch_fft_twiddles into
+// caller-provided device memory; fft_forward_tables runs the passes and returns the buffer
+// holding the result.  Results equal fft_forward's for the same length.
+int fft_codelet_passes(size_t n, int* radix, int cap);
+int fft_twiddle_count(size_t n);
+void launch_fft_twiddles(size_t n, float2* tw, hipStream_t st);
+float2* fft_forward_tables(size_t n, const int* radix, int n_radix, const float2* tw,
+                           float2* data, float2* scratch, size_t batch, hipStream_t st);
+
 // Double precision variant for the RFI spectrum (rfi.rs:96-99 uses an f64 FFT).
 struct FftPlanD;
 const FftPlanD* fft_plan_d(size_t n);

@@ -28,7 +28,8 @@ from .api import (AcquisitionResult, CorrelationPeak, Plan, SystemSearch,  # noq
                   acquire_beidou_single, antenna_test_json, acquire_full, acquire_galileo_single,
                   acquire_l1c_single, acquire_qzss, acquire_sbas, acquire_single,
                   b1c_code_resampled, code_chips, code_length, code_resampled, code_spectra,
-                  correlate_code, device_count, e1c_code_resampled, estimate_cn0, format_f64,
+                  correlate_code, correlate_code_noalloc, device_count, e1c_code_resampled,
+                  estimate_cn0, format_f64,
                   gold_code,
                   l1c_code_resampled, qzss_gold_code, rfi, rfi_device, rfi_json,
                   rfi_report_json, sbas_gold_code,

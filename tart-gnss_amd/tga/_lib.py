@@ -80,6 +80,9 @@ PROTOTYPES = {
     "tga_code_spectra": (C.c_int, [_dp, C.c_size_t, C.c_size_t, _fp, _fp]),
     "tga_correlate_code": (C.c_int, [_fp, _dp, _fp, _fp, _dp, C.c_size_t, C.c_size_t,
                                      C.c_size_t, C.c_double, C.POINTER(Peak)]),
+    "tga_correlate_code_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t,
+                                             C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),
     "tga_acquire_single": (C.c_int, [C.c_int32, _fp, _dp, C.c_size_t, C.c_double, C.c_double,
                                      C.c_double, C.c_int32, C.c_size_t, C.POINTER(AcqResult)]),
     "tga_plan_create": (C.c_int, [C.POINTER(PlanDesc), C.POINTER(C.c_void_p)]),

@@ -127,6 +127,21 @@ def correlate_code(signal_f32, phasepoints, code_complex, code_complex_alt, fc, 
     return CorrelationPeak(pk.best_peak, pk.second_peak, pk.best_freq_idx, pk.best_codephase)
 
 
+def correlate_code_noalloc(d_signal, d_phasepoints, d_code_complex, d_code_complex_alt, d_fc,
+                           n_fc, num_samples, code_period_samples, sampling_freq, d_out,
+                           d_nan=None, stream=None):
+    """correlate_code on device-resident inputs with no device allocation (integer device
+    addresses; the peak is written to d_out, a tga_peak in device memory, the NaN flag to d_nan).
+    Only enqueues work on `stream`; the caller synchronises."""
+    check(lib().tga_correlate_code_noalloc(
+        C.c_void_p(int(d_signal)), C.c_void_p(int(d_phasepoints)),
+        C.c_void_p(int(d_code_complex)), C.c_void_p(int(d_code_complex_alt)),
+        C.c_void_p(int(d_fc)), int(n_fc), int(num_samples), int(code_period_samples),
+        float(sampling_freq), C.c_void_p(int(d_out)),
+        C.c_void_p(int(d_nan)) if d_nan else None,
+        C.c_void_p(int(stream)) if stream else None))
+
+
 def search_frequencies(center_freq, search_band, freq_bin_size=300.0):
     """The bin grid of acquire_full and its siblings (acquisition.rs:189-198): round(2 band /
     300 Hz) + 1 bins spread evenly over [center - band, center + band]; a Plan's grid."""
