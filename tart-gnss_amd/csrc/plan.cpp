@@ -302,8 +302,12 @@ void Plan::build(const tga_plan_desc& d) {
       upload(g.d_tw, fused_twiddles(), stream);
       autotune(g);
     } else if (g.big) {
+      // long rows read their spectra segment by segment in the prime-factor order of 16368
+      rep_to_pfa(g, static_cast<long long>(n_prn) * 2 * static_cast<long long>(M));
       g.d_X.alloc(g.n_xrow * M * sizeof(float2));
       g.d_Xs.alloc(g.n_xrow * M * sizeof(float2));
+      const char* pin_sv = std::getenv("TGA_SPLIT_VARIANT");
+      g.rows_cfg.variant = g.big16 ? 0 : pin_sv ? (std::atoi(pin_sv) != 0 ? 1 : 0) : 1;
       upload(g.d_tw, fused_twiddles(), stream);
       upload(g.d_twb, big_twiddles(g.P), stream);
       // sub-rows of a chunk of rows per pre-pass / transform pair, on the plan's stream (a
@@ -518,6 +522,11 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
         launch_wipe_obs(d_in, row_bytes, fmt, d_means.as<double>(), g.d_fc.as<double>(), n_ant,
                         g.n_bin, M, phase_const, g.d_X.as<float2>(), g.big16, st);
       g.X = fft_forward(fft_plan(g.M), g.d_X.as<float2>(), g.d_Xs.as<float2>(), g.n_xrow, st);
+      if (g.big) {  // segment-wise prime-factor order of the long rows
+        float2* dst = g.X == g.d_X.as<float2>() ? g.d_Xs.as<float2>() : g.d_X.as<float2>();
+        launch_to_pfa(g.X, dst, static_cast<long long>(g.n_xrow) * M, st);
+        g.X = dst;
+      }
     }
   }
   time_end(1, st, fwd_bytes);
@@ -545,7 +554,7 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
                        g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(), st);
         launch_big_sub(g.d_tw.as<float2>(), g.big_r0, g.n_prn, g.n_bin, n_ant, hs, r0, r1,
                        g.d_Y.p, g.big16, g.d_rowkey.as<unsigned long long>(),
-                       g.d_rownan.as<int>(), g.d_rows.as<RowOut>(), st);
+                       g.d_rownan.as<int>(), g.d_rows.as<RowOut>(), st, g.rows_cfg.variant);
       }
     } else {
       const FftPlan* fp = fft_plan(g.M);

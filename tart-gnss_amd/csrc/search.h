@@ -130,10 +130,13 @@ void launch_select(const RowOut* rows, const PeakSlot* slots, int n_ant, int n_p
 // ---- long code periods P = R0 x 16368 (R0 = 4: E1-C, R0 = 10: B1C / L1C at 16.368 MHz) ------
 // Folded rows of length P are transformed as R0 twiddled sub-rows of 16368 points by the fused
 // LDS kernel; each row keeps only its argmax, and the second peak comes from a recomputation of
-// the winning row of every (antenna, PRN) pair.
+// the winning row of every (antenna, PRN) pair.  The forward and replica spectra of these rows
+// are stored segment by segment in the prime-factor order of 16368 (launch_to_pfa over the
+// whole buffer: position 16368 n0 + t holds natural index 16368 n0 + pfa_nat(t)).
 // folded rows (N = 2P) or unfolded single-period rows (N = P, a 10 ms B1C / L1C record)
 bool big_supported(size_t P, size_t N, bool folded);
-// W_P^i (i < 512) then W_P^{512 i} (i < ceil(P / 512)).
+// The pre-pass twiddles W_P^{pfa_nat(s)} for the positions s < 16368 of a segment in the
+// prime-factor order.
 std::vector<float2> big_twiddles(size_t P);
 // Rows [r0, r1) of the tiled enumeration (tiles of 8 forward spectra x 8 replica spectra, whose
 // rows reuse them from the caches); results land at the usual row index.  launch_big_pre writes the twiddled
@@ -147,9 +150,12 @@ std::vector<float2> big_twiddles(size_t P);
 void launch_big_pre(const float2* X, const float2* rep, const float2* twb, int R0, int n_prn,
                     int n_bin, int n_ant, int hs, long long r0, long long r1, void* Y, bool f16,
                     unsigned long long* rowkey, int* rownan, hipStream_t st);
+// variant: the radix-31 implementation of the FP32 sub-row transforms (0 VALU codelet, 1 matrix
+// cores with the next sub-row prefetched into registers); the FP16 form has one.
 void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, int hs,
                     long long r0, long long r1, const void* Y, bool f16,
-                    unsigned long long* rowkey, int* rownan, RowOut* rows, hipStream_t st);
+                    unsigned long long* rowkey, int* rownan, RowOut* rows, hipStream_t st,
+                    int variant);
 // second peaks of the winning rows; Z, Zs hold n_ant * n_prn * M complex values
 void launch_big_second(const float2* X, const float2* rep, const PeakSlot* slots, int M, int P,
                        int cw, int n_ant, int n_prn, int n_bin, bool folded, const int* win,

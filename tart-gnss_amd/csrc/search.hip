@@ -682,12 +682,10 @@ __device__ __forceinline__ void row_finish_key(const Top2& main_set, const Top2&
 //           global memory (buffer loads) into registers
 //   pass B  radix 16 (NS = 33) through LDS
 //   pass C  radix 31 (NS = 528); outputs stay in registers as |z|^2.
-// Full rows (SUB == 0) read X and R in the prime-factor order of 16368 = 33 x 16 x 31, for
-// which the passes need no twiddle factors and output r of pass-C butterfly j is sample
-// out_idx(j, r), the CRT number of (j mod 33, j / 33, r).  Sub-rows of long rows (SUB > 0)
-// arrive in natural order and take the Stockham twiddles: W_528^{rk} in pass B from a [r][k]
-// LDS table (lanes read consecutive entries: no bank conflicts) and W_P^{rj} in pass C.
-// Pass C has two implementations:
+// Rows (SUB == 0: conj(X) . R from the spectra) and sub-rows of long rows (SUB > 0: the
+// pre-pass output) are all stored in the prime-factor order of 16368 = 33 x 16 x 31, for which
+// the passes need no twiddle factors and output r of pass-C butterfly j is sample out_idx(j, r),
+// the CRT number of (j mod 33, j / 33, r).  Pass C has two implementations:
 //     VARIANT 0 (VALU): one butterfly per thread by the symmetric codelet (its twiddles in
 //           registers); the 16 butterflies beyond one per thread are split into
 //           256 (butterfly, output pair) units, one per thread of waves 0-3.
@@ -707,38 +705,21 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
   constexpr int TILES = NBC / 16;  // 33
   __shared__ float2 buf[P];
   __shared__ float2 stw[256];
-  __shared__ float2 tb[16 * 33];  // W_528^{r k}, [r][k]
-  __shared__ float2 tx[VARIANT == 0 ? 31 * LEFT + 31 : 1023];
+  __shared__ float2 r31[31];  // variant 0: (cos, sin)(2 pi m / 31)
   __shared__ RowRed<NW> red;
   const int tid = threadIdx.x;
   for (int i = tid; i < 256; i += T) stw[i] = a.tw[i];
   __syncthreads();
-  for (int i = tid; i < 16 * 33; i += T) tb[i] = tw_lds(stw, (i / 33) * (i % 33) * 31);
   const int wave = tid >> 6, lane = tid & 63, ln16 = lane & 15, grp = lane >> 4;
-  // variant 0: tl = tx[0 .. 31*16): W_P^{r (512 + b)} as [r][b]; r31 = tx[496 ..): (cos, sin)
-  //            (2 pi m / 31); wc: W_P^{r tid} in registers
-  // variant 1: t1023 = tx: W_1023^m; GEMM constants cA/sA; twl: W_P^{r ln16}, rr: the lane's r
-  float2* tl = tx;
-  float2* r31 = tx + 31 * LEFT;
-  float2* t1023 = tx;
-  float2 wc[31];
+  // variant 1: GEMM constants cA/sA and the lane's radix-31 inputs rr
   float cA[4], sA[4];
-  float2 twl[8];
   int rr[8];
   if constexpr (VARIANT == 0) {
-    if constexpr (SUB > 0)
-      for (int i = tid; i < 31 * LEFT; i += T) tl[i] = tw_lds(stw, (i / LEFT) * (T + i % LEFT));
     if (tid < 31) {
       const float2 w = tw_lds(stw, tid * NBC);  // exp(-2 pi i m / 31)
       r31[tid] = make_float2(w.x, -w.y);
     }
-    if constexpr (SUB > 0) {
-#pragma unroll
-      for (int r = 1; r < 31; ++r) wc[r] = tw_lds(stw, r * tid);
-    }
   } else {
-    if constexpr (SUB > 0)
-      for (int i = tid; i < 1023; i += T) t1023[i] = tw_lds(stw, 16 * i);
 #pragma unroll
     for (int st = 0; st < 4; ++st) {
       const int j = 4 * st + grp;
@@ -747,21 +728,10 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
       sA[st] = j == 0 ? 0.0f : -w.y;
       rr[2 * st] = j;
       rr[2 * st + 1] = j == 0 ? 0 : 31 - j;
-      if constexpr (SUB > 0) {
-        twl[2 * st] = tw_lds(stw, j * ln16);
-        twl[2 * st + 1] = j == 0 ? make_float2(0.f, 0.f) : tw_lds(stw, (31 - j) * ln16);
-      }
     }
   }
-  // Sample index of output r of pass-C butterfly j: Stockham order (sub-rows, natural-order
-  // input) j + 528 r; prime-factor order (X and R stored in it) the CRT number of
-  // (j mod 33, j / 33, r), and no twiddle factor between the passes.
-  auto out_idx = [](int j, int r) {
-    if constexpr (SUB == 0)
-      return pfa_wrap(pfa_base(j) + NBC * r);
-    else
-      return j + NBC * r;
-  };
+  // Sample index of output r of pass-C butterfly j: the CRT number of (j mod 33, j / 33, r).
+  auto out_idx = [](int j, int r) { return pfa_wrap(pfa_base(j) + NBC * r); };
   // One transform of a correlation row (or sub-row): `operand(ta, r)` gives pass A's input r of
   // butterfly ta; every output power goes to `sink(slot, p, idx)` with slot < 31 for the
   // thread's main set (outputs more than 32 samples apart) and slot 31 / 32 for its extra set.
@@ -800,10 +770,6 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
         if (b == 0 || j < NBB) {
           const int q33 = j / 33;
           const int k = j - q33 * 33;
-          if constexpr (SUB > 0) {
-#pragma unroll
-            for (int r = 1; r < 16; ++r) v[b][r] = cmul(v[b][r], tb[r * 33 + k]);
-          }
           Dft<16>::run(v[b]);
           const int base = q33 * NBC + k;
 #pragma unroll
@@ -826,10 +792,6 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
         float2 w[31];
 #pragma unroll
         for (int r = 0; r < 31; ++r) w[r] = buf[q + r * NBC];
-        if constexpr (SUB > 0) {
-#pragma unroll
-          for (int r = 1; r < 31; ++r) w[r] = cmul(w[r], tl[r * LEFT + b]);
-        }
         float ar = w[0].x, ai = w[0].y, br = 0.f, bi = 0.f;
         int m = 0;
 #pragma unroll
@@ -855,10 +817,6 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
           sink(32, p2, out_idx(q, 31 - kp));
         }
       }
-      if constexpr (SUB > 0) {
-#pragma unroll
-        for (int r = 1; r < 31; ++r) v[r] = cmul(v[r], wc[r]);
-      }
       Dft<31>::run(v);
 #pragma unroll
       for (int r = 0; r < 31; ++r) {
@@ -880,26 +838,12 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
           if (ts < ntile) {
             const int t = wave + 8 * ts;
             const int q = 16 * t + ln16;
-            // m1 = (r1 t) mod 1023 stepping r1 = gc, gc+4, ...; m2 = ((31 - r1) t) mod 1023
-            const int d4 = (4 * t) % 1023, c31 = (31 * t) % 1023;
-            int m1 = (gc * t) % 1023;
 #pragma unroll
             for (int st = 0; st < 4; ++st) {
               const int r1 = rr[2 * st], r2 = rr[2 * st + 1];
-              if (st > 0) {
-                m1 += d4;
-                if (m1 >= 1023) m1 -= 1023;
-              }
-              int m2 = c31 - m1;
-              if (m2 < 0) m2 += 1023;
-              float2 v1 = buf[q + r1 * NBC];
+              const float2 v1 = buf[q + r1 * NBC];
               float2 v2 = buf[q + r2 * NBC];
-              if constexpr (SUB > 0) {
-                v1 = cmul(v1, cmul(twl[2 * st], t1023[m1]));
-                v2 = cmul(v2, cmul(twl[2 * st + 1], t1023[m2]));  // 0 for r1 = 0
-              } else if (r1 == 0) {
-                v2 = make_float2(0.f, 0.f);  // a_0 = v_0 (no partner input)
-              }
+              if (r1 == 0) v2 = make_float2(0.f, 0.f);  // a_0 = v_0 (no partner input)
               fr[u][st][0] = v1.x + v2.x;
               fr[u][st][1] = v1.y + v2.y;
               fr[u][st][2] = v1.x - v2.x;
@@ -1075,19 +1019,47 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
                     a.rows + ((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) * 2 + h);
     }
   } else {
-    // sub-row s of the chunk: row r0 + s / SUB, output samples k0 + SUB m (k0 = s mod SUB)
+    // sub-row s of the chunk: row r0 + s / SUB, output samples k0 + SUB m (k0 = s mod SUB).
+    // The matrix-core variant keeps the sub-row's 33 pass-A inputs per thread in registers and
+    // loads the next sub-row's right after pass A, so the HBM latency hides behind passes B, C.
+    constexpr bool kPrefetch = VARIANT == 1;
+    float2 yn[kPrefetch ? 33 : 1];
+    auto load_sub = [&](long long sidx) {
+      const __amdgpu_buffer_rsrc_t yv = buffer_view(a.Y + sidx * P, P * 8);
+#pragma unroll
+      for (int r = 0; r < 33; ++r)
+        yn[r] = tid < NBA ? buffer_load_f2(yv, tid * 8, r * NBA * 8) : make_float2(0.f, 0.f);
+    };
+    if constexpr (kPrefetch) {
+      if (static_cast<long long>(blockIdx.x) < a.n_sub) load_sub(blockIdx.x);
+    }
     for (long long sidx = blockIdx.x; sidx < a.n_sub; sidx += gridDim.x) {
       const long long rq = sidx / SUB;
       const int k0 = static_cast<int>(sidx - rq * SUB);
       const __amdgpu_buffer_rsrc_t yv = buffer_view(a.Y + sidx * P, P * 8);
+      float2 yc[kPrefetch ? 33 : 1];
+      if constexpr (kPrefetch) {
+#pragma unroll
+        for (int r = 0; r < 33; ++r) yc[r] = yn[r];
+      }
       Top2 ms, es;
       float accn = 0.0f;
-      transform([&](int ta, int r) { return buffer_load_f2(yv, ta * 8, r * NBA * 8); },
-                [&](int slot, float p, int idx) {
-                  accn += p;
-                  (slot < 31 ? ms : es).add(p, idx);
-                },
-                [] {});
+      transform(
+          [&](int ta, int r) {
+            if constexpr (kPrefetch)
+              return yc[r];
+            else
+              return buffer_load_f2(yv, ta * 8, r * NBA * 8);
+          },
+          [&](int slot, float p, int idx) {
+            accn += p;
+            (slot < 31 ? ms : es).add(p, idx);
+          },
+          [&] {
+            if constexpr (kPrefetch) {
+              if (sidx + gridDim.x < a.n_sub) load_sub(sidx + gridDim.x);
+            }
+          });
       const long long r = big_row(a.r0 + rq, a.n_prn, a.n_bin, a.n_ant, a.hs);
       row_finish_key<T>(ms, es, accn != accn, red, k0, SUB, a.rowkey + r, a.rownan + r);
     }
@@ -1104,14 +1076,10 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
 // radix 33 -> 5 x 3 fragments, radix 16 -> 2 x 1, radix 31 -> 4 x 2.  A B tile is 16
 // butterflies: lane l supplies butterfly l%16 and k-rows 8(l/16)..+7 of a k-block (inputs
 // r = 4(l/16)..+3), and receives D rows 4(l/16)..+3 of a tile (outputs q = 2(l/16), 2(l/16)+1
-// of each 8-output tile).  The Stockham twiddles are applied in f32 to the pass outputs before
-// they are rounded to f16 in LDS:
-//   pass A (33, NS 1): output q of butterfly j carries pass B's input twiddle W_528^{(j/31) q}
-//     and, moved through pass B's DFT by linearity, the part W_P^{a k} of pass C's twiddle
-//     that depends only on pass B's butterfly (a = j_B / 33, k = j_B mod 33): W_P^{q j} in all;
-//   pass B (16, NS 33): output q of butterfly j then needs W_496^{(j/33) q}.
-// Pass C (31, NS 528) keeps its outputs in registers: output q of butterfly j is sample
-// j + 528 q, and |.|^2 feeds the same top-2 sets as the FP32 form.
+// of each 8-output tile).  Rows and sub-rows are stored in the prime-factor order of
+// 16368 = 33 x 16 x 31, so the pass outputs are rounded to f16 in LDS with no twiddle product.
+// Pass C (31) keeps its outputs in registers: output q of butterfly j is sample
+// pfa_base(j) + 528 q (mod P), and |.|^2 feeds the same top-2 sets as the FP32 form.
 struct Row16Args {
   const uint32_t* X;    // [xrow][P] f16 pairs
   const uint32_t* rep;  // [prn][2][P] f16 pairs
@@ -1162,10 +1130,6 @@ __device__ __forceinline__ halfx8 dft_frag(const float2* stw, int t, int kb, int
   return f;
 }
 
-__device__ __forceinline__ float2 cmulf(float a, float b, float2 w) {
-  return make_float2(a * w.x - b * w.y, a * w.y + b * w.x);
-}
-
 // Raw operand words of one pass-A tile for this lane: X and R at j + 496 r for the lane's
 // inputs r = 16 kb + 4 grp + i2 (kb < 2) and r = 32 (kb = 2, i2 = 0); r >= 33 lies past the
 // row, where buffer loads return 0.
@@ -1211,12 +1175,10 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
   __shared__ u32x4 fa_lds[15 * 64];  // radix-33 fragments [t*3 + kb][lane]
   __shared__ u32x4 fc_lds[8 * 64];   // radix-31 fragments [t*2 + kb][lane] (registers would spill)
   __shared__ float2 stw[256];
-  __shared__ float2 t16[31 * 16];  // W_496^{a q}, [a][q]
   __shared__ RowRed<NW16> red;
   const int tid = threadIdx.x;
   for (int i = tid; i < 256; i += T16) stw[i] = a.tw[i];
   __syncthreads();
-  for (int i = tid; i < 31 * 16; i += T16) t16[i] = tw_lds(stw, 33 * (i >> 4) * (i & 15));
   for (int i = tid; i < 15 * 64; i += T16)
     fa_lds[i] = __builtin_bit_cast(u32x4, dft_frag<33>(stw, (i >> 6) / 3, (i >> 6) % 3, i & 63));
   for (int i = tid; i < 8 * 64; i += T16)
@@ -1307,17 +1269,10 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
             b[kb] = __builtin_bit_cast(halfx8, w);
           }
         }
-        // output q = 8t + 2 grp (+1) of butterfly j is pass-B input r = j / 31 of butterfly
-        // 33 (j mod 31) + q.  Prime-factor order (SUB == 0): j = 31 n2 + n3 and no twiddle;
-        // Stockham order (sub-rows): twiddle W_P^{q j}.
+        // output q = 8t + 2 grp (+1) of butterfly j = 31 n2 + n3 is pass-B input r = j / 31 of
+        // butterfly 33 (j mod 31) + q (prime-factor order: no twiddle)
         const int jd = j / 31;
         uint32_t* dst = buf0 + 33 * (j - 31 * jd) + SB * jd + 2 * grp;
-        float2 s1, s8, w0;
-        if constexpr (SUB > 0) {
-          s1 = tw_lds(stw, j);
-          s8 = tw_lds(stw, 8 * j);
-          w0 = tw_lds(stw, 2 * grp * j);
-        }
 #pragma unroll
         for (int t = 0; t < 5; ++t) {
           floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
@@ -1326,15 +1281,8 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
             acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(
                 __builtin_bit_cast(halfx8, fa_lds[(t * 3 + kb) * 64 + lane]), b[kb], acc, 0, 0,
                 0);
-          if constexpr (SUB == 0) {
-            if (t < 4 || grp == 0) dst[8 * t] = pack_f16(make_float2(acc[0], acc[1]), 1.0f);
-            if (t < 4) dst[8 * t + 1] = pack_f16(make_float2(acc[2], acc[3]), 1.0f);
-          } else {
-            if (t > 0) w0 = cmul(w0, s8);
-            const float2 w1 = cmul(w0, s1);
-            if (t < 4 || grp == 0) dst[8 * t] = pack_f16(cmulf(acc[0], acc[1], w0), 1.0f);
-            if (t < 4) dst[8 * t + 1] = pack_f16(cmulf(acc[2], acc[3], w1), 1.0f);
-          }
+          if (t < 4 || grp == 0) dst[8 * t] = pack_f16(make_float2(acc[0], acc[1]), 1.0f);
+          if (t < 4) dst[8 * t + 1] = pack_f16(make_float2(acc[2], acc[3]), 1.0f);
         }
       }
     }
@@ -1353,22 +1301,14 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
       floatx4 acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(fb[1], b, floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
       if (ok) {
         // output q of butterfly j (a = j / 33, k = j mod 33) is pass-C input r = a of
-        // butterfly k + 33 q; the Stockham order twiddles it by W_496^{a q}
+        // butterfly k + 33 q
         const int aa = j / 33;
         const int k = j - 33 * aa;
         uint32_t* dst = buf1 + aa * SC + k + 66 * grp;
-        if constexpr (SUB == 0) {
-          dst[0] = pack_f16(make_float2(acc0[0], acc0[1]), 1.0f);
-          dst[33] = pack_f16(make_float2(acc0[2], acc0[3]), 1.0f);
-          dst[264] = pack_f16(make_float2(acc1[0], acc1[1]), 1.0f);
-          dst[297] = pack_f16(make_float2(acc1[2], acc1[3]), 1.0f);
-        } else {
-          const float2* tq = t16 + aa * 16 + 2 * grp;
-          dst[0] = pack_f16(cmulf(acc0[0], acc0[1], tq[0]), 1.0f);
-          dst[33] = pack_f16(cmulf(acc0[2], acc0[3], tq[1]), 1.0f);
-          dst[264] = pack_f16(cmulf(acc1[0], acc1[1], tq[8]), 1.0f);
-          dst[297] = pack_f16(cmulf(acc1[2], acc1[3], tq[9]), 1.0f);
-        }
+        dst[0] = pack_f16(make_float2(acc0[0], acc0[1]), 1.0f);
+        dst[33] = pack_f16(make_float2(acc0[2], acc0[3]), 1.0f);
+        dst[264] = pack_f16(make_float2(acc1[0], acc1[1]), 1.0f);
+        dst[297] = pack_f16(make_float2(acc1[2], acc1[3]), 1.0f);
       }
     }
     lds_barrier();
@@ -1376,9 +1316,9 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
     // 8-15) on every wave, and the four output tiles t of tile 32 on waves 0-3 (wave t takes
     // output tile t).  All products are issued before the first power is used, and the 16
     // powers of the two main tiles are merged as a tree, so neither the matrix-core latency nor
-    // the top-2 chain serialises.  Output q of butterfly j is sample j + 528 q (Stockham order)
-    // or pfa_base(j) + 528 q mod P (prime-factor order); either way the 16 outputs of a lane
-    // are pairwise at least 128 samples apart (ring distance), so each Top2 set may hold them.
+    // the top-2 chain serialises.  Output q of butterfly j is sample pfa_base(j) + 528 q mod P;
+    // the 16 outputs of a lane are pairwise at least 128 samples apart (ring distance), so each
+    // Top2 set may hold them.
     Top2 main_set, extra_set;
     float accn = 0.0f;  // sum of powers: NaN / overflow detector
     {
@@ -1396,11 +1336,9 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
       };
       const int j0 = 16 * (wave < 8 ? wave : wave + 8) + ln16, j1 = j0 + 128;
       // sample index of output 2 grp of butterflies j0, j1 (add 528 q' and wrap for q')
-      const int base0 = SUB == 0 ? pfa_base(j0) + NBC * 2 * grp : j0 + NBC * 2 * grp;
-      const int base1 = SUB == 0 ? pfa_base(j1) + NBC * 2 * grp : j1 + NBC * 2 * grp;
-      auto sample = [&](int base, int dq) {
-        return SUB == 0 ? pfa_wrap(base + NBC * dq) : base + NBC * dq;
-      };
+      const int base0 = pfa_base(j0) + NBC * 2 * grp;
+      const int base1 = pfa_base(j1) + NBC * 2 * grp;
+      auto sample = [&](int base, int dq) { return pfa_wrap(base + NBC * dq); };
       halfx8 b0[2], b1[2];
       load_c(j0, b0);
       load_c(j1, b1);
@@ -1448,7 +1386,7 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
         floatx4 acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(f0, b[0], floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(f1, b[1], acc, 0, 0, 0);
         const int q = 8 * wave + 2 * grp;
-        const int be = SUB == 0 ? pfa_base(j) + NBC * q : j + NBC * q;
+        const int be = pfa_base(j) + NBC * q;
         const float pa = acc[0] * acc[0] + acc[1] * acc[1];
         accn += pa;
         extra_set.add(pa, sample(be, 0));
@@ -1626,9 +1564,13 @@ __global__ __launch_bounds__(kRedT) void k_row_reduce(const float2* __restrict__
 // With n = n1 + 16368 n0 and k = k0 + R0 k1, the P-point transform of z = conj(X) R is
 //   Z[k0 + R0 k1] = FFT_16368( W_P^{n1 k0} sum_{n0} z[n1 + 16368 n0] W_R0^{n0 k0} )[k1]:
 // k_big_pre forms the R0 twiddled sub-rows y_k0 of each row (one R0-point DFT per n1), and the
-// fused rows kernel in sub-row mode transforms each sub-row in LDS.  F16: the sub-rows are
-// stored as f16 (re, im) pairs scaled by kSplitF16Scale for the matrix-core rows kernel (the
-// products and the R0-point DFT stay f32).
+// fused rows kernel in sub-row mode transforms each sub-row in LDS.  X and R of long rows are
+// stored segment by segment in the prime-factor order of 16368 (position 16368 n0 + t holds
+// natural index 16368 n0 + pfa_nat(t)), so the pre-pass reads and writes position t of every
+// segment with contiguous accesses and its sub-rows arrive in the order the rows kernels
+// transform without twiddle factors.  F16: the sub-rows are stored as f16 (re, im) pairs
+// scaled by kSplitF16Scale for the matrix-core rows kernel (the products and the R0-point DFT
+// stay f32).
 template <int R0, bool F16>
 __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
                                                  const float2* __restrict__ rep,
@@ -1645,8 +1587,8 @@ __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
   // contiguous eighth of every row and its L2 holds contiguous slices of the tile's spectra
   const long long rr = blockIdx.x >> 6;
   const int chunk = 8 * static_cast<int>(blockIdx.x & 7) + static_cast<int>((blockIdx.x >> 3) & 7);
-  const int n1 = 256 * chunk + static_cast<int>(threadIdx.x);
-  if (rr >= n_rows || n1 >= P1) return;
+  const int t = 256 * chunk + static_cast<int>(threadIdx.x);  // position within a segment
+  if (rr >= n_rows || t >= P1) return;
   const long long I = r0 + rr;  // tiled order (big_item)
   const BigItem it = big_item(I, n_prn, n_bin, n_ant, hs);
   const long long xrow = it.xrow;
@@ -1656,12 +1598,12 @@ __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
   const float2* c = rep + (static_cast<long long>(prn) * 2 + h) * P;
   float2 v[R0];
 #pragma unroll
-  for (int n0 = 0; n0 < R0; ++n0) v[n0] = cconjmul(x[n1 + P1 * n0], c[n1 + P1 * n0]);
+  for (int n0 = 0; n0 < R0; ++n0) v[n0] = cconjmul(x[t + P1 * n0], c[t + P1 * n0]);
   Dft<R0>::run(v);
-  // W_P^{n1 k0} = (W_P^{n1})^k0 by repeated products from one two-level table lookup: two
-  // gathers per thread instead of two per output keep the texture path free for the operand
-  // streams (at most R0 - 1 = 9 products: relative error below 1e-6)
-  const float2 w1 = cmul(twb[n1 & 511], twb[512 + (n1 >> 9)]);
+  // W_P^{n1 k0} = (W_P^{n1})^k0 by repeated products from one lookup of W_P^{n1} in a table
+  // kept in the prime-factor order of the positions, so the lanes of a wave read consecutive
+  // entries (at most R0 - 1 = 9 products: relative error below 1e-6)
+  const float2 w1 = twb[t];
   float2 wk = w1;
 #pragma unroll
   for (int k0 = 1; k0 < R0; ++k0) {
@@ -1675,17 +1617,19 @@ __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
 #pragma unroll
     for (int k0 = 0; k0 < R0; ++k0)
       __builtin_nontemporal_store(fused::pack_f16(v[k0], kSplitF16Scale),
-                                  y + static_cast<long long>(k0) * P1 + n1);
+                                  y + static_cast<long long>(k0) * P1 + t);
   } else {
+    // (the two float stores merge into one non-temporal dwordx2; a 64-bit integer store
+    // through the same builtin loses the non-temporal bit)
     float2* y = static_cast<float2*>(Yv) + rr * P;
 #pragma unroll
     for (int k0 = 0; k0 < R0; ++k0) {
-      float* d = reinterpret_cast<float*>(y + static_cast<long long>(k0) * P1 + n1);
+      float* d = reinterpret_cast<float*>(y + static_cast<long long>(k0) * P1 + t);
       __builtin_nontemporal_store(v[k0].x, d);
       __builtin_nontemporal_store(v[k0].y, d + 1);
     }
   }
-  if (n1 == 0) {
+  if (t == 0) {
     const long long r = big_row(I, n_prn, n_bin, n_ant, hs);
     rowkey[r] = 0ull;
     rownan[r] = 0;
@@ -1710,7 +1654,14 @@ __global__ void k_big_rowout(const unsigned long long* __restrict__ rowkey,
   if (hs == 2) rows[r + 1] = RowOut{0.0f, -1.0f, 0, 0};  // the coinciding hypothesis never wins
 }
 
-// Z[w][i] = conj(X) R of the winning row win[w] of (antenna, PRN) pair w.
+// Position of natural index n (< 16368) in the prime-factor order of 16368: the inverse of
+// fused::pfa_nat, 496 (n mod 33) + 31 ((-n) mod 16) + (n mod 31).
+__device__ __forceinline__ int pfa_pos(int n) {
+  return 496 * (n % 33) + 31 * ((16 - (n & 15)) & 15) + n % 31;
+}
+
+// Z[w][i] = conj(X) R of the winning row win[w] of (antenna, PRN) pair w, in natural order
+// (X and R of long rows are stored segment by segment in the prime-factor order).
 __global__ void k_mul_win(const float2* __restrict__ X, const float2* __restrict__ rep, int M,
                           int n_prn, int n_bin, int folded, const int* __restrict__ win,
                           long long total, float2* __restrict__ Z) {
@@ -1718,11 +1669,14 @@ __global__ void k_mul_win(const float2* __restrict__ X, const float2* __restrict
   if (gid >= total) return;
   const long long w = gid / M;
   const int i = static_cast<int>(gid - w * M);
+  const int seg = i / kFusedP;
+  const int pos = seg * kFusedP + pfa_pos(i - seg * kFusedP);
   const int ant = static_cast<int>(w / n_prn);
   const int prn = static_cast<int>(w - static_cast<long long>(ant) * n_prn);
   const int bh = win[w];  // bin * 2 + h
   const long long xrow = static_cast<long long>(ant) * n_bin * 2 + bh;
-  Z[gid] = cconjmul(X[(folded ? xrow : xrow >> 1) * M + i], rep[(static_cast<long long>(prn) * 2 + (bh & 1)) * M + i]);
+  Z[gid] = cconjmul(X[(folded ? xrow : xrow >> 1) * M + pos],
+                    rep[(static_cast<long long>(prn) * 2 + (bh & 1)) * M + pos]);
 }
 
 // Second peak of each winning row around the selected code phase: max |z|^2 more than cw
@@ -2011,12 +1965,13 @@ bool big_supported(size_t P, size_t N, bool folded) {
 }
 
 std::vector<float2> big_twiddles(size_t P) {
-  const size_t n2 = (P + 511) / 512;
-  std::vector<float2> t(512 + n2);
-  for (size_t i = 0; i < 512 + n2; ++i) {
-    const unsigned long long e = i < 512 ? i : 512ull * (i - 512);
-    const double a = -2.0 * kPiD * static_cast<double>(e % P) / static_cast<double>(P);
-    t[i] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+  std::vector<float2> t(kFusedP);
+  for (int s = 0; s < kFusedP; ++s) {
+    // natural index of prime-factor position s (fused::pfa_nat)
+    const int n1 = s / 496, b = s - 496 * n1, n2 = b / 31, n3 = b - 31 * n2;
+    const long long e = (496LL * n1 + 1023LL * n2 + 528LL * n3) % kFusedP;
+    const double a = -2.0 * kPiD * static_cast<double>(e) / static_cast<double>(P);
+    t[s] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
   }
   return t;
 }
@@ -2046,7 +2001,7 @@ void launch_big_pre(const float2* X, const float2* rep, const float2* twb, int R
 
 void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, int hs, long long r0,
                     long long r1, const void* Y, bool f16, unsigned long long* rowkey,
-                    int* rownan, RowOut* rows, hipStream_t st) {
+                    int* rownan, RowOut* rows, hipStream_t st, int variant) {
   const long long nr = r1 - r0;
   const long long n_sub = nr * R0;
   const unsigned blocks = static_cast<unsigned>(n_sub < 2048 ? n_sub : 2048);
@@ -2081,10 +2036,14 @@ void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, i
     a.r0 = r0;
     a.rowkey = rowkey;
     a.rownan = rownan;
-    if (R0 == 4)
+    if (R0 == 4 && variant == 0)
       hipLaunchKernelGGL((fused::k_rows<0, 4>), dim3(blocks), dim3(fused::T), 0, st, a);
-    else
+    else if (R0 == 4)
+      hipLaunchKernelGGL((fused::k_rows<1, 4>), dim3(blocks), dim3(fused::T), 0, st, a);
+    else if (variant == 0)
       hipLaunchKernelGGL((fused::k_rows<0, 10>), dim3(blocks), dim3(fused::T), 0, st, a);
+    else
+      hipLaunchKernelGGL((fused::k_rows<1, 10>), dim3(blocks), dim3(fused::T), 0, st, a);
   }
   TGA_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_big_rowout, dim3(grid_for(nr, 256)), dim3(256), 0, st, rowkey, rownan,

@@ -59,6 +59,13 @@ def fused_variant(request, monkeypatch):
     return request.param
 
 
+@pytest.fixture(params=[0, 1], ids=["split-valu31", "split-mfma31"])
+def split_variant(request, monkeypatch):
+    """Both radix-31 implementations of the long rows' sub-row transforms."""
+    monkeypatch.setenv("TGA_SPLIT_VARIANT", str(request.param))
+    return request.param
+
+
 @pytest.mark.parametrize("fmt", [1, 2])
 def test_fused_ca_plan_matches_oracle(gpu, fmt, fused_variant):
     L = 163680
@@ -174,7 +181,7 @@ def test_masked_second_peak_many_code_phases(gpu, fused_variant):
         assert int(peaks[0][a, 0]["best_codephase"]) % 16368 == (16368 - d) % 16368
 
 
-def test_split_e1c_at_16368khz(gpu):
+def test_split_e1c_at_16368khz(gpu, split_variant):
     # E1-C at 16.368 MHz: P = 65472 = 4 x 16368, rows transformed as 4 twiddled sub-rows by the
     # fused LDS kernel; the second peak comes from recomputing the winning row
     fs, L = FS, 2 * 65472 + 500
@@ -185,12 +192,13 @@ def test_split_e1c_at_16368khz(gpu):
     systems = [("galileo", [3, 11, 27], fc)]
     plan, peaks, packed = run_case(fs, L, 2, 1, systems, signals, seed=21)
     assert "split4x16368" in plan.describe()
+    assert f"rows_variant={split_variant}" in plan.describe()
     check_against_oracle(plan, peaks, packed, fs, L, 1, systems)
     assert int(peaks[0][0, 1]["best_codephase"]) == (65472 - 40000) % 65472
     assert int(peaks[0][1, 2]["best_codephase"]) == (65472 - 123) % 65472
 
 
-def test_split_row_tiles_ragged(gpu):
+def test_split_row_tiles_ragged(gpu, split_variant):
     # long rows are enumerated in tiles of 8 forward spectra x 8 replica spectra, one hypothesis
     # at a time: 10 PRNs and 2 x 5 (bin, h) spectra leave full and ragged tiles in both
     # directions, and every row must still land at its own index
@@ -205,7 +213,7 @@ def test_split_row_tiles_ragged(gpu):
 
 
 @pytest.mark.parametrize("system,prn", [("beidou", 19), ("l1c", 7)])
-def test_split_b1c_l1c_at_16368khz(gpu, system, prn):
+def test_split_b1c_l1c_at_16368khz(gpu, split_variant, system, prn):
     # B1C / L1C at 16.368 MHz: P = 163680 = 10 x 16368 (BOC(1,1) half-chips)
     fs, L = FS, 2 * 163680
     # Doppler on the grid: 20 ms of coherent integration has its first null 50 Hz off
@@ -294,7 +302,7 @@ def test_noncoherent_plan_errors(gpu):
         tga.Plan(FS, 1, 4 * 65472, [tga.SystemSearch("galileo", fc, [1], nc_epochs=2)])
 
 
-def test_split_unfolded_10ms_b1c(gpu):
+def test_split_unfolded_10ms_b1c(gpu, split_variant):
     # a 10 ms record holds one B1C period: N = P = 163680, unfolded rows through the split path,
     # only the [c,c] rows transformed (correlate.rs:53-56: the hypotheses coincide)
     fs, L = FS, 163680
