@@ -430,6 +430,81 @@ __global__ __launch_bounds__(T) void k_forward(FwdArgs a) {
   }
 }
 
+// Pre-pass of positions [p_begin, p_end) of row rr of a chunk (enumerated row r0 + rr): thread
+// lt takes the two consecutive positions p_begin + 2 lt (+1) with 16-byte loads and stores.
+template <int R0, bool F16>
+__device__ __forceinline__ void big_pre_positions(const float2* __restrict__ X,
+                                                  const float2* __restrict__ rep,
+                                                  const float2* __restrict__ twb, int n_prn,
+                                                  int n_bin, int n_ant, int hs, long long r0,
+                                                  long long rr, int p_begin, int p_end, int lt,
+                                                  void* __restrict__ Yv,
+                                                  unsigned long long* __restrict__ rowkey,
+                                                  int* __restrict__ rownan) {
+  constexpr int P1 = kFusedP;
+  constexpr int P = R0 * P1;
+  const int t = p_begin + 2 * lt;
+  if (t >= p_end) return;
+  const long long I = r0 + rr;  // tiled order (big_item)
+  const BigItem it = big_item(I, n_prn, n_bin, n_ant, hs);
+  const long long xrow = it.xrow;
+  const int prn = it.prn;
+  const int h = static_cast<int>(xrow & 1);
+  const float4* x = reinterpret_cast<const float4*>(X + (hs == 1 ? xrow : xrow >> 1) * P);
+  const float4* c = reinterpret_cast<const float4*>(rep + (static_cast<long long>(prn) * 2 + h) * P);
+  float2 v[R0], u[R0];
+#pragma unroll
+  for (int n0 = 0; n0 < R0; ++n0) {
+    const float4 xa = x[(t + P1 * n0) >> 1], ca = c[(t + P1 * n0) >> 1];
+    v[n0] = cconjmul(make_float2(xa.x, xa.y), make_float2(ca.x, ca.y));
+    u[n0] = cconjmul(make_float2(xa.z, xa.w), make_float2(ca.z, ca.w));
+  }
+  Dft<R0>::run(v);
+  Dft<R0>::run(u);
+  // W_P^{n1 k0} = (W_P^{n1})^k0 by repeated products from one lookup of W_P^{n1} per position
+  // in a table kept in the prime-factor order of the positions (at most R0 - 1 = 9 products:
+  // relative error below 1e-6)
+  const float4 wq = reinterpret_cast<const float4*>(twb)[t >> 1];
+  const float2 w1 = make_float2(wq.x, wq.y), w2 = make_float2(wq.z, wq.w);
+  float2 wk = w1, wl = w2;
+#pragma unroll
+  for (int k0 = 1; k0 < R0; ++k0) {
+    v[k0] = cmul(v[k0], wk);
+    u[k0] = cmul(u[k0], wl);
+    if (k0 + 1 < R0) {
+      wk = cmul(wk, w1);
+      wl = cmul(wl, w2);
+    }
+  }
+  // Non-temporal stores: the sub-rows are read once, by the transforms, and should not displace
+  // the tile's spectra from L2 (the float stores merge into one non-temporal dwordx4 per
+  // sub-row; a 64-bit integer store through the same builtin would lose the non-temporal bit).
+  if constexpr (F16) {
+    uint32_t* y = static_cast<uint32_t*>(Yv) + rr * P;
+#pragma unroll
+    for (int k0 = 0; k0 < R0; ++k0) {
+      uint32_t* d = y + static_cast<long long>(k0) * P1 + t;
+      __builtin_nontemporal_store(pack_f16(v[k0], kSplitF16Scale), d);
+      __builtin_nontemporal_store(pack_f16(u[k0], kSplitF16Scale), d + 1);
+    }
+  } else {
+    float2* y = static_cast<float2*>(Yv) + rr * P;
+#pragma unroll
+    for (int k0 = 0; k0 < R0; ++k0) {
+      float* d = reinterpret_cast<float*>(y + static_cast<long long>(k0) * P1 + t);
+      __builtin_nontemporal_store(v[k0].x, d);
+      __builtin_nontemporal_store(v[k0].y, d + 1);
+      __builtin_nontemporal_store(u[k0].x, d + 2);
+      __builtin_nontemporal_store(u[k0].y, d + 3);
+    }
+  }
+  if (t == 0) {
+    const long long r = big_row(I, n_prn, n_bin, n_ant, hs);
+    rowkey[r] = 0ull;
+    rownan[r] = 0;
+  }
+}
+
 struct RowArgs {
   const float2* X;
   const float2* rep;
@@ -1571,6 +1646,10 @@ __global__ __launch_bounds__(kRedT) void k_row_reduce(const float2* __restrict__
 // transform without twiddle factors.  F16: the sub-rows are stored as f16 (re, im) pairs
 // scaled by kSplitF16Scale for the matrix-core rows kernel (the products and the R0-point DFT
 // stay f32).
+// Standalone pre-pass of a chunk: 32 blocks of 512 positions per row.  Blocks are dealt
+// round-robin to the 8 XCDs, so block b takes chunk 4 (b mod 8) + (b / 8) mod 4 of row b / 32:
+// each XCD works on one contiguous eighth of every row and its L2 holds contiguous slices of
+// the tile's spectra.
 template <int R0, bool F16>
 __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
                                                  const float2* __restrict__ rep,
@@ -1580,60 +1659,13 @@ __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
                                                  void* __restrict__ Yv,
                                                  unsigned long long* __restrict__ rowkey,
                                                  int* __restrict__ rownan) {
-  constexpr int P1 = kFusedP;
-  constexpr int P = R0 * P1;
-  // 64 blocks of 256 samples per row; blocks are dealt round-robin to the 8 XCDs, so block b
-  // takes chunk 8 (b mod 8) + (b / 8) mod 8 of row b / 64: each XCD then works on one
-  // contiguous eighth of every row and its L2 holds contiguous slices of the tile's spectra
-  const long long rr = blockIdx.x >> 6;
-  const int chunk = 8 * static_cast<int>(blockIdx.x & 7) + static_cast<int>((blockIdx.x >> 3) & 7);
-  const int t = 256 * chunk + static_cast<int>(threadIdx.x);  // position within a segment
-  if (rr >= n_rows || t >= P1) return;
-  const long long I = r0 + rr;  // tiled order (big_item)
-  const BigItem it = big_item(I, n_prn, n_bin, n_ant, hs);
-  const long long xrow = it.xrow;
-  const int prn = it.prn;
-  const int h = static_cast<int>(xrow & 1);
-  const float2* x = X + (hs == 1 ? xrow : xrow >> 1) * P;  // unfolded: one spectrum per bin
-  const float2* c = rep + (static_cast<long long>(prn) * 2 + h) * P;
-  float2 v[R0];
-#pragma unroll
-  for (int n0 = 0; n0 < R0; ++n0) v[n0] = cconjmul(x[t + P1 * n0], c[t + P1 * n0]);
-  Dft<R0>::run(v);
-  // W_P^{n1 k0} = (W_P^{n1})^k0 by repeated products from one lookup of W_P^{n1} in a table
-  // kept in the prime-factor order of the positions, so the lanes of a wave read consecutive
-  // entries (at most R0 - 1 = 9 products: relative error below 1e-6)
-  const float2 w1 = twb[t];
-  float2 wk = w1;
-#pragma unroll
-  for (int k0 = 1; k0 < R0; ++k0) {
-    v[k0] = cmul(v[k0], wk);
-    if (k0 + 1 < R0) wk = cmul(wk, w1);
-  }
-  // non-temporal stores: the sub-rows are read once, by the next kernel, and should not
-  // displace the tile's spectra from L2
-  if constexpr (F16) {
-    uint32_t* y = static_cast<uint32_t*>(Yv) + rr * P;
-#pragma unroll
-    for (int k0 = 0; k0 < R0; ++k0)
-      __builtin_nontemporal_store(fused::pack_f16(v[k0], kSplitF16Scale),
-                                  y + static_cast<long long>(k0) * P1 + t);
-  } else {
-    // (the two float stores merge into one non-temporal dwordx2; a 64-bit integer store
-    // through the same builtin loses the non-temporal bit)
-    float2* y = static_cast<float2*>(Yv) + rr * P;
-#pragma unroll
-    for (int k0 = 0; k0 < R0; ++k0) {
-      float* d = reinterpret_cast<float*>(y + static_cast<long long>(k0) * P1 + t);
-      __builtin_nontemporal_store(v[k0].x, d);
-      __builtin_nontemporal_store(v[k0].y, d + 1);
-    }
-  }
-  if (t == 0) {
-    const long long r = big_row(I, n_prn, n_bin, n_ant, hs);
-    rowkey[r] = 0ull;
-    rownan[r] = 0;
-  }
+  const long long rr = blockIdx.x >> 5;
+  if (rr >= n_rows) return;
+  const int chunk = 4 * static_cast<int>(blockIdx.x & 7) + static_cast<int>((blockIdx.x >> 3) & 3);
+  const int p0 = 512 * chunk;
+  fused::big_pre_positions<R0, F16>(X, rep, twb, n_prn, n_bin, n_ant, hs, r0, rr, p0,
+                             p0 + 512 < kFusedP ? p0 + 512 : kFusedP,
+                             static_cast<int>(threadIdx.x), Yv, rowkey, rownan);
 }
 
 // Row keys of long rows -> RowOut (p2 is filled later from the winning row only).
@@ -1980,8 +2012,8 @@ void launch_big_pre(const float2* X, const float2* rep, const float2* twb, int R
                     int n_bin, int n_ant, int hs, long long r0, long long r1, void* Y, bool f16,
                     unsigned long long* rowkey, int* rownan, hipStream_t st) {
   const long long nr = r1 - r0;
-  const long long total = nr;  // rows of this chunk, 64 blocks each
-  const dim3 grid(static_cast<unsigned>(nr * 64));
+  const long long total = nr;  // rows of this chunk, 32 blocks each
+  const dim3 grid(static_cast<unsigned>(nr * 32));
   if (R0 == 4 && !f16)
     hipLaunchKernelGGL((k_big_pre<4, false>), grid, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
                        n_ant, hs, r0, total, Y, rowkey, rownan);
