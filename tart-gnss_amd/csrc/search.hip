@@ -721,17 +721,18 @@ __device__ __forceinline__ void row_finish(const Top2& main_set, const Top2& ext
   row_flush<NT>(red, pend);
 }
 
-// End of one sub-row of a long row: block argmax, folded into the row's 64-bit key (power
-// bits, sample index) by an atomic maximum, so the last maximum over all sub-rows wins.
+// End of one sub-row of a long row: block argmax of the threads' best keys (power bits, sample
+// index: kmax), folded into the row's 64-bit key by an atomic maximum, so the last maximum over
+// all sub-rows wins.  Only the argmax is kept: the masked second peak of long rows comes from a
+// recomputation of the winning row.
 template <int NT>
-__device__ __forceinline__ void row_finish_key(const Top2& main_set, const Top2& extra_set,
-                                               bool bad, RowRed<NT / 64>& red, int k0, int r0f,
+__device__ __forceinline__ void row_finish_key(unsigned long long kmax, bool bad,
+                                               RowRed<NT / 64>& red, int k0, int r0f,
                                                unsigned long long* key, int* nan) {
   constexpr int NWR = NT / 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  float bp = main_set.best();
-  int bi = main_set.index();
-  arg_combine(bp, bi, extra_set.best(), extra_set.index());
+  const float bp = __uint_as_float(static_cast<unsigned int>(kmax >> 32));
+  const int bi = static_cast<int>(kmax & 0xffffffffu);
   unsigned int pw, iw;
   wave_argmax(bp, bi, pw, iw);
   const bool wave_bad = __ballot(bad) != 0;
@@ -1117,8 +1118,9 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
 #pragma unroll
         for (int r = 0; r < 33; ++r) yc[r] = yn[r];
       }
-      Top2 ms, es;
-      float accn = 0.0f;
+      // the sub-row's argmax only: a running 64-bit key maximum per thread (a NaN power has
+      // the largest key, so it reaches the row's key and is flagged there)
+      unsigned long long km = 0ull;
       transform(
           [&](int ta, int r) {
             if constexpr (kPrefetch)
@@ -1126,9 +1128,9 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
             else
               return buffer_load_f2(yv, ta * 8, r * NBA * 8);
           },
-          [&](int slot, float p, int idx) {
-            accn += p;
-            (slot < 31 ? ms : es).add(p, idx);
+          [&](int, float p, int idx) {
+            const unsigned long long k = pkey(p, idx);
+            km = k > km ? k : km;
           },
           [&] {
             if constexpr (kPrefetch) {
@@ -1136,7 +1138,8 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
             }
           });
       const long long r = big_row(a.r0 + rq, a.n_prn, a.n_bin, a.n_ant, a.hs);
-      row_finish_key<T>(ms, es, accn != accn, red, k0, SUB, a.rowkey + r, a.rownan + r);
+      row_finish_key<T>(km, (km >> 32) > 0x7f800000ull, red, k0, SUB, a.rowkey + r,
+                        a.rownan + r);
     }
   }
 }
@@ -1396,6 +1399,13 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
     // Top2 set may hold them.
     Top2 main_set, extra_set;
     float accn = 0.0f;  // sum of powers: NaN / overflow detector
+    // sub-rows keep only their argmax: a running key maximum (inf or NaN powers have the
+    // largest keys and are flagged at the end)
+    unsigned long long km = 0ull;
+    auto keep = [&](float p, int idx) {
+      const unsigned long long k = pkey(p, idx);
+      km = k > km ? k : km;
+    };
     {
       auto load_c = [&](int j, halfx8* b) {
 #pragma unroll
@@ -1433,25 +1443,37 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
         c0[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(f, b0[1], c0[t], 0, 0, 0);
         c1[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(f, b1[1], c1[t], 0, 0, 0);
       }
-      Top2 lv[16];
+      if constexpr (SUB == 0) {
+        Top2 lv[16];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const float pa0 = c0[t][0] * c0[t][0] + c0[t][1] * c0[t][1];
-        const float pb0 = c0[t][2] * c0[t][2] + c0[t][3] * c0[t][3];
-        const float pa1 = c1[t][0] * c1[t][0] + c1[t][1] * c1[t][1];
-        const float pb1 = c1[t][2] * c1[t][2] + c1[t][3] * c1[t][3];
-        const bool okb = t < 3 || grp < 3;  // output 8 t + 2 grp + 1 = 31 does not exist
-        accn += (pa0 + pa1) + (okb ? pb0 + pb1 : 0.0f);
-        lv[4 * t + 0] = Top2::leaf(pa0, sample(base0, 8 * t));
-        lv[4 * t + 1] = Top2::leaf(pa1, sample(base1, 8 * t));
-        lv[4 * t + 2] = okb ? Top2::leaf(pb0, sample(base0, 8 * t + 1)) : Top2();
-        lv[4 * t + 3] = okb ? Top2::leaf(pb1, sample(base1, 8 * t + 1)) : Top2();
+        for (int t = 0; t < 4; ++t) {
+          const float pa0 = c0[t][0] * c0[t][0] + c0[t][1] * c0[t][1];
+          const float pb0 = c0[t][2] * c0[t][2] + c0[t][3] * c0[t][3];
+          const float pa1 = c1[t][0] * c1[t][0] + c1[t][1] * c1[t][1];
+          const float pb1 = c1[t][2] * c1[t][2] + c1[t][3] * c1[t][3];
+          const bool okb = t < 3 || grp < 3;  // output 8 t + 2 grp + 1 = 31 does not exist
+          accn += (pa0 + pa1) + (okb ? pb0 + pb1 : 0.0f);
+          lv[4 * t + 0] = Top2::leaf(pa0, sample(base0, 8 * t));
+          lv[4 * t + 1] = Top2::leaf(pa1, sample(base1, 8 * t));
+          lv[4 * t + 2] = okb ? Top2::leaf(pb0, sample(base0, 8 * t + 1)) : Top2();
+          lv[4 * t + 3] = okb ? Top2::leaf(pb1, sample(base1, 8 * t + 1)) : Top2();
+        }
+#pragma unroll
+        for (int w = 8; w >= 1; w >>= 1)
+#pragma unroll
+          for (int i = 0; i < w; ++i) lv[i] = Top2::merge(lv[i], lv[i + w]);
+        main_set = lv[0];
+      } else {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          keep(c0[t][0] * c0[t][0] + c0[t][1] * c0[t][1], sample(base0, 8 * t));
+          keep(c1[t][0] * c1[t][0] + c1[t][1] * c1[t][1], sample(base1, 8 * t));
+          if (t < 3 || grp < 3) {  // output 8 t + 2 grp + 1 = 31 does not exist
+            keep(c0[t][2] * c0[t][2] + c0[t][3] * c0[t][3], sample(base0, 8 * t + 1));
+            keep(c1[t][2] * c1[t][2] + c1[t][3] * c1[t][3], sample(base1, 8 * t + 1));
+          }
+        }
       }
-#pragma unroll
-      for (int w = 8; w >= 1; w >>= 1)
-#pragma unroll
-        for (int i = 0; i < w; ++i) lv[i] = Top2::merge(lv[i], lv[i + w]);
-      main_set = lv[0];
       if (wave < 4) {
         const int j = 512 + ln16;
         halfx8 b[2];
@@ -1463,12 +1485,18 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
         const int q = 8 * wave + 2 * grp;
         const int be = pfa_base(j) + NBC * q;
         const float pa = acc[0] * acc[0] + acc[1] * acc[1];
-        accn += pa;
-        extra_set.add(pa, sample(be, 0));
-        if (wave < 3 || grp < 3) {
-          const float pb = acc[2] * acc[2] + acc[3] * acc[3];
-          accn += pb;
-          extra_set.add(pb, sample(be, 1));
+        const float pb = acc[2] * acc[2] + acc[3] * acc[3];
+        const bool okb = wave < 3 || grp < 3;
+        if constexpr (SUB == 0) {
+          accn += pa;
+          extra_set.add(pa, sample(be, 0));
+          if (okb) {
+            accn += pb;
+            extra_set.add(pb, sample(be, 1));
+          }
+        } else {
+          keep(pa, sample(be, 0));
+          if (okb) keep(pb, sample(be, 1));
         }
       }
     }
@@ -1481,7 +1509,7 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
       const long long rq = it / SUB;
       const int k0 = static_cast<int>(it - rq * SUB);
       const long long r = big_row(a.r0 + rq, a.n_prn, a.n_bin, a.n_ant, a.hs);
-      row_finish_key<T16>(main_set, extra_set, !(accn <= 3.0e38f), red, k0, SUB, a.rowkey + r,
+      row_finish_key<T16>(km, (km >> 32) >= 0x7f800000ull, red, k0, SUB, a.rowkey + r,
                           a.rownan + r);
     }
   }
