@@ -182,6 +182,63 @@ class DeviceInput:
             raise RuntimeError(f"{what} failed with HIP error {rc}")
 
 
+def surface_errors(dev, L, fmt, wsystems, signals, levels0):
+    """Correlation-surface error of the FP16 and FP32 forms against an FP64 truth, per system:
+    a one-antenna plan of the workload's systems over antenna 0 of the resident observation
+    (per system the first injected PRN and one more, the 3 central bins) through
+    tga_plan_surface, and the truth |IFFT(FFT(x e^{j 2 pi f n / fs}) conj(FFT(code)))| / sqrt(N)
+    of the de-meaned record in float64 (numpy).  Error of a row: max |diff| / the row's peak;
+    the worst row (both hypotheses, code phases 0..P-1) of each system."""
+    import ctypes as C
+    import tga
+    x = levels0.astype(np.float64)
+    x -= x.sum() / len(x)
+    sub = []
+    for s, prns, fc in wsystems:
+        inj = [g["prn"] for g in signals if g["system"] == s][:1]
+        pr = inj + [p for p in prns if p not in inj][:2 - len(inj)]
+        mid = len(fc) // 2
+        sub.append((s, pr, fc[max(mid - 1, 0):mid + 2]))
+    searches = [tga.SystemSearch(s, fc, pr) for s, pr, fc in sub]
+    out = {}
+    for prec in ("fp16", "fp32"):
+        plan = tga.Plan(FS, 1, L, searches, input_format=fmt, precision=prec)
+        n = plan.surface_size()
+        buf = C.c_void_p()
+        dev._check(dev.hip.hipMalloc(C.byref(buf), C.c_size_t(4 * n)), "hipMalloc")
+        try:
+            plan.surface_device(dev.ptr, buf.value, n)  # antenna 0 is the first packed row
+            flat = np.empty(n, dtype=np.float32)
+            dev._check(dev.hip.hipMemcpy(flat.ctypes.data_as(C.c_void_p), buf,
+                                         C.c_size_t(4 * n), 2), "hipMemcpy")  # 2 = D2H
+        finally:
+            dev.hip.hipFree(buf)
+        err, off = {}, 0
+        for si, (s, pr, fc) in enumerate(sub):
+            k = plan.surface_size(si)
+            P = k // (len(pr) * len(fc) * 2)
+            surf = flat[off:off + k].reshape(len(pr), len(fc), 2, P)
+            off += k
+            N = min(2 * P, L)
+            ph = (2.0 * math.pi / FS) * np.arange(N, dtype=np.float64)
+            worst = 0.0
+            for b, f in enumerate(fc):
+                X = np.fft.fft(x[:N] * np.exp(1j * ph * f))
+                for j, prn in enumerate(pr):
+                    code = tga.code_resampled(s, prn, float(P), N)
+                    alt = code.copy()
+                    alt[P:] *= -1.0
+                    for h, c in enumerate((code, alt) if N == 2 * P else (code,)):
+                        y = np.fft.ifft(X * np.conj(np.fft.fft(c))) * N
+                        t = np.abs(y[:P]) / math.sqrt(N)
+                        g = surf[j, b, h].astype(np.float64)
+                        worst = max(worst, float(np.abs(g - t).max() / t.max()))
+            err[s] = worst
+        plan.close()
+        out[prec] = err
+    return out
+
+
 def load_traffic(path, kernel, workload_name):
     """HBM bytes per launch (per search for a rows stage of several kernels) of `kernel`
     measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE) on this workload, from the committed
@@ -207,6 +264,8 @@ def main():
     ap.add_argument("--traffic", default=os.path.join(REPO, "profiles", "pmc_traffic.json"))
     ap.add_argument("--no-fp16", action="store_true",
                     help="skip the FP16 matrix-core form (reported beside the FP32 value)")
+    ap.add_argument("--no-surface", action="store_true",
+                    help="skip the correlation-surface error of the FP16 / FP32 forms")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="first run one elementwise copy of MIB MiB (reads and writes MIB MiB) "
                          "so hardware byte counters can be scaled against a known size")
@@ -339,6 +398,15 @@ def main():
                 "same_bin_and_codephase_as_fp32": f"{same}/{rel.size}",
                 "plan": plan16.describe().splitlines()[0]}
 
+    surface = None
+    if not args.no_surface and not nc and not stream:
+        errs = surface_errors(dev, L, fmt, w["systems"], signals,
+                              synth.unpack_levels(packed[:1], L, fmt)[0])
+        surface = {"fp16_vs_fp64": errs["fp16"], "fp32_vs_fp64": errs["fp32"],
+                   "measure": "worst row of max |diff| / row peak over code phases 0..P-1",
+                   "sample": "antenna 0; per system the first injected PRN and one more, the 3 "
+                             "central bins, both hypotheses"}
+
     # detections: every injected SV found at its code phase on most antennas
     sysidx = {s: i for i, (s, _, _) in enumerate(w["systems"])}
 
@@ -422,7 +490,8 @@ def main():
                   "plan_create_s": t_plan, "synth_s": t_synth, "cn0_valid": n_cn0,
                   "injected_found": f"{found}/{len(signals)}", "rfi": rfi,
                   "observations_per_s": args.steps / dt if stream else None,
-                  "graph": plan.describe().strip().splitlines()[-1], "fp16_mfma": fp16},
+                  "graph": plan.describe().strip().splitlines()[-1], "fp16_mfma": fp16,
+                  "surface_error": surface},
     }
     print(json.dumps(out))
 

@@ -250,6 +250,23 @@ const char* tga_plan_kernel_name(const tga_plan* plan, int32_t kernel);
 /* Describes the plan's execution path for each system ("fused16368", "generic", ...). */
 int tga_plan_describe(const tga_plan* plan, char* buf, size_t cap);
 
+/* Correlation surface of a search (no counterpart in the reference, which forms the same
+ * values, |IFFT(X . code_complex)|[i] / sqrt(N) per bin and hypothesis in correlate.rs:118-146,
+ * and keeps only their maxima): for stating the error of a precision form against an FP64
+ * truth.  Searches d_packed (device memory, as tga_search_device, eagerly) and writes every
+ * row's values at code phases i < P -- the reference's surface at i and i + P for N = 2P --
+ * to d_surface (device memory), system by system in plan order as [ant][prn][bin][h][P]
+ * floats on the scale of tga_peak::best_peak (the winning row's maximum equals its best_peak).
+ * Unfolded rows (N = P) have no h = 1 row: zeros.  out (host, optional) receives the search's
+ * peaks as tga_search_device would.  Covers plans whose groups run the fused (P = 16368) or
+ * split (P = 4 or 10 x 16368) kernels, else TGA_ERR_UNSUPPORTED; cap < the size
+ * tga_plan_surface_size reports is TGA_ERR_INVALID_ARGUMENT.  Allocates a staging buffer of the
+ * same size. */
+/* Floats of system sys_index's surface, or of the whole plan's for sys_index = -1. */
+int tga_plan_surface_size(const tga_plan* plan, int32_t sys_index, size_t* n_floats);
+int tga_plan_surface(tga_plan* plan, const void* d_packed, void* stream, float* d_surface,
+                     size_t cap, tga_peak* out);
+
 /* ---- ACR C/N0 (acr.rs:49-84; gate acquisition.rs:345-352) ------------------------------ */
 
 /* estimate_cn0: *valid = 0 for None (r_a <= 1 or not finite). */

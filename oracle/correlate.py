@@ -105,6 +105,21 @@ def bin_rows(signal, pp, cc, cc_alt, fc, num_samples, mode="fp32"):
     return out
 
 
+def surfaces(signal, pp, cc, cc_alt, fc, num_samples, P, mode="fp64"):
+    """Every row's correlation values at code phases 0..P-1, [n_bin, 2, P]: corr =
+    |IFFT(X . rep)| / sqrt(N) for rep = cc, cc_alt (correlate.rs:118-131, the array whose last
+    maximum bin_rows keeps).  For N = 2P the rows are P-periodic (the replicas are [c,c] and
+    [c,-c]), so phases P..2P-1 repeat these."""
+    N = num_samples
+    out = np.empty((len(fc), 2, P), dtype=np.float64)
+    for b, freq in enumerate(fc):
+        X = np.fft.fft(wipe(signal, pp, freq, N, mode))
+        for h, rep in enumerate((cc, cc_alt)):
+            y = np.fft.ifft(X * np.asarray(rep, dtype=np.complex128)) * N
+            out[b, h] = np.abs(y[:P]) / math.sqrt(N)
+    return out
+
+
 def second_peak(best_corr, best_codephase, P, fs):
     """correlate.rs:156-175."""
     cw = int(math.ceil(fs / 1.023e6))

@@ -491,7 +491,14 @@ void Plan::time_collect() {
   }
 }
 
-void Plan::search(const uint8_t* d_in, hipStream_t st) {
+// Row-value gain of k_select: folded rows carry half the 2P-point gain; the FP16 forms' values
+// carry kF16Scale^2 (fused) or kSplitF16Scale (split).
+static float select_gain(const Group& g) {
+  const float fold = g.folded ? 2.0f : 1.0f;
+  return fold / (g.fp16 ? kF16Scale * kF16Scale : g.big16 ? kSplitF16Scale : 1.0f);
+}
+
+void Plan::search(const uint8_t* d_in, hipStream_t st, float* dump) {
   if (st == nullptr) st = stream;
   launch_ant_sums(d_in, row_bytes, n_ant, L, fmt, d_partial.as<long long>(),
                   d_means.as<double>(), d_nan.as<int>(), st);
@@ -546,10 +553,11 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
     } else if (g.fp16) {
       launch_fused_rows16(g.d_X16.as<uint32_t>(), g.d_rep16.as<uint32_t>(), g.d_tw.as<float2>(),
                           n_ant, g.n_bin, g.n_prn, g.cw, g.d_rows.as<RowOut>(),
-                          g.rows_cfg.groups, st);
+                          g.rows_cfg.groups, st, dump ? dump + g.dump_off : nullptr);
     } else if (g.fused) {
       launch_fused_rows(g.X, g.d_rep.as<float2>(), g.d_tw.as<float2>(), n_ant, g.n_bin, g.n_prn,
-                        g.cw, g.d_rows.as<RowOut>(), st, g.rows_cfg);
+                        g.cw, g.d_rows.as<RowOut>(), st, g.rows_cfg,
+                        dump ? dump + g.dump_off : nullptr);
     } else if (g.big) {
       const int hs = g.folded ? 1 : 2;
       const long long n_enum = g.n_rows / hs;
@@ -560,7 +568,8 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
                        g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(), st);
         launch_big_sub(g.d_tw.as<float2>(), g.big_r0, g.n_prn, g.n_bin, n_ant, hs, r0, r1,
                        g.d_Y.p, g.big16, g.d_rowkey.as<unsigned long long>(),
-                       g.d_rownan.as<int>(), g.d_rows.as<RowOut>(), st, g.rows_cfg.variant);
+                       g.d_rownan.as<int>(), g.d_rows.as<RowOut>(), st, g.rows_cfg.variant,
+                       dump ? dump + g.dump_off : nullptr);
       }
     } else {
       const FftPlan* fp = fft_plan(g.M);
@@ -578,10 +587,9 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
   time_end(0, st, rows_bytes);
   time_begin(2, st);
   for (auto& g : groups) {
-    // folded rows carry half the 2P-point gain; the FP16 forms' values carry kF16Scale^2
-    // (fused) or kSplitF16Scale (split); the winners' second peaks are recomputed in f32
+    // the winners' second peaks of split groups are recomputed in f32
     const float fold = g.folded ? 2.0f : 1.0f;
-    const float gain = fold / (g.fp16 ? kF16Scale * kF16Scale : g.big16 ? kSplitF16Scale : 1.0f);
+    const float gain = select_gain(g);
     launch_select(g.d_rows.as<RowOut>(), g.d_slots.as<PeakSlot>(), n_ant, g.n_prn, g.n_bin,
                   gain, g.scale, d_peaks.as<tga_peak>(), d_nan.as<int>(), st,
                   g.big ? g.d_win.as<int>() : nullptr);
@@ -592,6 +600,50 @@ void Plan::search(const uint8_t* d_in, hipStream_t st) {
                         g.d_Zws.as<float2>(), d_peaks.as<tga_peak>(), st);
   }
   time_end(2, st, 0.0);
+}
+
+size_t Plan::surface_floats(int sys_index) const {
+  size_t n = 0;
+  for (size_t i = 0; i < systems.size(); ++i) {
+    const SysEntry& e = systems[i];
+    const Group& g = groups[e.group];
+    if (sys_index < 0 || static_cast<size_t>(sys_index) == i)
+      n += static_cast<size_t>(n_ant) * e.prns.size() * g.n_bin * 2 * g.P;
+  }
+  return n;
+}
+
+void Plan::surface(const uint8_t* d_in, float* d_out, hipStream_t st) {
+  if (st == nullptr) st = stream;
+  size_t tot = 0;
+  for (auto& g : groups) {
+    if (g.nc > 0 || !(g.fused || g.fp16 || g.big))
+      fail(TGA_ERR_UNSUPPORTED,
+           "the surface form covers the fused (16368) and split (4 or 10 x 16368) groups only");
+    g.dump_off = tot;
+    tot += static_cast<size_t>(n_ant) * g.n_prn * g.n_bin * 2 * g.P;
+  }
+  DevBuf stage(tot * sizeof(float));
+  // unfolded long rows have no h = 1 row: those stay zero
+  TGA_HIP(hipMemsetAsync(stage.p, 0, tot * sizeof(float), st));
+  search(d_in, st, stage.as<float>());
+  for (auto& g : groups)
+    launch_surface_finish(stage.as<float>() + g.dump_off,
+                          static_cast<long long>(n_ant) * g.n_prn * g.n_bin * 2 * g.P,
+                          select_gain(g), g.scale, st);
+  // group layout [ant][group prn][bin][h][P] -> system by system [ant][prn][bin][h][P]
+  size_t o = 0;
+  for (const auto& e : systems) {
+    const Group& g = groups[e.group];
+    const size_t per_prn = static_cast<size_t>(g.n_bin) * 2 * g.P;
+    const size_t width = e.prns.size() * per_prn;
+    TGA_HIP(hipMemcpy2DAsync(d_out + o, width * sizeof(float),
+                             stage.as<float>() + g.dump_off + e.first_prn * per_prn,
+                             g.n_prn * per_prn * sizeof(float), width * sizeof(float), n_ant,
+                             hipMemcpyDeviceToDevice, st));
+    o += width * n_ant;
+  }
+  TGA_HIP(hipStreamSynchronize(st));
 }
 
 void Plan::finish(tga_peak* out, hipStream_t st) {
@@ -716,6 +768,31 @@ int tga_search_device(tga_plan* plan, const void* d_packed, void* stream, tga_pe
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : p->stream;
     p->run(static_cast<const uint8_t*>(d_packed), st);
     p->finish(out, st);
+  });
+}
+
+int tga_plan_surface_size(const tga_plan* plan, int32_t sys_index, size_t* n_floats) {
+  return plan_guard([&] {
+    if (!plan || !n_floats) fail(TGA_ERR_INVALID_ARGUMENT, "null argument");
+    const Plan* p = reinterpret_cast<const Plan*>(plan);
+    if (sys_index < -1 || sys_index >= static_cast<int32_t>(p->systems.size()))
+      fail(TGA_ERR_INVALID_ARGUMENT, "system index out of range");
+    *n_floats = p->surface_floats(sys_index);
+  });
+}
+
+int tga_plan_surface(tga_plan* plan, const void* d_packed, void* stream, float* d_surface,
+                     size_t cap, tga_peak* out) {
+  return plan_guard([&] {
+    if (!plan || !d_packed || !d_surface) fail(TGA_ERR_INVALID_ARGUMENT, "null argument");
+    Plan* p = reinterpret_cast<Plan*>(plan);
+    const size_t need = p->surface_floats();
+    if (cap < need)
+      fail(TGA_ERR_INVALID_ARGUMENT, "surface buffer holds " + std::to_string(cap) +
+                                         " floats, the plan's rows need " + std::to_string(need));
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : p->stream;
+    p->surface(static_cast<const uint8_t*>(d_packed), d_surface, st);
+    if (out) p->finish(out, st);
   });
 }
 

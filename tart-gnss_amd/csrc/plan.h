@@ -36,6 +36,7 @@ struct Group {
   FusedRowsConfig rows_cfg;  // rows-kernel shape picked by the autotuner
   std::vector<std::pair<FusedRowsConfig, double>> tune;  // candidates timed, ms per launch
   double row_bytes_alg = 0, fwd_bytes_alg = 0;
+  size_t dump_off = 0;  // surface form: first float of the group's rows in the staging buffer
 };
 
 struct SysEntry {
@@ -82,7 +83,13 @@ struct Plan {
 
   ~Plan();
   void build(const tga_plan_desc& d);
-  void search(const uint8_t* d_in, hipStream_t st);
+  // dump (surface form): when not null, every row's output powers also go to
+  // dump + g.dump_off + row * P (group layout, see surface())
+  void search(const uint8_t* d_in, hipStream_t st, float* dump = nullptr);
+  // Correlation surface of every row of a search of d_in, system by system in plan order:
+  // [ant][prn][bin][h][P] floats on the scale of tga_peak::best_peak.
+  size_t surface_floats(int sys_index = -1) const;
+  void surface(const uint8_t* d_in, float* d_out, hipStream_t st);
   // Enqueue one search of d_in on st: eager, or a copy into d_input plus a graph replay.
   void run(const uint8_t* d_in, hipStream_t st);
   void drop_graph();

@@ -59,9 +59,11 @@ struct FusedRowsConfig {
   int groups = 8;
 };
 // rows[((ant*n_prn + prn)*n_bin + bin)*2 + h]
+// dump (surface form, here and in launch_fused_rows16 / launch_big_sub): when not null, every
+// output power of row (or long row) i is also stored at dump[i * P + sample], P the code period.
 void launch_fused_rows(const float2* X, const float2* rep, const float2* tw, int n_ant,
                        int n_bin, int n_prn, int cw, RowOut* rows, hipStream_t st,
-                       const FusedRowsConfig& cfg);
+                       const FusedRowsConfig& cfg, float* dump = nullptr);
 // Non-coherent form (P = 16368): X[(ant*n_bin + bin)*epochs + e][P] are the spectra of the
 // one-period epochs; rows[((ant*n_prn + prn)*n_bin + bin)*2] hold the top-2 of the power sums
 // over epochs (the h = 1 slot is a row that never wins the selection).
@@ -89,7 +91,7 @@ void launch_fused_forward16(const uint8_t* packed, size_t row_bytes, int fmt,
                             const float2* carrier, uint32_t* X16, hipStream_t st);
 void launch_fused_rows16(const uint32_t* X16, const uint32_t* rep16, const float2* tw,
                          int n_ant, int n_bin, int n_prn, int cw, RowOut* rows, int groups,
-                         hipStream_t st);
+                         hipStream_t st, float* dump = nullptr);
 // n values = n / 16368 natural-order rows -> f16 pairs times scale in the prime-factor order
 void launch_to_f16(const float2* in, uint32_t* out, long long n, float scale, hipStream_t st);
 // Twiddle table of the fused kernels: W_P^i (i < 128) then W_P^{128 i} (i < 128).
@@ -155,7 +157,9 @@ void launch_big_pre(const float2* X, const float2* rep, const float2* twb, int R
 void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, int hs,
                     long long r0, long long r1, const void* Y, bool f16,
                     unsigned long long* rowkey, int* rownan, RowOut* rows, hipStream_t st,
-                    int variant);
+                    int variant, float* dump = nullptr);
+// surface form: v[i] = (sqrt(v[i]) * gain) * scale in place (the row values of k_select)
+void launch_surface_finish(float* v, long long n, float gain, float scale, hipStream_t st);
 // second peaks of the winning rows; Z, Zs hold n_ant * n_prn * M complex values
 void launch_big_second(const float2* X, const float2* rep, const PeakSlot* slots, int M, int P,
                        int cw, int n_ant, int n_prn, int n_bin, bool folded, const int* win,

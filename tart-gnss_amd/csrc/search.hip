@@ -524,6 +524,9 @@ struct RowArgs {
   int epochs;  // non-coherent form: one-period epochs per (antenna, bin)
   int hs;      // sub-row mode: 2 when only h = 0 rows are enumerated (unfolded rows)
   int n_ant;   // sub-row mode: antennas (for the tiled row order)
+  // surface form (DUMP): every output power also goes to dump[row * period + sample], row the
+  // RowOut index of the correlation row, period 16368 (rows) or SUB x 16368 (sub-rows)
+  float* dump;
 };
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -773,7 +776,8 @@ __device__ __forceinline__ void row_finish_key(unsigned long long kmax, bool bad
 // Each thread keeps the top two powers of sets whose indices lie more than 32 samples apart
 // pairwise, so at most one member of a set falls inside the +-cw exclusion window around the
 // block's argmax and the masked second maximum needs no second pass over the row.
-template <int VARIANT, int SUB = 0, bool NC = false>
+// DUMP: the surface form, which also stores every output power (a diagnostic, not the search).
+template <int VARIANT, int SUB = 0, bool NC = false, bool DUMP = false>
 __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
   constexpr int NBA = P / 33;  // 496
   constexpr int NBB = P / 16;  // 1023
@@ -1064,6 +1068,7 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
     }
     for (int prn = p0; prn < p1; ++prn) {
       const __amdgpu_buffer_rsrc_t rv = rep_view(prn);
+      const size_t orow = ((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) * 2 + h;
       Top2 ms, es;
       float accn = 0.0f;  // sum of powers: NaN-propagating detector
       float2 rc[kKeepX ? 33 : 1];
@@ -1083,6 +1088,7 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
           [&](int slot, float p, int idx) {
             accn += p;
             (slot < 31 ? ms : es).add(p, idx);
+            if constexpr (DUMP) a.dump[orow * P + idx] = p;
           },
           [&] {
             if constexpr (kKeepX) {
@@ -1091,8 +1097,7 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
           });
       // the two-barrier finish: deferring its second half to the next row's first barrier
       // measured 6% slower here (wave 0 then carries the flush into the next pass)
-      row_finish<T>(ms, es, accn != accn, a.cw, red,
-                    a.rows + ((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) * 2 + h);
+      row_finish<T>(ms, es, accn != accn, a.cw, red, a.rows + orow);
     }
   } else {
     // sub-row s of the chunk: row r0 + s / SUB, output samples k0 + SUB m (k0 = s mod SUB).
@@ -1112,6 +1117,7 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
     for (long long sidx = blockIdx.x; sidx < a.n_sub; sidx += gridDim.x) {
       const long long rq = sidx / SUB;
       const int k0 = static_cast<int>(sidx - rq * SUB);
+      const long long r = big_row(a.r0 + rq, a.n_prn, a.n_bin, a.n_ant, a.hs);
       const __amdgpu_buffer_rsrc_t yv = buffer_view(a.Y + sidx * P, P * 8);
       float2 yc[kPrefetch ? 33 : 1];
       if constexpr (kPrefetch) {
@@ -1131,13 +1137,13 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
           [&](int, float p, int idx) {
             const unsigned long long k = pkey(p, idx);
             km = k > km ? k : km;
+            if constexpr (DUMP) a.dump[static_cast<size_t>(r) * (SUB * P) + k0 + SUB * idx] = p;
           },
           [&] {
             if constexpr (kPrefetch) {
               if (sidx + gridDim.x < a.n_sub) load_sub(sidx + gridDim.x);
             }
           });
-      const long long r = big_row(a.r0 + rq, a.n_prn, a.n_bin, a.n_ant, a.hs);
       row_finish_key<T>(km, (km >> 32) > 0x7f800000ull, red, k0, SUB, a.rowkey + r,
                         a.rownan + r);
     }
@@ -1177,6 +1183,7 @@ struct Row16Args {
   int hs = 1;
   int n_ant = 0;  // sub-row mode: antennas (for the tiled row order)
   long long n_items = 0;  // SUB == 0: (xrow, PRN group) items, n_xrow * groups
+  float* dump = nullptr;  // surface form: as RowArgs::dump
 };
 
 // conj(x) * r on f16 pairs (two packed instructions)
@@ -1239,7 +1246,7 @@ __device__ __forceinline__ void load_tile_y(uint32_t (&y)[9], __amdgpu_buffer_rs
 constexpr int T16 = 1024;
 constexpr int NW16 = T16 / 64;
 
-template <int SUB>
+template <int SUB, bool DUMP = false>
 __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
   constexpr int NBA = P / 33;  // 496
   constexpr int NBB = P / 16;  // 1023
@@ -1295,6 +1302,22 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
   }
   for (long long it = it0; it < it1; it += step) {
     const int prn = static_cast<int>(it);
+    // surface form: output sample idx of this row (sub-row) is stored at dbase + dstride idx
+    size_t dbase = 0;
+    int dstride = 1;
+    if constexpr (DUMP) {
+      if constexpr (SUB == 0) {
+        dbase = ((((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) * 2 + h)) * P;
+      } else {
+        const long long rq = it / SUB;
+        const long long r = big_row(a.r0 + rq, a.n_prn, a.n_bin, a.n_ant, a.hs);
+        dbase = static_cast<size_t>(r) * (SUB * P) + static_cast<size_t>(it - rq * SUB);
+        dstride = SUB;
+      }
+    }
+    auto dump_at = [&](float p, int idx) {
+      if constexpr (DUMP) a.dump[dbase + static_cast<size_t>(dstride) * idx] = p;
+    };
     // SUB == 0: replica row; SUB > 0: the sub-row itself
     const __amdgpu_buffer_rsrc_t rv =
         SUB == 0 ? buffer_view(a.rep + (static_cast<size_t>(prn) * 2 + h) * P, P * 4)
@@ -1405,6 +1428,7 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
     auto keep = [&](float p, int idx) {
       const unsigned long long k = pkey(p, idx);
       km = k > km ? k : km;
+      dump_at(p, idx);
     };
     {
       auto load_c = [&](int j, halfx8* b) {
@@ -1453,6 +1477,14 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
           const float pb1 = c1[t][2] * c1[t][2] + c1[t][3] * c1[t][3];
           const bool okb = t < 3 || grp < 3;  // output 8 t + 2 grp + 1 = 31 does not exist
           accn += (pa0 + pa1) + (okb ? pb0 + pb1 : 0.0f);
+          if constexpr (DUMP) {
+            dump_at(pa0, sample(base0, 8 * t));
+            dump_at(pa1, sample(base1, 8 * t));
+            if (okb) {
+              dump_at(pb0, sample(base0, 8 * t + 1));
+              dump_at(pb1, sample(base1, 8 * t + 1));
+            }
+          }
           lv[4 * t + 0] = Top2::leaf(pa0, sample(base0, 8 * t));
           lv[4 * t + 1] = Top2::leaf(pa1, sample(base1, 8 * t));
           lv[4 * t + 2] = okb ? Top2::leaf(pb0, sample(base0, 8 * t + 1)) : Top2();
@@ -1490,9 +1522,11 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
         if constexpr (SUB == 0) {
           accn += pa;
           extra_set.add(pa, sample(be, 0));
+          dump_at(pa, sample(be, 0));
           if (okb) {
             accn += pb;
             extra_set.add(pb, sample(be, 1));
+            dump_at(pb, sample(be, 1));
           }
         } else {
           keep(pa, sample(be, 0));
@@ -1873,12 +1907,19 @@ long long persistent_blocks(long long n_items, int groups) {
 
 void launch_fused_rows(const float2* X, const float2* rep, const float2* tw, int n_ant,
                        int n_bin, int n_prn, int cw, RowOut* rows, hipStream_t st,
-                       const FusedRowsConfig& cfg) {
+                       const FusedRowsConfig& cfg, float* dump) {
   const int groups = n_prn < cfg.groups ? n_prn : cfg.groups;
   const int per = (n_prn + groups - 1) / groups;
   fused::RowArgs a{X, rep, tw, n_bin, n_prn, groups, per, cw, rows};
+  a.dump = dump;
   const long long blocks = static_cast<long long>(n_ant) * n_bin * 2 * groups;
-  if (cfg.variant == 0)
+  if (dump && cfg.variant == 0)
+    hipLaunchKernelGGL((fused::k_rows<0, 0, false, true>), dim3(static_cast<unsigned>(blocks)),
+                       dim3(fused::T), 0, st, a);
+  else if (dump)
+    hipLaunchKernelGGL((fused::k_rows<1, 0, false, true>), dim3(static_cast<unsigned>(blocks)),
+                       dim3(fused::T), 0, st, a);
+  else if (cfg.variant == 0)
     hipLaunchKernelGGL(fused::k_rows<0>, dim3(static_cast<unsigned>(blocks)), dim3(fused::T), 0,
                        st, a);
   else
@@ -1927,14 +1968,19 @@ void launch_fused_forward16(const uint8_t* packed, size_t row_bytes, int fmt,
 
 void launch_fused_rows16(const uint32_t* X16, const uint32_t* rep16, const float2* tw,
                          int n_ant, int n_bin, int n_prn, int cw, RowOut* rows, int groups,
-                         hipStream_t st) {
+                         hipStream_t st, float* dump) {
   groups = n_prn < groups ? n_prn : groups;
   const int per = (n_prn + groups - 1) / groups;
   fused::Row16Args a{X16, rep16, tw, n_bin, n_prn, groups, per, cw, rows};
   a.n_items = static_cast<long long>(n_ant) * n_bin * 2 * groups;
+  a.dump = dump;
   const long long blocks = persistent_blocks(a.n_items, groups);
-  hipLaunchKernelGGL(fused::k_rows16<0>, dim3(static_cast<unsigned>(blocks)),
-                     dim3(fused::T16), 0, st, a);
+  if (dump)
+    hipLaunchKernelGGL((fused::k_rows16<0, true>), dim3(static_cast<unsigned>(blocks)),
+                       dim3(fused::T16), 0, st, a);
+  else
+    hipLaunchKernelGGL(fused::k_rows16<0>, dim3(static_cast<unsigned>(blocks)),
+                       dim3(fused::T16), 0, st, a);
   TGA_HIP(hipGetLastError());
 }
 
@@ -2061,7 +2107,7 @@ void launch_big_pre(const float2* X, const float2* rep, const float2* twb, int R
 
 void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, int hs, long long r0,
                     long long r1, const void* Y, bool f16, unsigned long long* rowkey,
-                    int* rownan, RowOut* rows, hipStream_t st, int variant) {
+                    int* rownan, RowOut* rows, hipStream_t st, int variant, float* dump) {
   const long long nr = r1 - r0;
   const long long n_sub = nr * R0;
   const unsigned blocks = static_cast<unsigned>(n_sub < 2048 ? n_sub : 2048);
@@ -2079,7 +2125,12 @@ void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, i
     a.rownan = rownan;
     a.hs = hs;
     a.n_ant = n_ant;
-    if (R0 == 4)
+    a.dump = dump;
+    if (dump && R0 == 4)
+      hipLaunchKernelGGL((fused::k_rows16<4, true>), dim3(blocks), dim3(fused::T16), 0, st, a);
+    else if (dump)
+      hipLaunchKernelGGL((fused::k_rows16<10, true>), dim3(blocks), dim3(fused::T16), 0, st, a);
+    else if (R0 == 4)
       hipLaunchKernelGGL((fused::k_rows16<4>), dim3(blocks), dim3(fused::T16), 0, st, a);
     else
       hipLaunchKernelGGL((fused::k_rows16<10>), dim3(blocks), dim3(fused::T16), 0, st, a);
@@ -2096,7 +2147,15 @@ void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, i
     a.r0 = r0;
     a.rowkey = rowkey;
     a.rownan = rownan;
-    if (R0 == 4 && variant == 0)
+    a.dump = dump;
+    // the surface form runs the matrix-core radix-31 variant whatever `variant` says
+    if (dump && R0 == 4)
+      hipLaunchKernelGGL((fused::k_rows<1, 4, false, true>), dim3(blocks), dim3(fused::T), 0, st,
+                         a);
+    else if (dump)
+      hipLaunchKernelGGL((fused::k_rows<1, 10, false, true>), dim3(blocks), dim3(fused::T), 0,
+                         st, a);
+    else if (R0 == 4 && variant == 0)
       hipLaunchKernelGGL((fused::k_rows<0, 4>), dim3(blocks), dim3(fused::T), 0, st, a);
     else if (R0 == 4)
       hipLaunchKernelGGL((fused::k_rows<1, 4>), dim3(blocks), dim3(fused::T), 0, st, a);
@@ -2108,6 +2167,18 @@ void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, i
   TGA_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_big_rowout, dim3(grid_for(nr, 256)), dim3(256), 0, st, rowkey, rownan,
                      r0, nr, n_prn, n_bin, n_ant, hs, rows);
+  TGA_HIP(hipGetLastError());
+}
+
+__global__ void k_surface_finish(float* __restrict__ v, long long n, float gain, float scale) {
+  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (i < n) v[i] = (__fsqrt_rn(v[i]) * gain) * scale;
+}
+
+void launch_surface_finish(float* v, long long n, float gain, float scale, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_surface_finish, dim3(grid_for(n, 256)), dim3(256), 0, st, v, n, gain,
+                     scale);
   TGA_HIP(hipGetLastError());
 }
 

@@ -325,6 +325,46 @@ class Plan:
                                       out.ctypes.data_as(C.POINTER(L.Peak))))
         return self._split(out)
 
+    def surface_size(self, sys_index=-1):
+        """Floats of one system's correlation surface (all systems' for -1)."""
+        n = C.c_size_t(0)
+        check(lib().tga_plan_surface_size(self._h, int(sys_index), C.byref(n)))
+        return n.value
+
+    def surface_device(self, dev_ptr, d_surface, cap, stream=None):
+        """Correlation surface of a search of the observation at dev_ptr (tga_plan_surface):
+        writes cap >= surface_size() floats at the device address d_surface and returns the
+        search's peaks."""
+        out = np.zeros(self.n_peaks, dtype=PEAK_DTYPE)
+        check(lib().tga_plan_surface(self._h, C.c_void_p(dev_ptr),
+                                     C.c_void_p(stream) if stream else None,
+                                     C.cast(C.c_void_p(d_surface), C.POINTER(C.c_float)),
+                                     int(cap), out.ctypes.data_as(C.POINTER(L.Peak))))
+        return self._split(out)
+
+    def surface(self, packed):
+        """packed: uint8 [n_ant, row_bytes] host array -> (surfaces, peaks): per system a float32
+        array [n_ant, n_prn, n_bin, 2, P] of the rows' values at code phases 0..P-1 on the
+        best_peak scale, and the search's peaks.  Device memory comes from torch."""
+        import torch
+        pk = np.ascontiguousarray(packed, dtype=np.uint8)
+        if pk.ndim != 2 or pk.shape != (self.n_ant, self.row_bytes):
+            raise ValueError("packed must be [n_ant, row_bytes]")
+        dev = torch.from_numpy(pk).cuda()
+        n = self.surface_size()
+        surf = torch.empty(n, dtype=torch.float32, device=dev.device)
+        torch.cuda.synchronize()
+        peaks = self.surface_device(dev.data_ptr(), surf.data_ptr(), n)
+        flat = surf.cpu().numpy()
+        out, off = [], 0
+        for i, s in enumerate(self.systems):
+            k = self.surface_size(i)
+            n_bin = len(s.fc)
+            P = k // (self.n_ant * len(self.prns[i]) * n_bin * 2)
+            out.append(flat[off:off + k].reshape(self.n_ant, len(self.prns[i]), n_bin, 2, P))
+            off += k
+        return out, peaks
+
     def set_timing(self, enable=True):
         check(lib().tga_plan_set_timing(self._h, 1 if enable else 0))
 

@@ -142,6 +142,30 @@ def test_contrast():
     assert a["signal_strength"] > 10 * b["signal_strength"]
 
 
+@pytest.mark.parametrize("N,P", [(64, 32), (62, 31), (33, 33)])
+def test_surfaces_match_brute_force(N, P):
+    # oracle.surfaces is the array the GPU's surface form is checked against: every row, both
+    # hypotheses, phases 0..P-1, against the direct circular correlation
+    rng = np.random.default_rng(N + 7 * P)
+    fs = 1.0e6
+    sig = rng.standard_normal(N)
+    pp = oc.phasepoints(fs, N)
+    code = rng.choice([-1.0, 1.0], size=N)
+    cc, ca = oc.code_spectra(code, P, mode="fp64")
+    fc = [0.0, 2500.0]
+    surf = oc.surfaces(sig, pp, cc, ca, fc, N, P, mode="fp64")
+    assert surf.shape == (2, 2, P)
+    alt = code.copy()
+    alt[min(P, N):] *= -1
+    for b, f in enumerate(fc):
+        iq = oc.wipe(sig, pp, f, N, "fp64")
+        for h, c in enumerate((code, alt)):
+            want = brute_corr(iq, c)[:P] / math.sqrt(N)
+            assert np.allclose(surf[b, h], want, rtol=1e-10, atol=1e-10), (b, h)
+        rows = oc.bin_rows(sig, pp, cc, ca, fc, N, mode="fp64")
+        assert rows[b][1] == pytest.approx(surf[b].max(), rel=1e-12)
+
+
 @pytest.mark.parametrize("flip", [False, True])
 def test_folding_identity(flip):
     # The GPU's P-point folded rows must equal the reference's 2P-point rows, up to m -> m mod P.
