@@ -68,6 +68,58 @@ def ratio(system, cn0_db, trials, rng, plan=None):
     return float(np.sum(vm * vm) / np.sum(vs * vs)), found, plan
 
 
+# The reference's end-to-end C/N0 tests (acquisition.rs:551-595, galileo.rs:465-510,
+# beidou.rs:414-459, l1c.rs:426-470): trials without / with the sign flip, code-phase step per
+# trial (delay = t * step mod P).
+E2E = {"gps": (80, 120, 137), "galileo": (60, 120, 251), "beidou": (50, 120, 379),
+       "l1c": (50, 120, 431)}
+
+
+def flip_rows(system, cn0_db, trials, flip, rng, prn=1):
+    """Packed one-bit rows [trials][2P/8] of testutil.rs:60-81 synth_signal_cn0: code delayed by
+    t * step mod P samples on a cos carrier at 100 kHz (phase 0), amplitude 2 sqrt(C/N0 / fs),
+    unit Gaussian noise; with `flip` the sign flips at the code epoch between the two periods,
+    absolute sample P + (P - delay) mod P.  Quantized like quantize_bipolar_demean
+    (simulator.rs:237-244: v >= 0 -> +1; the plan de-means the record)."""
+    _, P, _, _ = SPECS[system]
+    n = 2 * P
+    code = api.code_resampled(system, prn, float(P), n)
+    a = 2.0 * math.sqrt(10.0 ** (cn0_db / 10.0) / FS)
+    i = np.arange(n)
+    carrier = a * np.cos(2.0 * math.pi * F_CARRIER * i / FS)
+    delays = (np.arange(trials) * E2E[system][2]) % P
+    rows = np.empty((trials, n // 8), dtype=np.uint8)
+    for t in range(trials):
+        d = int(delays[t])
+        cont = code[(i + d) % P] * carrier
+        if flip:
+            cont[P + (P - d) % P:] *= -1.0
+        cont += rng.standard_normal(n)
+        rows[t] = synth.pack_1bit(cont.astype(np.float32))
+    return rows, (P - delays) % P
+
+
+def end_to_end(system, cn0_db=45.0, flip=False, trials=None, seed=None):
+    """Median ACR C/N0 estimate of the reference's end-to-end test: per trial r_A =
+    (signal_strength / second_peak)^2 of a single-bin search (PRN 1 at 100 kHz, every trial one
+    antenna row of one plan) through estimate_cn0 with the system's table; trials whose ratio
+    falls outside the table are dropped.  Returns (median, estimates used, detection rate).  The
+    noise is numpy's, not the reference's xorshift stream."""
+    name, P, _, _ = SPECS[system]
+    if trials is None:
+        trials = E2E[system][1 if flip else 0]
+    rng = np.random.default_rng(seed if seed is not None else (P + 7 * int(flip)))
+    rows, want = flip_rows(system, cn0_db, trials, flip, rng)
+    plan = api.Plan(FS, trials, 2 * P, [api.SystemSearch(system, [F_CARRIER], [1])],
+                    input_format=1)
+    pk = plan.search(rows)[0][:, 0]
+    plan.close()
+    r_a = (pk["best_peak"].astype(np.float64) / pk["second_peak"].astype(np.float64)) ** 2
+    ests = [e for e in (api.estimate_cn0(float(r), name) for r in r_a) if e is not None]
+    found = float(np.mean(pk["best_codephase"].astype(np.int64) % P == want))
+    return (float(np.median(ests)) if ests else None), len(ests), found
+
+
 def generate(system, trials=100, seed=2026, step=0.5):
     """[(r_A, C/N0)] over the reference's C/N0 grid for `system`."""
     name, P, lo, hi = SPECS[system]

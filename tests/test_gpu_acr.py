@@ -39,3 +39,19 @@ def test_generated_table_is_monotonic(gpu):
         r_a, _, plan = acr_tables.ratio("gps", cn0, 100, rng, plan)
         assert r_a > prev
         prev = r_a
+
+
+# The reference's end-to-end estimate tests on all four calibrated systems, both hypotheses:
+# no flip -> median within +-1 dB of the injected 45 dB-Hz; a sign flip at the code epoch
+# ([c,-c] wins) -> median in [-4, -0.5] dB below it (the flipped window is biased low by 0 to
+# -6 dB depending on where the flip lands, median -2.5 dB).
+@pytest.mark.parametrize("flip", [False, True])
+@pytest.mark.parametrize("system", ["gps", "galileo", "beidou", "l1c"])
+def test_cn0_end_to_end_with_and_without_flip(gpu, system, flip):
+    med, n, found = acr_tables.end_to_end(system, 45.0, flip)
+    lo, hi = (-4.0, -0.5) if flip else (-1.0, 1.0)
+    print(f"{system} flip={flip}: median {med} from {n} estimates, detected {found:.2f}",
+          flush=True)
+    assert n > 0, (system, flip)
+    assert lo <= med - 45.0 <= hi, (system, flip, med, n)
+

@@ -118,13 +118,16 @@ def test_acquire_recovers_delay(gpu, system, prn, fs, period, delay, seed):
     assert r.frequency == ref["frequency"]
 
 
-@pytest.mark.parametrize("system,prn,fs,period,seed,w", [
-    ("gps", 1, 2.046e6, 2046, 2, 3.3), ("sbas", 121, 1.023e6, 1023, 42, 2.1),
-    ("galileo", 1, 1.023e6, 4092, 12, 2.7), ("beidou", 1, 1.023e6, 10230, 22, 1.9)])
-def test_acquire_contrast(gpu, system, prn, fs, period, seed, w):
+# (system, PRN, fs, period, delay, seed, noise-tone step): acquisition.rs, sbas.rs, galileo.rs,
+# beidou.rs noise-contrast tests, qzss.rs:473-489 and l1c.rs:377-392
+@pytest.mark.parametrize("system,prn,fs,period,delay,seed,w", [
+    ("gps", 1, 2.046e6, 2046, 123, 2, 3.3), ("sbas", 121, 1.023e6, 1023, 123, 42, 2.1),
+    ("galileo", 1, 1.023e6, 4092, 123, 12, 2.7), ("beidou", 1, 1.023e6, 10230, 123, 22, 1.9),
+    ("qzss", 184, 1.023e6, 1023, 300, 52, 3.9), ("l1c", 1, 1.023e6, 10230, 700, 32, 1.3)])
+def test_acquire_contrast(gpu, system, prn, fs, period, delay, seed, w):
     code = ocodes.code_resampled(system, prn, float(period), 2 * period)
     n = len(code)
-    inj = synth.synth_signal(code, period, 123, 0.0, fs, 0.05, seed)
+    inj = synth.synth_signal(code, period, delay, 0.0, fs, 0.05, seed)
     noise = (np.sin(np.arange(n) * w) * 0.05).astype(np.float32)
     pp = synth.phasepoints(fs, n)
     a = ACQ[system](inj, pp, fs, 0.0, 3000.0, prn, period)
@@ -132,15 +135,19 @@ def test_acquire_contrast(gpu, system, prn, fs, period, seed, w):
     assert a.signal_strength > 10.0 * b.signal_strength
 
 
-@pytest.mark.parametrize("system,prn,fs,period", [("gps", 1, 2.046e6, 2046),
-                                                  ("sbas", 120, 1.023e6, 1023),
-                                                  ("galileo", 1, 1.023e6, 4092)])
-def test_sweep_around_period_boundary(gpu, system, prn, fs, period):
-    """acquisition.rs:486-515 and siblings: lengths P-3..P+12 and 2P-3..2P+4."""
+@pytest.mark.parametrize("system,prn,fs,period,delay", [("gps", 1, 2.046e6, 2046, 100),
+                                                        ("sbas", 120, 1.023e6, 1023, 100),
+                                                        ("qzss", 184, 1.023e6, 1023, 100),
+                                                        ("galileo", 1, 1.023e6, 4092, 100),
+                                                        ("l1c", 1, 1.023e6, 10230, 1000)])
+def test_sweep_around_period_boundary(gpu, system, prn, fs, period, delay):
+    """acquisition.rs:486-515 and siblings (qzss.rs:491-520, l1c.rs:394-423): lengths
+    P-3..P+12 and 2P-3..2P+4, several not whole multiples of the period."""
     lengths = list(range(period - 3, period + 13)) + list(range(2 * period - 3, 2 * period + 5))
     for n in lengths:
         code = ocodes.code_resampled(system, prn, float(period), n)
-        sig = code[(np.arange(n) + 100) % n].astype(np.float32)
+        assert len(code) == n
+        sig = code[(np.arange(n) + delay) % n].astype(np.float32)
         r = ACQ[system](sig, synth.phasepoints(fs, n), fs, 0.0, 3000.0, prn, period)
         assert 0.0 <= r.codephase_frac < 1.0, n
         assert r.signal_strength > 0.0, n
