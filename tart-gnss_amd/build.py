@@ -6,6 +6,11 @@ Steps: generate ``_build/gen/tables_gen.h`` from the data files in ``csrc/tables
 every ``csrc/*.cpp`` / ``csrc/*.hip`` with hipcc for gfx950 (in parallel), link
 ``tga/libtga.so``.  Host C++ is compiled with ``-ffp-contract=off`` so f64 arithmetic that
 must match the reference (grids, ACR interpolation, replica resampling) is never fused.
+
+``build(sanitize=True)`` (``python tart-gnss_amd/build.py --sanitize``) builds the same library
+with AddressSanitizer and UndefinedBehaviorSanitizer on the host code only (device code is
+unchanged) into ``_build/san/libtga.so``; ``tart-gnss_amd/sanitize.py`` runs the CPU tests
+against it.
 """
 import os
 import subprocess
@@ -72,14 +77,22 @@ def gen_tables():
             f.write(text)
 
 
-def compile_one(src, verbose):
-    obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+# host-only sanitizers: each -fsanitize= applies to the host compilation alone
+SANITIZE = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer", "-g"]
+SAN_DIR = os.path.join(BUILD, "san")
+
+
+def compile_one(src, verbose, sanitize=False):
+    obj = os.path.join(SAN_DIR if sanitize else BUILD, os.path.basename(src) + ".o")
+    extra = SANITIZE if sanitize else []
     if src.endswith(".hip"):
-        cmd = [HIPCC, "-x", "hip"] + COMMON + DEVICE + ["-c", src, "-o", obj]
+        cmd = [HIPCC, "-x", "hip"] + COMMON + DEVICE + extra + ["-c", src, "-o", obj]
     else:
-        cmd = [HIPCC, "-x", "c++"] + COMMON + ["-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",
-                                              "-I", "/opt/rocm/include",
-                                              "-c", src, "-o", obj]
+        cmd = [HIPCC, "-x", "c++"] + COMMON + extra + ["-ffp-contract=off",
+                                                      "-D__HIP_PLATFORM_AMD__",
+                                                      "-I", "/opt/rocm/include",
+                                                      "-c", src, "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -92,23 +105,26 @@ def compile_one(src, verbose):
     return obj
 
 
-def build(verbose=False):
-    os.makedirs(BUILD, exist_ok=True)
+def build(verbose=False, sanitize=False):
+    os.makedirs(SAN_DIR if sanitize else BUILD, exist_ok=True)
     gen_tables()
     srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC)
                   if f.endswith(".cpp") or f.endswith(".hip"))
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(lambda s: compile_one(s, verbose), srcs))
-    tmp = OUT + ".tmp"
+        objs = list(ex.map(lambda s: compile_one(s, verbose, sanitize), srcs))
+    out = os.path.join(SAN_DIR, "libtga.so") if sanitize else OUT
+    tmp = out + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    if sanitize:
+        cmd[1:1] = SANITIZE[:6]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed\n" + r.stdout + r.stderr)
-    os.replace(tmp, OUT)
-    return OUT
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, sanitize="--sanitize" in sys.argv))

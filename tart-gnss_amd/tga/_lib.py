@@ -3,7 +3,8 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libtga.so")
+# TGA_LIBRARY selects another build of the same library (the sanitizer build of build.py)
+LIB_PATH = os.environ.get("TGA_LIBRARY") or os.path.join(_HERE, "libtga.so")
 
 TGA_OK = 0
 ERRORS = {
