@@ -179,8 +179,18 @@ typedef struct tga_system_desc {
    * fused P = 16368 path (TGA_ERR_UNSUPPORTED otherwise); ACR C/N0 tables are not calibrated
    * for it. */
   int32_t nc_epochs;
-  int32_t pad;
+  /* TGA_REPLICA_REFERENCE (0): the reference's replica for the system.  TGA_REPLICA_E1C_BOC11
+   * (1), Galileo only, an extension with no counterpart in the reference (whose E1-C replica is
+   * the plain 4092-chip code, galileo.rs:112-122): the E1-C chips with the BOC(1,1) subcarrier,
+   * each chip c as the half-chips [+c, -c] (8184 per 4 ms period), the usual receiver model of
+   * the E1-C pilot's CBOC(6,1,1/11) modulation.  Its secondary code (CS25, one chip per 4 ms
+   * period) needs no wipe inside an N = 2P search: the [c,c] / [c,-c] hypotheses cover both
+   * signs of the second period.  ACR C/N0 uses the E1-C table, calibrated for the plain replica
+   * on plain-chip signals. */
+  int32_t replica;
 } tga_system_desc;
+#define TGA_REPLICA_REFERENCE 0
+#define TGA_REPLICA_E1C_BOC11 1
 
 /* Arithmetic form of the correlation rows.
  *   TGA_PRECISION_FP32  the reference form: f32 spectra and f32 transforms, results within

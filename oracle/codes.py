@@ -203,6 +203,12 @@ def chips(system, prn):
     return SYSTEMS[system][0](prn)
 
 
+def e1c_boc11(prn):
+    """The BOC(1,1) E1-C replica of the plans' replica option (an extension: the reference
+    correlates E1-C with plain chips, galileo.rs:112-122): each E1-C chip as [+c, -c]."""
+    return boc11(e1c(prn))
+
+
 def resample(chip_seq, samples_per_code, num_samples):
     """acquisition.rs:146-159: c[n] = chips[floor(n / (spc / L)) mod L] in f64 arithmetic."""
     L = len(chip_seq)

@@ -193,6 +193,21 @@ std::vector<int8_t> code_chips(int system, int prn) {
   fail(TGA_ERR_INVALID_ARGUMENT, "unknown system");
 }
 
+std::vector<int8_t> replica_chips(int system, int prn, int replica) {
+  if (replica == TGA_REPLICA_REFERENCE) return code_chips(system, prn);
+  if (replica != TGA_REPLICA_E1C_BOC11)
+    fail(TGA_ERR_INVALID_ARGUMENT, "unknown replica " + std::to_string(replica));
+  if (system != TGA_SYS_GALILEO)
+    fail(TGA_ERR_UNSUPPORTED, "the BOC(1,1) replica is defined for Galileo E1-C only");
+  const std::vector<int8_t> c = code_chips(system, prn);
+  std::vector<int8_t> out(2 * c.size());
+  for (size_t i = 0; i < c.size(); ++i) {  // sine-phased subcarrier: + then - within a chip
+    out[2 * i] = c[i];
+    out[2 * i + 1] = static_cast<int8_t>(-c[i]);
+  }
+  return out;
+}
+
 void resample(const std::vector<int8_t>& chips, double samples_per_code, size_t n,
               double* out) {
   const size_t L = chips.size();

@@ -36,6 +36,9 @@ struct SystemInfo {
 const SystemInfo& system_info(int system);
 void check_prn(int system, int prn);
 std::vector<int8_t> code_chips(int system, int prn);
+// chips of a plan replica: code_chips for TGA_REPLICA_REFERENCE, the BOC(1,1) half-chips of
+// the E1-C code for TGA_REPLICA_E1C_BOC11 (Galileo only)
+std::vector<int8_t> replica_chips(int system, int prn, int replica);
 // c[n] = chips[floor(n / (spc / L)) mod L], f64 index arithmetic
 void resample(const std::vector<int8_t>& chips, double samples_per_code, size_t n, double* out);
 size_t samples_per_period(int system, double fs);

@@ -185,9 +185,13 @@ void Plan::build(const tga_plan_desc& d) {
     // replica of N samples (acquisition.rs:219); folding needs c[n+P] == c[n]
     std::vector<std::vector<double>> codes;
     bool fold = nc == 0 && N == 2 * P;
+    if (sd.replica != TGA_REPLICA_REFERENCE && sd.replica != TGA_REPLICA_E1C_BOC11)
+      fail(TGA_ERR_INVALID_ARGUMENT, "unknown replica " + std::to_string(sd.replica));
+    if (sd.replica == TGA_REPLICA_E1C_BOC11 && sd.system != TGA_SYS_GALILEO)
+      fail(TGA_ERR_UNSUPPORTED, "the BOC(1,1) replica is defined for Galileo E1-C only");
     for (int p : e.prns) {
       std::vector<double> c(N);
-      resample(code_chips(sd.system, p), static_cast<double>(P), N, c.data());
+      resample(replica_chips(sd.system, p, sd.replica), static_cast<double>(P), N, c.data());
       if (fold)
         for (size_t n = 0; n < P; ++n)
           if (c[n + P] != c[n]) { fold = false; break; }

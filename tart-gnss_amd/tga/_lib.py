@@ -46,7 +46,10 @@ class AcqResult(C.Structure):
 class SystemDesc(C.Structure):
     _fields_ = [("system", C.c_int32), ("n_prn", C.c_int32), ("prns", C.POINTER(C.c_int32)),
                 ("n_fc", C.c_int32), ("fc", C.POINTER(C.c_double)),
-                ("num_samples", C.c_int64), ("nc_epochs", C.c_int32), ("pad", C.c_int32)]
+                ("num_samples", C.c_int64), ("nc_epochs", C.c_int32), ("replica", C.c_int32)]
+
+
+REPLICAS = {"reference": 0, "boc11": 1}  # TGA_REPLICA_*
 
 
 class PlanDesc(C.Structure):

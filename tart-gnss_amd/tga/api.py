@@ -217,12 +217,15 @@ def estimate_cn0(r_a, table):
 class SystemSearch:
     """One system of a batched plan: PRNs (None = full range), absolute bin frequencies,
     coherent length (0 = two code periods, as the reference), and nc_epochs > 0 for the
-    non-coherent form (that many one-period epochs, powers summed)."""
+    non-coherent form (that many one-period epochs, powers summed).  replica "boc11" selects
+    the BOC(1,1) E1-C replica (Galileo only, TGA_REPLICA_E1C_BOC11) instead of the reference's
+    plain chips."""
     system: str
     fc: list
     prns: list = None
     num_samples: int = 0
     nc_epochs: int = 0
+    replica: str = "reference"
 
 
 PEAK_DTYPE = np.dtype([("best_peak", np.float32), ("second_peak", np.float32),
@@ -246,6 +249,7 @@ class Plan:
             descs[i].fc = _ptr(fc, C.c_double)
             descs[i].num_samples = int(s.num_samples)
             descs[i].nc_epochs = int(s.nc_epochs)
+            descs[i].replica = L.REPLICAS[s.replica]
             if s.prns is None:
                 descs[i].n_prn = 0
                 descs[i].prns = None

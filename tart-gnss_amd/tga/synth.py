@@ -128,12 +128,31 @@ def random_positions(n, diameter, rng):
     return np.stack([rr * np.cos(th), rr * np.sin(th), np.zeros(n)], axis=1)
 
 
+# Galileo E1-C secondary code CS25 (Galileo OS SIS ICD, E1-C secondary code "380AD90" hex, first
+# 25 bits), one chip per 4 ms primary period; bit 1 -> -1.
+E1C_CS25 = "0011100000001010110110010"
+
+
+def e1c_boc11_code(prn, samples_per_code, num_samples):
+    """E1-C chips with the BOC(1,1) subcarrier ([+c, -c] half-chips), resampled like the
+    reference's replicas: c[n] = half_chips[floor(n / (spc / 8184)) mod 8184]."""
+    chips = api.code_chips("galileo", prn).astype(np.float64)
+    half = np.empty(2 * len(chips))
+    half[0::2] = chips
+    half[1::2] = -chips
+    spchip = float(samples_per_code) / len(half)
+    idx = np.floor(np.arange(num_samples, dtype=np.float64) / spchip).astype(np.int64) % len(half)
+    return half[idx]
+
+
 def synth_observation(n_ant, L, fs, signals, fmt=1, seed=DEFAULT_SEED, intermediate=4.092e6,
                       tones=None, positions=None, threshold=1.0):
     """Packed observation [n_ant, row_bytes].
 
     signals: list of dicts {system, prn, cn0, doppler, delay (samples), phase (rad),
-             az, el (deg, for the per-antenna plane-wave carrier offset), flip (bool)}.
+             az, el (deg, for the per-antenna plane-wave carrier offset), flip (bool)};
+             Galileo only: boc (bool, BOC(1,1) subcarrier on the chips) and secondary (the
+             CS25 chip of the first period, 0..24: each later period carries the next chip).
     tones:   optional list per antenna of (frequency Hz, amplitude) CW interferers.
     """
     rng = np.random.default_rng(seed)
@@ -144,8 +163,15 @@ def synth_observation(n_ant, L, fs, signals, fmt=1, seed=DEFAULT_SEED, intermedi
     for s in signals:
         P = int(fs / 1000.0) if s["system"] in ("gps", "sbas", "qzss") else int(
             fs * {"galileo": 0.004, "beidou": 0.010, "l1c": 0.010}[s["system"]])
-        code = api.code_resampled(s["system"], s["prn"], float(P), P)
+        if s.get("boc"):
+            code = e1c_boc11_code(s["prn"], float(P), P)
+        else:
+            code = api.code_resampled(s["system"], s["prn"], float(P), P)
         c = code[(np.arange(L) + int(s["delay"])) % P]
+        if s.get("secondary") is not None:  # one CS25 chip per primary period
+            k = (np.arange(L) + int(s["delay"])) // P + int(s["secondary"])
+            sec = np.array([1.0 - 2.0 * int(b) for b in E1C_CS25])
+            c = c * sec[k % 25]
         if s.get("flip"):
             epoch = P + (P - int(s["delay"])) % P
             c = c.copy()

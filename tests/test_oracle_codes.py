@@ -128,3 +128,24 @@ def test_resample_integer_index():
     # a partial last period (acquisition.rs:474-484)
     for n in (1000, 1023, 2046, 5049):
         assert len(codes.resample(ch, 1000.0, n)) == n
+
+
+def test_e1c_boc11_is_the_plain_code_times_a_half_chip_square_wave():
+    # resampled at 16 samples per chip: the BOC(1,1) replica equals the plain E1-C replica times
+    # +1 on the first 8 samples of every chip and -1 on the last 8
+    from oracle import codes as oc
+    P = 65472
+    n = np.arange(2 * P)
+    sq = np.where((n % 16) < 8, 1.0, -1.0)
+    for prn in (1, 27, 50):
+        boc = oc.resample(oc.e1c_boc11(prn), float(P), 2 * P)
+        plain = oc.resample(oc.e1c(prn), float(P), 2 * P)
+        assert np.array_equal(boc, plain * sq), prn
+
+
+def test_synth_boc_code_matches_oracle_replica():
+    from oracle import codes as oc
+    from tga import synth
+    for prn in (3, 44):
+        got = synth.e1c_boc11_code(prn, 65472.0, 2 * 65472)
+        assert np.array_equal(got, oc.resample(oc.e1c_boc11(prn), 65472.0, 2 * 65472)), prn
