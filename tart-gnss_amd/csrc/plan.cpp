@@ -7,6 +7,7 @@
 // reference's common grid form one group of 100 PRNs).
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -47,7 +48,8 @@ std::string group_kernels(const Group& g, int slot) {
              : g.fp16  ? "k_rows16"
              : g.fused ? "k_rows"
              : g.big16 ? "k_big_pre<" + r0 + ",f16>+k_rows16<" + r0 + ">"
-             : g.big   ? "k_big_pre<" + r0 + ">+k_rows<0," + r0 + ">"
+             : g.big   ? "k_big_pre<" + r0 + ">+k_rows<" + std::to_string(g.rows_cfg.variant) +
+                             "," + r0 + ">"
                        : "k_mul_rows+k_pass+k_row_reduce";
     case 1:
       return g.fused ? "k_forward" : g.folded ? "k_fold_obs+k_pass" : "k_wipe_obs+k_pass";
@@ -310,8 +312,6 @@ void Plan::build(const tga_plan_desc& d) {
       rep_to_pfa(g, static_cast<long long>(n_prn) * 2 * static_cast<long long>(M));
       g.d_X.alloc(g.n_xrow * M * sizeof(float2));
       g.d_Xs.alloc(g.n_xrow * M * sizeof(float2));
-      const char* pin_sv = std::getenv("TGA_SPLIT_VARIANT");
-      g.rows_cfg.variant = g.big16 ? 0 : pin_sv ? (std::atoi(pin_sv) != 0 ? 1 : 0) : 1;
       upload(g.d_tw, fused_twiddles(), stream);
       upload(g.d_twb, big_twiddles(g.P), stream);
       // sub-rows of a chunk of rows per pre-pass / transform pair, on the plan's stream (a
@@ -337,6 +337,7 @@ void Plan::build(const tga_plan_desc& d) {
       g.d_win.alloc(nw * sizeof(int));
       g.d_Zw.alloc(nw * M * sizeof(float2));
       g.d_Zws.alloc(nw * M * sizeof(float2));
+      autotune_split(g);
     } else {
       g.d_X.alloc(g.n_xrow * M * sizeof(float2));
       g.d_Xs.alloc(g.n_xrow * M * sizeof(float2));
@@ -429,6 +430,78 @@ void Plan::autotune(Group& g) {
     else
       launch_fused_rows(g.d_X.as<float2>(), g.d_rep.as<float2>(), g.d_tw.as<float2>(), n_ant,
                         g.n_bin, g.n_prn, g.cw, g.d_rows.as<RowOut>(), stream, c);
+  };
+  hipEvent_t e0, e1;
+  TGA_HIP(hipEventCreate(&e0));
+  TGA_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  for (const FusedRowsConfig& c : cands) {
+    launch(c);
+    TGA_HIP(hipEventRecord(e0, stream));
+    launch(c);
+    TGA_HIP(hipEventRecord(e1, stream));
+    TGA_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    TGA_HIP(hipEventElapsedTime(&ms, e0, e1));
+    g.tune.push_back({c, static_cast<double>(ms)});
+    if (ms < best) {
+      best = ms;
+      g.rows_cfg = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+}
+
+// Split-path autotuner: times the first chunk of rows (pre-pass + sub-row transforms, after one
+// warm-up) for every candidate on the group's real sizes, with zeroed spectra, and keeps the
+// fastest.  Candidates: the radix-31 implementation of the FP32 sub-row transform (VALU codelet
+// or matrix cores with the next sub-row prefetched; the FP16 form has one) and the tile shape
+// of the row order (TX forward spectra x TP replica spectra whose rows run together, so their
+// spectra are fetched once into L2).  TGA_SPLIT_VARIANT=0|1 and TGA_SPLIT_TILE=TXxTP pin them.
+void Plan::autotune_split(Group& g) {
+  const char* pin_v = std::getenv("TGA_SPLIT_VARIANT");
+  const char* pin_t = std::getenv("TGA_SPLIT_TILE");
+  std::vector<int> variants;
+  if (g.big16)
+    variants = {0};
+  else if (pin_v)
+    variants = {std::atoi(pin_v) != 0 ? 1 : 0};
+  else
+    variants = {1, 0};
+  std::vector<int> tiles;
+  if (pin_t) {
+    int tx = 0, tp = 0;
+    if (std::sscanf(pin_t, "%dx%d", &tx, &tp) != 2 || tx < 1 || tp < 1 || tx > 4096 ||
+        tp > 4096)
+      fail(TGA_ERR_INVALID_ARGUMENT, "TGA_SPLIT_TILE must read TXxTP (e.g. 8x8)");
+    tiles = {tx | (tp << 16)};
+  } else {
+    for (auto [tx, tp] : {std::pair{8, 8}, {4, 16}, {16, 4}, {2, 32}, {32, 2}})
+      tiles.push_back(tx | (tp << 16));
+  }
+  std::vector<FusedRowsConfig> cands;
+  for (int v : variants)
+    for (int t : tiles) {
+      FusedRowsConfig c;
+      c.variant = v;
+      c.groups = 1;
+      c.tile = t;
+      cands.push_back(c);
+    }
+  g.rows_cfg = cands[0];
+  g.tune.clear();
+  if (cands.size() == 1) return;
+  TGA_HIP(hipMemsetAsync(g.d_X.p, 0, g.d_X.bytes, stream));
+  const int hs = g.folded ? 1 : 2;
+  const long long r1 = std::min(g.n_rows / hs, g.chunk_rows);
+  auto launch = [&](const FusedRowsConfig& c) {
+    launch_big_pre(g.d_X.as<float2>(), g.d_rep.as<float2>(), g.d_twb.as<float2>(), g.big_r0,
+                   g.n_prn, g.n_bin, n_ant, hs, c.tile, 0, r1, g.d_Y.p, g.big16,
+                   g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(), stream);
+    launch_big_sub(g.d_tw.as<float2>(), g.big_r0, g.n_prn, g.n_bin, n_ant, hs, c.tile, 0, r1,
+                   g.d_Y.p, g.big16, g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(),
+                   g.d_rows.as<RowOut>(), stream, c.variant);
   };
   hipEvent_t e0, e1;
   TGA_HIP(hipEventCreate(&e0));
@@ -568,9 +641,10 @@ void Plan::search(const uint8_t* d_in, hipStream_t st, float* dump) {
       for (long long r0 = 0; r0 < n_enum; r0 += g.chunk_rows) {
         const long long r1 = std::min(n_enum, r0 + g.chunk_rows);
         launch_big_pre(g.X, g.d_rep.as<float2>(), g.d_twb.as<float2>(), g.big_r0, g.n_prn,
-                       g.n_bin, n_ant, hs, r0, r1, g.d_Y.p, g.big16,
+                       g.n_bin, n_ant, hs, g.rows_cfg.tile, r0, r1, g.d_Y.p, g.big16,
                        g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(), st);
-        launch_big_sub(g.d_tw.as<float2>(), g.big_r0, g.n_prn, g.n_bin, n_ant, hs, r0, r1,
+        launch_big_sub(g.d_tw.as<float2>(), g.big_r0, g.n_prn, g.n_bin, n_ant, hs,
+                       g.rows_cfg.tile, r0, r1,
                        g.d_Y.p, g.big16, g.d_rowkey.as<unsigned long long>(),
                        g.d_rownan.as<int>(), g.d_rows.as<RowOut>(), st, g.rows_cfg.variant,
                        dump ? dump + g.dump_off : nullptr);
@@ -680,10 +754,21 @@ std::string Plan::describe() const {
                           : (g.folded ? "generic-folded" : "generic-unfolded"),
                   g.rows_cfg.variant, g.rows_cfg.groups);
     s += buf;
-    // every candidate the autotuner timed: v<variant>g<groups>:<ms>
+    // every candidate the autotuner timed: v<variant>g<groups>:<ms> (fused groups) or
+    // v<variant>t<TX>x<TP>:<ms> (split groups)
     for (size_t k = 0; k < g.tune.size(); ++k) {
-      std::snprintf(buf, sizeof buf, "%sv%dg%d:%.3f", k ? "," : "", g.tune[k].first.variant,
-                    g.tune[k].first.groups, g.tune[k].second);
+      const FusedRowsConfig& c = g.tune[k].first;
+      if (g.big)
+        std::snprintf(buf, sizeof buf, "%sv%dt%dx%d:%.3f", k ? "," : "", c.variant,
+                      c.tile & 0xffff, c.tile >> 16, g.tune[k].second);
+      else
+        std::snprintf(buf, sizeof buf, "%sv%dg%d:%.3f", k ? "," : "", c.variant, c.groups,
+                      g.tune[k].second);
+      s += buf;
+    }
+    if (g.big) {
+      std::snprintf(buf, sizeof buf, " tile=%dx%d", g.rows_cfg.tile & 0xffff,
+                    g.rows_cfg.tile >> 16);
       s += buf;
     }
     s += g.tune.empty() ? "-\n" : "\n";

@@ -100,6 +100,7 @@ struct Plan {
   void time_end(int slot, hipStream_t st, double bytes);
   void time_collect();
   void autotune(Group& g);
+  void autotune_split(Group& g);
   void rep_to_pfa(Group& g, long long n);
 };
 

@@ -57,6 +57,7 @@ void launch_to_pfa(const float2* in, float2* out, long long n, hipStream_t st);
 struct FusedRowsConfig {
   int variant = 0;
   int groups = 8;
+  int tile = 8 | (8 << 16);  // split groups: tile shape of the row order, TX | TP << 16
 };
 // rows[((ant*n_prn + prn)*n_bin + bin)*2 + h]
 // dump (surface form, here and in launch_fused_rows16 / launch_big_sub): when not null, every
@@ -149,13 +150,14 @@ std::vector<float2> big_twiddles(size_t P);
 // f16: Y holds f16 (re, im) pairs scaled by kSplitF16Scale (uint32 per value) and the sub-rows
 // run through the matrix-core rows kernel, whose powers then carry kSplitF16Scale^2; otherwise
 // Y holds float2 and the f32 rows kernel transforms them.
+// tile: TX | TP << 16, rows enumerated in tiles of TX forward spectra x TP replica spectra
 void launch_big_pre(const float2* X, const float2* rep, const float2* twb, int R0, int n_prn,
-                    int n_bin, int n_ant, int hs, long long r0, long long r1, void* Y, bool f16,
-                    unsigned long long* rowkey, int* rownan, hipStream_t st);
+                    int n_bin, int n_ant, int hs, int tile, long long r0, long long r1, void* Y,
+                    bool f16, unsigned long long* rowkey, int* rownan, hipStream_t st);
 // variant: the radix-31 implementation of the FP32 sub-row transforms (0 VALU codelet, 1 matrix
 // cores with the next sub-row prefetched into registers); the FP16 form has one.
 void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, int hs,
-                    long long r0, long long r1, const void* Y, bool f16,
+                    int tile, long long r0, long long r1, const void* Y, bool f16,
                     unsigned long long* rowkey, int* rownan, RowOut* rows, hipStream_t st,
                     int variant, float* dump = nullptr);
 // surface form: v[i] = (sqrt(v[i]) * gain) * scale in place (the row values of k_select)

@@ -128,42 +128,42 @@ __device__ __forceinline__ float block_max(float v, float* sp) {
   return v;
 }
 
-// Long rows are enumerated in tiles of kTileX xrows x kTileP PRNs, xrow fastest inside a
-// tile, with xrow = (ant * n_bin + bin) * 2 + h taken h = 0 first, then h = 1; hs = 2
-// enumerates h = 0 only (unfolded rows, whose two hypotheses coincide).  The rows of a tile share kTileX forward spectra and kTileP
-// replica spectra, which stay in the caches while the tile passes, so the pre-pass does not
-// fetch one replica spectrum from memory per row.  Tiles at the end of the xrow or PRN range
-// are narrower.
-constexpr int kTileX = 8, kTileP = 8;
+// Long rows are enumerated in tiles of TX xrows x TP PRNs (tile = TX | TP << 16, picked by the
+// autotuner), xrow fastest inside a tile, with xrow = (ant * n_bin + bin) * 2 + h taken h = 0
+// first, then h = 1; hs = 2 enumerates h = 0 only (unfolded rows, whose two hypotheses
+// coincide).  The rows of a tile share TX forward spectra and TP replica spectra, which stay in
+// the caches while the tile passes, so the pre-pass does not fetch one replica spectrum from
+// memory per row.  Tiles at the end of the xrow or PRN range are narrower.
 struct BigItem {
   long long xrow;
   int prn;
 };
 __device__ __forceinline__ BigItem big_item(long long I, int n_prn, int n_bin, int n_ant,
-                                            int hs) {
+                                            int hs, int tile) {
+  const int tx = tile & 0xffff, tp = tile >> 16;
   const long long n_xe = static_cast<long long>(n_ant) * n_bin * 2 / hs;  // enumerated xrows
-  const long long blk = static_cast<long long>(kTileX) * n_prn;           // rows per xrow block
+  const long long blk = static_cast<long long>(tx) * n_prn;               // rows per xrow block
   const long long xb = I / blk;
   long long rem = I - xb * blk;
-  const long long left = n_xe - xb * kTileX;
-  const int a = static_cast<int>(left < kTileX ? left : kTileX);  // xrows in this block
-  const long long tile = static_cast<long long>(a) * kTileP;
-  const int pt = static_cast<int>(rem / tile);
-  rem -= pt * tile;
+  const long long left = n_xe - xb * tx;
+  const int a = static_cast<int>(left < tx ? left : tx);  // xrows in this block
+  const long long tl = static_cast<long long>(a) * tp;
+  const int pt = static_cast<int>(rem / tl);
+  rem -= pt * tl;
   const int q = static_cast<int>(rem / a);
   // enumerated xrow xe -> xrow with h outermost (hs = 1), so a tile's xrows share h and with it
   // the replica spectra (prn, h) of its rows
-  const long long xe = xb * kTileX + (rem - static_cast<long long>(q) * a);
+  const long long xe = xb * tx + (rem - static_cast<long long>(q) * a);
   const long long n_ab = static_cast<long long>(n_ant) * n_bin;
   BigItem it;
   it.xrow = hs == 1 ? (xe >= n_ab ? (xe - n_ab) * 2 + 1 : xe * 2) : xe * 2;
-  it.prn = pt * kTileP + q;
+  it.prn = pt * tp + q;
   return it;
 }
 // the row index ((ant * n_prn + prn) * n_bin + bin) * 2 + h of enumerated row I
 __device__ __forceinline__ long long big_row(long long I, int n_prn, int n_bin, int n_ant,
-                                             int hs) {
-  const BigItem it = big_item(I, n_prn, n_bin, n_ant, hs);
+                                             int hs, int tile) {
+  const BigItem it = big_item(I, n_prn, n_bin, n_ant, hs, tile);
   const long long ant = it.xrow / (2 * n_bin);
   const int bh = static_cast<int>(it.xrow - ant * 2 * n_bin);
   return ((ant * n_prn + it.prn) * n_bin) * 2 + bh;
@@ -436,8 +436,9 @@ template <int R0, bool F16>
 __device__ __forceinline__ void big_pre_positions(const float2* __restrict__ X,
                                                   const float2* __restrict__ rep,
                                                   const float2* __restrict__ twb, int n_prn,
-                                                  int n_bin, int n_ant, int hs, long long r0,
-                                                  long long rr, int p_begin, int p_end, int lt,
+                                                  int n_bin, int n_ant, int hs, int tile,
+                                                  long long r0, long long rr, int p_begin,
+                                                  int p_end, int lt,
                                                   void* __restrict__ Yv,
                                                   unsigned long long* __restrict__ rowkey,
                                                   int* __restrict__ rownan) {
@@ -446,7 +447,7 @@ __device__ __forceinline__ void big_pre_positions(const float2* __restrict__ X,
   const int t = p_begin + 2 * lt;
   if (t >= p_end) return;
   const long long I = r0 + rr;  // tiled order (big_item)
-  const BigItem it = big_item(I, n_prn, n_bin, n_ant, hs);
+  const BigItem it = big_item(I, n_prn, n_bin, n_ant, hs, tile);
   const long long xrow = it.xrow;
   const int prn = it.prn;
   const int h = static_cast<int>(xrow & 1);
@@ -499,7 +500,7 @@ __device__ __forceinline__ void big_pre_positions(const float2* __restrict__ X,
     }
   }
   if (t == 0) {
-    const long long r = big_row(I, n_prn, n_bin, n_ant, hs);
+    const long long r = big_row(I, n_prn, n_bin, n_ant, hs, tile);
     rowkey[r] = 0ull;
     rownan[r] = 0;
   }
@@ -524,6 +525,7 @@ struct RowArgs {
   int epochs;  // non-coherent form: one-period epochs per (antenna, bin)
   int hs;      // sub-row mode: 2 when only h = 0 rows are enumerated (unfolded rows)
   int n_ant;   // sub-row mode: antennas (for the tiled row order)
+  int tile;    // sub-row mode: tile shape of the row order (big_item)
   // surface form (DUMP): every output power also goes to dump[row * period + sample], row the
   // RowOut index of the correlation row, period 16368 (rows) or SUB x 16368 (sub-rows)
   float* dump;
@@ -1117,7 +1119,7 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
     for (long long sidx = blockIdx.x; sidx < a.n_sub; sidx += gridDim.x) {
       const long long rq = sidx / SUB;
       const int k0 = static_cast<int>(sidx - rq * SUB);
-      const long long r = big_row(a.r0 + rq, a.n_prn, a.n_bin, a.n_ant, a.hs);
+      const long long r = big_row(a.r0 + rq, a.n_prn, a.n_bin, a.n_ant, a.hs, a.tile);
       const __amdgpu_buffer_rsrc_t yv = buffer_view(a.Y + sidx * P, P * 8);
       float2 yc[kPrefetch ? 33 : 1];
       if constexpr (kPrefetch) {
@@ -1182,6 +1184,7 @@ struct Row16Args {
   int* rownan = nullptr;
   int hs = 1;
   int n_ant = 0;  // sub-row mode: antennas (for the tiled row order)
+  int tile = 0;   // sub-row mode: tile shape of the row order (big_item)
   long long n_items = 0;  // SUB == 0: (xrow, PRN group) items, n_xrow * groups
   float* dump = nullptr;  // surface form: as RowArgs::dump
 };
@@ -1310,7 +1313,7 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
         dbase = ((((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) * 2 + h)) * P;
       } else {
         const long long rq = it / SUB;
-        const long long r = big_row(a.r0 + rq, a.n_prn, a.n_bin, a.n_ant, a.hs);
+        const long long r = big_row(a.r0 + rq, a.n_prn, a.n_bin, a.n_ant, a.hs, a.tile);
         dbase = static_cast<size_t>(r) * (SUB * P) + static_cast<size_t>(it - rq * SUB);
         dstride = SUB;
       }
@@ -1542,7 +1545,7 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
       // sub-row it of the chunk: row r0 + it / SUB, output samples k0 + SUB m (k0 = it mod SUB)
       const long long rq = it / SUB;
       const int k0 = static_cast<int>(it - rq * SUB);
-      const long long r = big_row(a.r0 + rq, a.n_prn, a.n_bin, a.n_ant, a.hs);
+      const long long r = big_row(a.r0 + rq, a.n_prn, a.n_bin, a.n_ant, a.hs, a.tile);
       row_finish_key<T16>(km, (km >> 32) >= 0x7f800000ull, red, k0, SUB, a.rowkey + r,
                           a.rownan + r);
     }
@@ -1716,8 +1719,8 @@ template <int R0, bool F16>
 __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
                                                  const float2* __restrict__ rep,
                                                  const float2* __restrict__ twb, int n_prn,
-                                                 int n_bin, int n_ant, int hs, long long r0,
-                                                 long long n_rows,
+                                                 int n_bin, int n_ant, int hs, int tile,
+                                                 long long r0, long long n_rows,
                                                  void* __restrict__ Yv,
                                                  unsigned long long* __restrict__ rowkey,
                                                  int* __restrict__ rownan) {
@@ -1725,7 +1728,7 @@ __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
   if (rr >= n_rows) return;
   const int chunk = 4 * static_cast<int>(blockIdx.x & 7) + static_cast<int>((blockIdx.x >> 3) & 3);
   const int p0 = 512 * chunk;
-  fused::big_pre_positions<R0, F16>(X, rep, twb, n_prn, n_bin, n_ant, hs, r0, rr, p0,
+  fused::big_pre_positions<R0, F16>(X, rep, twb, n_prn, n_bin, n_ant, hs, tile, r0, rr, p0,
                              p0 + 512 < kFusedP ? p0 + 512 : kFusedP,
                              static_cast<int>(threadIdx.x), Yv, rowkey, rownan);
 }
@@ -1733,11 +1736,11 @@ __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
 // Row keys of long rows -> RowOut (p2 is filled later from the winning row only).
 __global__ void k_big_rowout(const unsigned long long* __restrict__ rowkey,
                              const int* __restrict__ rownan, long long r0, long long n_rows,
-                             int n_prn, int n_bin, int n_ant, int hs,
+                             int n_prn, int n_bin, int n_ant, int hs, int tile,
                              RowOut* __restrict__ rows) {
   const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
   if (i >= n_rows) return;
-  const long long r = big_row(r0 + i, n_prn, n_bin, n_ant, hs);
+  const long long r = big_row(r0 + i, n_prn, n_bin, n_ant, hs, tile);
   const unsigned long long k = rowkey[r];
   RowOut o;
   o.p1 = __uint_as_float(static_cast<unsigned int>(k >> 32));
@@ -2083,30 +2086,30 @@ std::vector<float2> big_twiddles(size_t P) {
 }
 
 void launch_big_pre(const float2* X, const float2* rep, const float2* twb, int R0, int n_prn,
-                    int n_bin, int n_ant, int hs, long long r0, long long r1, void* Y, bool f16,
-                    unsigned long long* rowkey, int* rownan, hipStream_t st) {
+                    int n_bin, int n_ant, int hs, int tile, long long r0, long long r1, void* Y,
+                    bool f16, unsigned long long* rowkey, int* rownan, hipStream_t st) {
   const long long nr = r1 - r0;
   const long long total = nr;  // rows of this chunk, 32 blocks each
   const dim3 grid(static_cast<unsigned>(nr * 32));
   if (R0 == 4 && !f16)
     hipLaunchKernelGGL((k_big_pre<4, false>), grid, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
-                       n_ant, hs, r0, total, Y, rowkey, rownan);
+                       n_ant, hs, tile, r0, total, Y, rowkey, rownan);
   else if (R0 == 4)
     hipLaunchKernelGGL((k_big_pre<4, true>), grid, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
-                       n_ant, hs, r0, total, Y, rowkey, rownan);
+                       n_ant, hs, tile, r0, total, Y, rowkey, rownan);
   else if (R0 == 10 && !f16)
     hipLaunchKernelGGL((k_big_pre<10, false>), grid, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
-                       n_ant, hs, r0, total, Y, rowkey, rownan);
+                       n_ant, hs, tile, r0, total, Y, rowkey, rownan);
   else if (R0 == 10)
     hipLaunchKernelGGL((k_big_pre<10, true>), grid, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
-                       n_ant, hs, r0, total, Y, rowkey, rownan);
+                       n_ant, hs, tile, r0, total, Y, rowkey, rownan);
   else
     fail(TGA_ERR_UNSUPPORTED, "long-period split supports 4 and 10 sub-rows");
   TGA_HIP(hipGetLastError());
 }
 
-void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, int hs, long long r0,
-                    long long r1, const void* Y, bool f16, unsigned long long* rowkey,
+void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, int hs, int tile,
+                    long long r0, long long r1, const void* Y, bool f16, unsigned long long* rowkey,
                     int* rownan, RowOut* rows, hipStream_t st, int variant, float* dump) {
   const long long nr = r1 - r0;
   const long long n_sub = nr * R0;
@@ -2125,6 +2128,7 @@ void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, i
     a.rownan = rownan;
     a.hs = hs;
     a.n_ant = n_ant;
+    a.tile = tile;
     a.dump = dump;
     if (dump && R0 == 4)
       hipLaunchKernelGGL((fused::k_rows16<4, true>), dim3(blocks), dim3(fused::T16), 0, st, a);
@@ -2138,6 +2142,7 @@ void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, i
     fused::RowArgs a{};
     a.hs = hs;
     a.n_ant = n_ant;
+    a.tile = tile;
     a.tw = tw;
     a.cw = 0;
     a.n_prn = n_prn;
@@ -2166,7 +2171,7 @@ void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, i
   }
   TGA_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_big_rowout, dim3(grid_for(nr, 256)), dim3(256), 0, st, rowkey, rownan,
-                     r0, nr, n_prn, n_bin, n_ant, hs, rows);
+                     r0, nr, n_prn, n_bin, n_ant, hs, tile, rows);
   TGA_HIP(hipGetLastError());
 }
 

@@ -212,6 +212,36 @@ def test_split_row_tiles_ragged(gpu, split_variant):
     assert int(peaks[0][0, 8]["best_codephase"]) == (65472 - 31000) % 65472
 
 
+def test_split_tile_shapes_give_identical_peaks(gpu, monkeypatch):
+    # the tile shape only reorders the long rows (and which spectra share the caches): every
+    # shape, square, skewed either way and ragged against 10 PRNs x 10 (bin, h) spectra, gives
+    # bit-identical peaks, matching the oracle
+    fs, L = FS, 2 * 65472
+    signals = [dict(system="galileo", prn=4, cn0=52.0, doppler=500.0, delay=1234)]
+    fc = grid(IF, 1000.0, 500.0)
+    systems = [("galileo", list(range(1, 11)), fc)]
+    packed = synth.synth_observation(2, L, fs, signals, fmt=1, seed=29)
+    base = None
+    # one radix-31 implementation throughout (the two round differently)
+    monkeypatch.setenv("TGA_SPLIT_VARIANT", "1")
+    for t in ("8x8", "2x32", "32x2", "3x7", "1x1"):
+        monkeypatch.setenv("TGA_SPLIT_TILE", t)
+        plan = tga.Plan(fs, 2, L, [tga.SystemSearch(n, f, p) for n, p, f in systems])
+        assert f"tile={t}" in plan.describe()
+        pk = plan.search(packed)[0]
+        if base is None:
+            base = pk
+            assert check_against_oracle(plan, [pk], packed, fs, L, 1, systems) == 20
+        for f in pk.dtype.names:
+            assert np.array_equal(pk[f], base[f]), (t, f)
+    # unpinned, the autotuner times both radix-31 implementations x 5 shapes
+    monkeypatch.delenv("TGA_SPLIT_TILE")
+    monkeypatch.delenv("TGA_SPLIT_VARIANT")
+    desc = tga.Plan(fs, 2, L, [tga.SystemSearch(n, f, p) for n, p, f in systems]).describe()
+    for c in ("v1t8x8:", "v1t4x16:", "v0t16x4:", "v0t2x32:", "v1t32x2:"):
+        assert c in desc, desc
+
+
 @pytest.mark.parametrize("system,prn", [("beidou", 19), ("l1c", 7)])
 def test_split_b1c_l1c_at_16368khz(gpu, split_variant, system, prn):
     # B1C / L1C at 16.368 MHz: P = 163680 = 10 x 16368 (BOC(1,1) half-chips)
