@@ -318,7 +318,7 @@ void Plan::build(const tga_plan_desc& d) {
       // second stream for the pre-pass of the next chunk measured no faster: the two kernels
       // then slow each other down).  Larger chunks amortise each launch pair's ramp and tail
       // (per all24 search: 1000 ms at 2 GiB, 894 ms at 16 GiB, 885 ms at 32 GiB of sub-rows),
-      // so a chunk holds up to 16 GiB, at most an eighth of the free device memory;
+      // so a chunk holds up to 32 GiB, at most an eighth of the free device memory;
       // TGA_SPLIT_CHUNK_MB sets it explicitly.
       const size_t y_elem = g.big16 ? sizeof(uint32_t) : sizeof(float2);
       const char* env_mb = std::getenv("TGA_SPLIT_CHUNK_MB");
@@ -326,7 +326,7 @@ void Plan::build(const tga_plan_desc& d) {
       TGA_HIP(hipMemGetInfo(&free_b, &total_b));
       const long long chunk_bytes =
           env_mb ? std::max(1, std::atoi(env_mb)) * (1LL << 20)
-                 : std::max<long long>(256LL << 20, std::min<long long>(16LL << 30, free_b / 8));
+                 : std::max<long long>(256LL << 20, std::min<long long>(32LL << 30, free_b / 8));
       const long long cap = std::max<long long>(
           1, chunk_bytes / (static_cast<long long>(M) * static_cast<long long>(y_elem)));
       g.chunk_rows = std::min<long long>(cap, g.n_rows);
