@@ -48,6 +48,7 @@ std::string group_kernels(const Group& g, int slot) {
              : g.fp16  ? "k_rows16"
              : g.fused ? "k_rows"
              : g.big16 ? "k_big_pre<" + r0 + ",f16>+k_rows16<" + r0 + ">"
+             : g.natural() ? "k_4s_pre<" + r0 + ">+k_4s_rows<" + r0 + ">"
              : g.big   ? "k_big_pre<" + r0 + ">+k_rows<" + std::to_string(g.rows_cfg.variant) +
                              "," + r0 + ">"
                        : "k_mul_rows+k_pass+k_row_reduce";
@@ -308,12 +309,14 @@ void Plan::build(const tga_plan_desc& d) {
       upload(g.d_tw, fused_twiddles(), stream);
       autotune(g);
     } else if (g.big) {
-      // long rows read their spectra segment by segment in the prime-factor order of 16368
-      rep_to_pfa(g, static_cast<long long>(n_prn) * 2 * static_cast<long long>(M));
       g.d_X.alloc(g.n_xrow * M * sizeof(float2));
       g.d_Xs.alloc(g.n_xrow * M * sizeof(float2));
       upload(g.d_tw, fused_twiddles(), stream);
       upload(g.d_twb, big_twiddles(g.P), stream);
+      if (!g.big16) {
+        upload(g.d_tw4, four_step_twiddles(g.P), stream);
+        upload(g.d_c31, radix31_table(), stream);
+      }
       // sub-rows of a chunk of rows per pre-pass / transform pair, on the plan's stream (a
       // second stream for the pre-pass of the next chunk measured no faster: the two kernels
       // then slow each other down).  Larger chunks amortise each launch pair's ramp and tail
@@ -338,6 +341,10 @@ void Plan::build(const tga_plan_desc& d) {
       g.d_Zw.alloc(nw * M * sizeof(float2));
       g.d_Zws.alloc(nw * M * sizeof(float2));
       autotune_split(g);
+      // the sub-row forms read their spectra segment by segment in the prime-factor order of
+      // 16368; the four-step form reads them in natural order
+      if (!g.natural())
+        rep_to_pfa(g, static_cast<long long>(n_prn) * 2 * static_cast<long long>(M));
     } else {
       g.d_X.alloc(g.n_xrow * M * sizeof(float2));
       g.d_Xs.alloc(g.n_xrow * M * sizeof(float2));
@@ -453,12 +460,15 @@ void Plan::autotune(Group& g) {
   (void)hipEventDestroy(e1);
 }
 
-// Split-path autotuner: times the first chunk of rows (pre-pass + sub-row transforms, after one
+// Split-path autotuner: times the first chunk of rows (both kernels of the form, after one
 // warm-up) for every candidate on the group's real sizes, with zeroed spectra, and keeps the
-// fastest.  Candidates: the radix-31 implementation of the FP32 sub-row transform (VALU codelet
-// or matrix cores with the next sub-row prefetched; the FP16 form has one) and the tile shape
-// of the row order (TX forward spectra x TP replica spectra whose rows run together, so their
-// spectra are fetched once into L2).  TGA_SPLIT_VARIANT=0|1 and TGA_SPLIT_TILE=TXxTP pin them.
+// fastest.  Candidates: the FP32 form (variants 0 and 1: R0 sub-rows of 16368 through the LDS
+// transform with the VALU or the matrix-core radix 31, the next sub-row prefetched in 1;
+// variant 3: the four-step N1 x 496 form; the FP16 form has one) and the tile shape of the row
+// order (TX forward spectra x TP replica spectra whose rows run together, so their spectra are
+// fetched once into L2).  TGA_SPLIT_VARIANT=0|1|3 and TGA_SPLIT_TILE=TXxTP pin them.  The
+// choice fixes the spectra's layout: natural order for variant 3, else the segment-wise
+// prime-factor order (the replicas are reordered after tuning).
 void Plan::autotune_split(Group& g) {
   const char* pin_v = std::getenv("TGA_SPLIT_VARIANT");
   const char* pin_t = std::getenv("TGA_SPLIT_TILE");
@@ -466,9 +476,9 @@ void Plan::autotune_split(Group& g) {
   if (g.big16)
     variants = {0};
   else if (pin_v)
-    variants = {std::atoi(pin_v) != 0 ? 1 : 0};
+    variants = {std::atoi(pin_v) == 3 ? 3 : std::atoi(pin_v) != 0 ? 1 : 0};
   else
-    variants = {1, 0};
+    variants = {1, 0, 3};
   std::vector<int> tiles;
   if (pin_t) {
     int tx = 0, tp = 0;
@@ -483,6 +493,9 @@ void Plan::autotune_split(Group& g) {
   std::vector<FusedRowsConfig> cands;
   for (int v : variants)
     for (int t : tiles) {
+      // the four-step form is timed on two shapes unless pinned (its step 1 reads whole tiles of
+      // columns, its step 2 none: the shape matters less)
+      if (v == 3 && !pin_t && t != (8 | (8 << 16)) && t != (4 | (16 << 16))) continue;
       FusedRowsConfig c;
       c.variant = v;
       c.groups = 1;
@@ -496,6 +509,13 @@ void Plan::autotune_split(Group& g) {
   const int hs = g.folded ? 1 : 2;
   const long long r1 = std::min(g.n_rows / hs, g.chunk_rows);
   auto launch = [&](const FusedRowsConfig& c) {
+    if (c.variant == 3) {
+      launch_big_4s(g.d_X.as<float2>(), g.d_rep.as<float2>(), g.d_tw4.as<float2>(),
+                    g.d_c31.as<float2>(), g.big_r0, g.n_prn, g.n_bin, n_ant, hs, c.tile, 0, r1,
+                    g.d_Y.as<float2>(), g.d_rowkey.as<unsigned long long>(),
+                    g.d_rownan.as<int>(), g.d_rows.as<RowOut>(), stream);
+      return;
+    }
     launch_big_pre(g.d_X.as<float2>(), g.d_rep.as<float2>(), g.d_twb.as<float2>(), g.big_r0,
                    g.n_prn, g.n_bin, n_ant, hs, c.tile, 0, r1, g.d_Y.p, g.big16,
                    g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(), stream);
@@ -612,7 +632,7 @@ void Plan::search(const uint8_t* d_in, hipStream_t st, float* dump) {
         launch_wipe_obs(d_in, row_bytes, fmt, d_means.as<double>(), g.d_fc.as<double>(), n_ant,
                         g.n_bin, M, phase_const, g.d_X.as<float2>(), g.big16, st);
       g.X = fft_forward(fft_plan(g.M), g.d_X.as<float2>(), g.d_Xs.as<float2>(), g.n_xrow, st);
-      if (g.big) {  // segment-wise prime-factor order of the long rows
+      if (g.big && !g.natural()) {  // segment-wise prime-factor order of the long rows
         float2* dst = g.X == g.d_X.as<float2>() ? g.d_Xs.as<float2>() : g.d_X.as<float2>();
         launch_to_pfa(g.X, dst, static_cast<long long>(g.n_xrow) * M, st);
         g.X = dst;
@@ -638,6 +658,15 @@ void Plan::search(const uint8_t* d_in, hipStream_t st, float* dump) {
     } else if (g.big) {
       const int hs = g.folded ? 1 : 2;
       const long long n_enum = g.n_rows / hs;
+      if (g.natural()) {
+        for (long long r0 = 0; r0 < n_enum; r0 += g.chunk_rows)
+          launch_big_4s(g.X, g.d_rep.as<float2>(), g.d_tw4.as<float2>(), g.d_c31.as<float2>(),
+                        g.big_r0, g.n_prn, g.n_bin, n_ant, hs, g.rows_cfg.tile, r0,
+                        std::min(n_enum, r0 + g.chunk_rows), g.d_Y.as<float2>(),
+                        g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(),
+                        g.d_rows.as<RowOut>(), st, dump ? dump + g.dump_off : nullptr);
+        continue;
+      }
       for (long long r0 = 0; r0 < n_enum; r0 += g.chunk_rows) {
         const long long r1 = std::min(n_enum, r0 + g.chunk_rows);
         launch_big_pre(g.X, g.d_rep.as<float2>(), g.d_twb.as<float2>(), g.big_r0, g.n_prn,
@@ -675,7 +704,7 @@ void Plan::search(const uint8_t* d_in, hipStream_t st, float* dump) {
       launch_big_second(g.X, g.d_rep.as<float2>(), g.d_slots.as<PeakSlot>(),
                         static_cast<int>(g.M), static_cast<int>(g.P), g.cw, n_ant, g.n_prn,
                         g.n_bin, g.folded, g.d_win.as<int>(), fold, g.scale, g.d_Zw.as<float2>(),
-                        g.d_Zws.as<float2>(), d_peaks.as<tga_peak>(), st);
+                        g.d_Zws.as<float2>(), d_peaks.as<tga_peak>(), st, g.natural());
   }
   time_end(2, st, 0.0);
 }
@@ -749,6 +778,7 @@ std::string Plan::describe() const {
                   g.nc > 0 ? "fused16368-noncoherent"
                   : g.fp16 ? "fused16368-fp16"
                   : g.fused ? "fused16368"
+                  : g.natural() ? (g.big_r0 == 4 ? "split132x496" : "split330x496")
                   : g.big ? (g.big_r0 == 4 ? (g.big16 ? "split4x16368-fp16" : "split4x16368")
                                            : (g.big16 ? "split10x16368-fp16" : "split10x16368"))
                           : (g.folded ? "generic-folded" : "generic-unfolded"),

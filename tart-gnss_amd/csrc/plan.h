@@ -32,11 +32,14 @@ struct Group {
   DevBuf d_carrier;  // fused groups: carrier (cos, sin) per (bin, sample), built at plan creation
   DevBuf d_rep16, d_X16;  // FP16 form: f16 (re, im) pairs scaled by kF16Scale
   DevBuf d_Y, d_rowkey, d_rownan, d_twb, d_win, d_Zw, d_Zws;  // long-period split
+  DevBuf d_tw4, d_c31;  // four-step split: W_P^{n2 k1} as [k1][n2], radix-31 cos / sin
   float2* X = nullptr;  // where the forward spectra of the last search landed
   FusedRowsConfig rows_cfg;  // rows-kernel shape picked by the autotuner
   std::vector<std::pair<FusedRowsConfig, double>> tune;  // candidates timed, ms per launch
   double row_bytes_alg = 0, fwd_bytes_alg = 0;
   size_t dump_off = 0;  // surface form: first float of the group's rows in the staging buffer
+  // FP32 split groups on the four-step form (variant 3) keep their spectra in natural order
+  bool natural() const { return big && !big16 && rows_cfg.variant == 3; }
 };
 
 struct SysEntry {

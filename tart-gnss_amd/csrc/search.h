@@ -162,10 +162,22 @@ void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, i
                     int variant, float* dump = nullptr);
 // surface form: v[i] = (sqrt(v[i]) * gain) * scale in place (the row values of k_select)
 void launch_surface_finish(float* v, long long n, float gain, float scale, hipStream_t st);
-// second peaks of the winning rows; Z, Zs hold n_ant * n_prn * M complex values
+// second peaks of the winning rows; Z, Zs hold n_ant * n_prn * M complex values; natural: X and
+// rep in natural order (the four-step split) instead of the segment-wise prime-factor order
 void launch_big_second(const float2* X, const float2* rep, const PeakSlot* slots, int M, int P,
                        int cw, int n_ant, int n_prn, int n_bin, bool folded, const int* win,
                        float gain, float scale, float2* Z, float2* Zs, tga_peak* out,
-                       hipStream_t st);
+                       hipStream_t st, bool natural = false);
+// Four-step split (FP32, variant 3): long rows in natural order as N1 x 496 (N1 = R0 x 33):
+// k_4s_pre writes the N1-point column transforms, twiddled, as rows of 496 into Y ((r1 - r0) * P
+// values) and clears the rows' keys; k_4s_rows transforms the rows of 496 and folds each pair's
+// argmax into the row keys; then the rows' RowOut.  tw4 = four_step_twiddles(P), c31 =
+// radix31_table(); dump as launch_big_sub's.
+std::vector<float2> four_step_twiddles(size_t P);
+std::vector<float2> radix31_table();
+void launch_big_4s(const float2* X, const float2* rep, const float2* tw4, const float2* c31,
+                   int R0, int n_prn, int n_bin, int n_ant, int hs, int tile, long long r0,
+                   long long r1, float2* Y, unsigned long long* rowkey, int* rownan,
+                   RowOut* rows, hipStream_t st, float* dump = nullptr);
 
 }  // namespace tga

@@ -1593,6 +1593,239 @@ __global__ void k_to_f16(const float2* __restrict__ in, uint32_t* __restrict__ o
   out[i] = pack_f16(in[row * P + pfa_nat(pos)], s);
 }
 
+// ---- long rows as N1 x 496 (N1 = R0 x 33): the four-step split with small transforms ----------
+// A long row of P = R0 x 16368 = N1 x 496 points in natural order (folded conj(X) R) is
+//   Z[k1 + N1 k2] = sum_{n2} W_496^{n2 k2} . W_P^{n2 k1} . sum_{n1} z[496 n1 + n2] W_N1^{n1 k1}:
+//   step 1 (k_4s_pre)  per column n2 the N1-point DFT (prime-factor R0 x 33: n1 = (33 a + R0 b)
+//                      mod N1, k1 = (F1 ka + F2 kb) mod N1) and the twiddle W_P^{n2 k1},
+//                      written as rows Y[k1][n2] of 496 values;
+//   step 2 (k_4s_rows) per k1 the 496-point DFT (prime-factor 16 x 31: n2 = (31 a + 16 b)
+//                      mod 496, k2 = (465 ka + 32 kb) mod 496; radix 31 on the matrix cores as
+//                      k_rows<1>), |.|^2 and the argmax of sample k1 + N1 k2 folded into the long
+//                      row's key.
+// Both kernels hold a few KB of LDS per wave, so several workgroups share a CU; the sub-row
+// variants above need a whole CU's LDS per 16368-point transform.
+constexpr int k4N2 = 496;
+constexpr int k4Cols = 16;  // columns (consecutive n2) per step-1 block: 128-byte segments
+constexpr int k4TA = 256;   // step-1 threads
+constexpr int k4TB = 256;   // step-2 threads: four waves, each two k1 rows at a time
+
+template <int R0>
+struct FourStep {
+  static constexpr int N1 = R0 * 33;
+  static constexpr int PL = N1 * k4N2;
+  static constexpr int F1 = 33 * inv_mod(33 % R0, R0);
+  static constexpr int F2 = R0 * inv_mod(R0 % 33, 33);
+  static_assert(F1 % R0 == 1 && F1 % 33 == 0 && F2 % 33 == 1 && F2 % R0 == 0, "coprime");
+};
+
+// Lanes of one wave exchanging values through its own LDS region: every LDS access of the wave
+// issued so far has completed (and the compiler moves none across).
+__device__ __forceinline__ void wave_lds_sync() {
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <int R0>
+__global__ __launch_bounds__(k4TA) void k_4s_pre(const float2* __restrict__ X,
+                                                  const float2* __restrict__ rep,
+                                                  const float2* __restrict__ tw4, int n_prn,
+                                                  int n_bin, int n_ant, int hs, int tile,
+                                                  long long r0, long long n_rows,
+                                                  float2* __restrict__ Y,
+                                                  unsigned long long* __restrict__ rowkey,
+                                                  int* __restrict__ rownan) {
+  using FS = FourStep<R0>;
+  constexpr int N1 = FS::N1, PL = FS::PL;
+  constexpr int NCB = k4N2 / k4Cols;  // 31 column blocks per row
+  __shared__ float2 l1[N1 * k4Cols];  // pass-1 outputs [ka][b][column]
+  const long long rr = blockIdx.x / NCB;
+  if (rr >= n_rows) return;
+  const int c0 = k4Cols * static_cast<int>(blockIdx.x - rr * NCB);
+  const long long I = r0 + rr;
+  const BigItem it = big_item(I, n_prn, n_bin, n_ant, hs, tile);
+  const int h = static_cast<int>(it.xrow & 1);
+  const float2* x = X + (hs == 1 ? it.xrow : it.xrow >> 1) * PL;
+  const float2* c = rep + (static_cast<long long>(it.prn) * 2 + h) * PL;
+  // pass 1: radix R0 from the spectra, units (column, b), consecutive threads on consecutive
+  // columns
+  constexpr int NU = (k4Cols * 33 + k4TA - 1) / k4TA;  // units per thread (the last partly)
+  float2 v[NU][R0];
+#pragma unroll
+  for (int s = 0; s < NU; ++s) {  // every unit's operands in flight before the first DFT
+    const int u = threadIdx.x + s * k4TA;
+    const int cc = u % k4Cols, b = u / k4Cols;
+    const int n2 = c0 + cc;
+    if (u < k4Cols * 33) {
+#pragma unroll
+      for (int a = 0; a < R0; ++a) {
+        const int n = k4N2 * ((33 * a + R0 * b) % N1) + n2;
+        v[s][a] = cconjmul(x[n], c[n]);
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < NU; ++s) {
+    const int u = threadIdx.x + s * k4TA;
+    const int cc = u % k4Cols, b = u / k4Cols;
+    if (u < k4Cols * 33) {
+      Dft<R0>::run(v[s]);
+#pragma unroll
+      for (int ka = 0; ka < R0; ++ka) l1[(ka * 33 + b) * k4Cols + cc] = v[s][ka];
+    }
+  }
+  __syncthreads();
+  // pass 2: radix 33, units (column, ka); twiddle W_P^{n2 k1} from the [k1][n2] table, rows of
+  // the output written with non-temporal stores (read once, by step 2)
+  float2* y = Y + rr * PL;
+  for (int u = threadIdx.x; u < k4Cols * R0; u += k4TA) {
+    const int cc = u % k4Cols, ka = u / k4Cols;
+    const int n2 = c0 + cc;
+    float2 v[33];
+#pragma unroll
+    for (int b = 0; b < 33; ++b) v[b] = l1[(ka * 33 + b) * k4Cols + cc];
+    Dft<33>::run(v);
+#pragma unroll
+    for (int kb = 0; kb < 33; ++kb) {
+      const int k1 = (FS::F1 * ka + FS::F2 * kb) % N1;
+      const float2 t = cmul(v[kb], tw4[k1 * k4N2 + n2]);
+      float* d = reinterpret_cast<float*>(y + k1 * k4N2 + n2);
+      __builtin_nontemporal_store(t.x, d);
+      __builtin_nontemporal_store(t.y, d + 1);
+    }
+  }
+  if (c0 == 0 && threadIdx.x == 0) {
+    const long long r = big_row(I, n_prn, n_bin, n_ant, hs, tile);
+    rowkey[r] = 0ull;
+    rownan[r] = 0;
+  }
+}
+
+// c31[m] = (cos, sin)(2 pi m / 31).  dump (DUMP): every power also goes to
+// dump[row * P + sample], row the RowOut index of the long row.
+template <int R0, bool DUMP>
+__global__ __launch_bounds__(k4TB, 4) void k_4s_rows(const float2* __restrict__ Y,
+                                                   const float2* __restrict__ c31, int n_prn,
+                                                   int n_bin, int n_ant, int hs, int tile,
+                                                   long long r0, long long n_pairs,
+                                                   unsigned long long* __restrict__ rowkey,
+                                                   int* __restrict__ rownan,
+                                                   float* __restrict__ dump) {
+  using FS = FourStep<R0>;
+  constexpr int N1 = FS::N1, PL = FS::PL;
+  constexpr int NP = 2 * k4N2;  // values of a wave's pair of k1 rows
+  // LDS position of value i: one pad slot per 16 values, so the radix-16 lanes (16 values apart)
+  // fall on different banks
+  auto pad = [](int i) { return i + (i >> 4); };
+  __shared__ float2 lds[k4TB / 64][NP + NP / 16];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, ln16 = lane & 15, grp = lane >> 4;
+  float2* L = lds[wave];
+  // the lane's radix-31 GEMM constants (A = [l%16][l/16] of the cos / sin matrices, as k_rows<1>)
+  float cA[4], sA[4];
+#pragma unroll
+  for (int st = 0; st < 4; ++st) {
+    const int j = 4 * st + grp;
+    const float2 w = c31[(j * ln16) % 31];
+    cA[st] = j == 0 ? 1.0f : w.x;
+    sA[st] = j == 0 ? 0.0f : w.y;
+  }
+  // the next pair's values are loaded into registers while the current pair is transformed
+  constexpr int NV = (NP + 63) / 64;  // 16 values per lane (the last partly)
+  const long long pstep = static_cast<long long>(gridDim.x) * (k4TB / 64);
+  float2 nx[NV];
+  auto load_pair = [&](long long pi) {
+    const float2* src = Y + 2 * pi * k4N2;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int idx = lane + 64 * i;
+      nx[i] = idx < NP ? src[idx] : make_float2(0.f, 0.f);
+    }
+  };
+  const long long p0 = static_cast<long long>(blockIdx.x) * (k4TB / 64) + wave;
+  if (p0 < n_pairs) load_pair(p0);
+  for (long long pi = p0; pi < n_pairs; pi += pstep) {
+    // k1 rows 2 pi, 2 pi + 1 of the chunk (N1 is even: both belong to one long row)
+    const long long g = 2 * pi;
+    const long long rq = g / N1;
+    const int k1 = static_cast<int>(g - rq * N1);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int idx = lane + 64 * i;
+      if (idx < NP) L[pad(idx)] = nx[i];
+    }
+    wave_lds_sync();
+    if (pi + pstep < n_pairs) load_pair(pi + pstep);
+    // radix 16 over a, in place: lane (q, b) owns the 16 positions (31 a + 16 b) mod 496 of
+    // row q and writes output ka where input a was
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    if (ln < 62) {
+      const int q = ln >= 31 ? 1 : 0;
+      const int b = ln - 31 * q;
+      float2 v[16];
+#pragma unroll
+      for (int a = 0; a < 16; ++a) v[a] = L[pad(q * k4N2 + (31 * a + 16 * b) % k4N2)];
+      Dft<16>::run(v);
+#pragma unroll
+      for (int a = 0; a < 16; ++a) L[pad(q * k4N2 + (31 * a + 16 * b) % k4N2)] = v[a];
+    }
+    wave_lds_sync();
+    // radix 31 over b on the matrix cores: one tile of 16 butterflies (ka = lane % 16) per row
+    unsigned long long km = 0ull;
+    long long rowoff = 0;
+    if constexpr (DUMP) rowoff = big_row(r0 + rq, n_prn, n_bin, n_ant, hs, tile) * PL;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      floatx4 acc[4];
+#pragma unroll
+      for (int cc = 0; cc < 4; ++cc) acc[cc] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int st = 0; st < 4; ++st) {
+        const int j1 = 4 * st + grp, j2 = j1 == 0 ? 0 : 31 - j1;
+        const float2 v1 = L[pad(q * k4N2 + (31 * ln16 + 16 * j1) % k4N2)];
+        float2 v2 = L[pad(q * k4N2 + (31 * ln16 + 16 * j2) % k4N2)];
+        if (j1 == 0) v2 = make_float2(0.f, 0.f);  // a_0 = v_0 (no partner input)
+        acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(cA[st], v1.x + v2.x, acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(cA[st], v1.y + v2.y, acc[1], 0, 0, 0);
+        acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(sA[st], v1.x - v2.x, acc[2], 0, 0, 0);
+        acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(sA[st], v1.y - v2.y, acc[3], 0, 0, 0);
+      }
+      const int kq = k1 + q;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int kb = 4 * grp + e;  // output row of the GEMM tile
+        const float ar = acc[0][e], ai = acc[1][e];
+        const float br = acc[2][e], bi = acc[3][e];
+        const float y1r = ar + bi, y1i = ai - br;  // Y_kb
+        const int s1 = kq + N1 * ((465 * ln16 + 32 * kb) % k4N2);
+        const float p1 = y1r * y1r + y1i * y1i;
+        const unsigned long long t1 = pkey(p1, s1);
+        km = t1 > km ? t1 : km;
+        if constexpr (DUMP) dump[rowoff + s1] = p1;
+        if (kb > 0) {
+          const float y2r = ar - bi, y2i = ai + br;  // Y_{31-kb}
+          const int s2 = kq + N1 * ((465 * ln16 + 32 * (31 - kb)) % k4N2);
+          const float p2 = y2r * y2r + y2i * y2i;
+          const unsigned long long t2 = pkey(p2, s2);
+          km = t2 > km ? t2 : km;
+          if constexpr (DUMP) dump[rowoff + s2] = p2;
+        }
+      }
+    }
+    // the pair's argmax into the long row's key (a NaN power has the largest key)
+    unsigned int pw, iw;
+    wave_argmax(__uint_as_float(static_cast<unsigned int>(km >> 32)),
+                static_cast<int>(km & 0xffffffffu), pw, iw);
+    const bool bad = __ballot((km >> 32) > 0x7f800000ull) != 0;
+    if (lane == 0) {
+      const long long r = big_row(r0 + rq, n_prn, n_bin, n_ant, hs, tile);
+      atomicMax(rowkey + r, pkey(funord(pw), static_cast<int>(iw) - 1));
+      if (bad) atomicOr(rownan + r, 1);
+    }
+    wave_lds_sync();  // the next pair's staging overwrites L
+  }
+}
+
 }  // namespace fused
 
 // ================================ generic path ===========================================
@@ -1761,13 +1994,13 @@ __device__ __forceinline__ int pfa_pos(int n) {
 // (X and R of long rows are stored segment by segment in the prime-factor order).
 __global__ void k_mul_win(const float2* __restrict__ X, const float2* __restrict__ rep, int M,
                           int n_prn, int n_bin, int folded, const int* __restrict__ win,
-                          long long total, float2* __restrict__ Z) {
+                          long long total, float2* __restrict__ Z, int natural) {
   const long long gid = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
   if (gid >= total) return;
   const long long w = gid / M;
   const int i = static_cast<int>(gid - w * M);
   const int seg = i / kFusedP;
-  const int pos = seg * kFusedP + pfa_pos(i - seg * kFusedP);
+  const int pos = natural ? i : seg * kFusedP + pfa_pos(i - seg * kFusedP);
   const int ant = static_cast<int>(w / n_prn);
   const int prn = static_cast<int>(w - static_cast<long long>(ant) * n_prn);
   const int bh = win[w];  // bin * 2 + h
@@ -2153,10 +2386,16 @@ void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, i
     a.rowkey = rowkey;
     a.rownan = rownan;
     a.dump = dump;
-    // the surface form runs the matrix-core radix-31 variant whatever `variant` says
-    if (dump && R0 == 4)
+    // the surface form runs the plan's radix-31 implementation, so its search is the same
+    if (dump && R0 == 4 && variant == 0)
+      hipLaunchKernelGGL((fused::k_rows<0, 4, false, true>), dim3(blocks), dim3(fused::T), 0, st,
+                         a);
+    else if (dump && R0 == 4)
       hipLaunchKernelGGL((fused::k_rows<1, 4, false, true>), dim3(blocks), dim3(fused::T), 0, st,
                          a);
+    else if (dump && variant == 0)
+      hipLaunchKernelGGL((fused::k_rows<0, 10, false, true>), dim3(blocks), dim3(fused::T), 0,
+                         st, a);
     else if (dump)
       hipLaunchKernelGGL((fused::k_rows<1, 10, false, true>), dim3(blocks), dim3(fused::T), 0,
                          st, a);
@@ -2168,6 +2407,64 @@ void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, i
       hipLaunchKernelGGL((fused::k_rows<0, 10>), dim3(blocks), dim3(fused::T), 0, st, a);
     else
       hipLaunchKernelGGL((fused::k_rows<1, 10>), dim3(blocks), dim3(fused::T), 0, st, a);
+  }
+  TGA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_big_rowout, dim3(grid_for(nr, 256)), dim3(256), 0, st, rowkey, rownan,
+                     r0, nr, n_prn, n_bin, n_ant, hs, tile, rows);
+  TGA_HIP(hipGetLastError());
+}
+
+std::vector<float2> four_step_twiddles(size_t P) {
+  const int n1 = static_cast<int>(P / fused::k4N2);
+  std::vector<float2> t(P);
+  for (int k1 = 0; k1 < n1; ++k1)
+    for (int n2 = 0; n2 < fused::k4N2; ++n2) {
+      const long long e = static_cast<long long>(n2) * k1;  // < P
+      const double a = -2.0 * kPiD * static_cast<double>(e) / static_cast<double>(P);
+      t[static_cast<size_t>(k1) * fused::k4N2 + n2] =
+          make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+    }
+  return t;
+}
+
+std::vector<float2> radix31_table() {
+  std::vector<float2> t(31);
+  for (int m = 0; m < 31; ++m) {
+    const double a = 2.0 * kPiD * m / 31.0;
+    t[m] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+  }
+  return t;
+}
+
+void launch_big_4s(const float2* X, const float2* rep, const float2* tw4, const float2* c31,
+                   int R0, int n_prn, int n_bin, int n_ant, int hs, int tile, long long r0,
+                   long long r1, float2* Y, unsigned long long* rowkey, int* rownan,
+                   RowOut* rows, hipStream_t st, float* dump) {
+  const long long nr = r1 - r0;
+  if (nr <= 0) return;
+  const dim3 g1(static_cast<unsigned>(nr * (fused::k4N2 / fused::k4Cols)));
+  const long long n_pairs = nr * R0 * 33 / 2;
+  const dim3 g2(static_cast<unsigned>(std::min<long long>((n_pairs + 3) / 4, 32768)));
+  if (R0 == 4) {
+    hipLaunchKernelGGL(fused::k_4s_pre<4>, g1, dim3(fused::k4TA), 0, st, X, rep, tw4, n_prn,
+                       n_bin, n_ant, hs, tile, r0, nr, Y, rowkey, rownan);
+    if (dump)
+      hipLaunchKernelGGL((fused::k_4s_rows<4, true>), g2, dim3(fused::k4TB), 0, st, Y, c31,
+                         n_prn, n_bin, n_ant, hs, tile, r0, n_pairs, rowkey, rownan, dump);
+    else
+      hipLaunchKernelGGL((fused::k_4s_rows<4, false>), g2, dim3(fused::k4TB), 0, st, Y, c31,
+                         n_prn, n_bin, n_ant, hs, tile, r0, n_pairs, rowkey, rownan, dump);
+  } else if (R0 == 10) {
+    hipLaunchKernelGGL(fused::k_4s_pre<10>, g1, dim3(fused::k4TA), 0, st, X, rep, tw4, n_prn,
+                       n_bin, n_ant, hs, tile, r0, nr, Y, rowkey, rownan);
+    if (dump)
+      hipLaunchKernelGGL((fused::k_4s_rows<10, true>), g2, dim3(fused::k4TB), 0, st, Y, c31,
+                         n_prn, n_bin, n_ant, hs, tile, r0, n_pairs, rowkey, rownan, dump);
+    else
+      hipLaunchKernelGGL((fused::k_4s_rows<10, false>), g2, dim3(fused::k4TB), 0, st, Y, c31,
+                         n_prn, n_bin, n_ant, hs, tile, r0, n_pairs, rowkey, rownan, dump);
+  } else {
+    fail(TGA_ERR_UNSUPPORTED, "the four-step split supports 4 and 10 sub-rows");
   }
   TGA_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_big_rowout, dim3(grid_for(nr, 256)), dim3(256), 0, st, rowkey, rownan,
@@ -2190,11 +2487,11 @@ void launch_surface_finish(float* v, long long n, float gain, float scale, hipSt
 void launch_big_second(const float2* X, const float2* rep, const PeakSlot* slots, int M, int P,
                        int cw, int n_ant, int n_prn, int n_bin, bool folded, const int* win,
                        float gain, float scale, float2* Z, float2* Zs, tga_peak* out,
-                       hipStream_t st) {
+                       hipStream_t st, bool natural) {
   const long long nw = static_cast<long long>(n_ant) * n_prn;
   const long long total = nw * M;
   hipLaunchKernelGGL(k_mul_win, dim3(grid_for(total, 256)), dim3(256), 0, st, X, rep, M, n_prn,
-                     n_bin, folded ? 1 : 0, win, total, Z);
+                     n_bin, folded ? 1 : 0, win, total, Z, natural ? 1 : 0);
   TGA_HIP(hipGetLastError());
   float2* z = fft_forward(fft_plan(static_cast<size_t>(M)), Z, Zs, static_cast<size_t>(nw), st);
   hipLaunchKernelGGL(k_second_win, dim3(static_cast<unsigned>(nw)), dim3(kRedT), 0, st, z, M, P,

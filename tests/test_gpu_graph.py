@@ -4,6 +4,7 @@ same peaks, bit for bit, as eager searches; timed searches report kernel times (
 event-record nodes, or eager launches where the runtime refuses such nodes in a capture);
 plans with a Bluestein transform stay eager.  The eager peaks themselves are pinned against the
 oracle in test_gpu_plan.py, test_gpu_fp16.py and test_gpu_correlate.py."""
+import os
 import re
 
 import numpy as np
@@ -37,6 +38,10 @@ CASES = {
                "split4x16368"),
     "split10_unfolded": (FS, 163680, 1, 1, [("beidou", [30], grid(IF, 500.0, 500.0), 0)],
                          "fp32", "split10x16368"),
+    "fourstep4": (FS, 2 * 65472, 1, 1, [("galileo", [3, 11], grid(IF, 500.0, 500.0), 0)],
+                  "fp32", "split132x496"),
+    "fourstep10_unfolded": (FS, 163680, 1, 1, [("beidou", [30], grid(IF, 500.0, 500.0), 0)],
+                            "fp32", "split330x496"),
     "generic_folded": (2.046e6, 3 * 2046, 2, 2, [("gps", [3, 4], FC2, 0),
                                                   ("qzss", [193], FC2, 0)], "fp32",
                        "generic-folded"),
@@ -56,8 +61,26 @@ def signals_for(systems, fs, seed):
     return out
 
 
+# the split cases pin their FP32 form: the sub-row transforms (matrix-core radix 31) or the
+# four-step form
+SPLIT_FORM = {"split4": "1", "split10_unfolded": "1", "fourstep4": "3", "fourstep10_unfolded": "3"}
+
+
 def make_plan(case):
     fs, L, n_ant, fmt, systems, precision, _ = CASES[case]
+    old = os.environ.get("TGA_SPLIT_VARIANT")
+    if case in SPLIT_FORM:
+        os.environ["TGA_SPLIT_VARIANT"] = SPLIT_FORM[case]
+    try:
+        return _plan(fs, n_ant, L, systems, fmt, precision)
+    finally:
+        if old is None:
+            os.environ.pop("TGA_SPLIT_VARIANT", None)
+        else:
+            os.environ["TGA_SPLIT_VARIANT"] = old
+
+
+def _plan(fs, n_ant, L, systems, fmt, precision):
     return tga.Plan(fs, n_ant, L, [tga.SystemSearch(n, f, p, nc_epochs=e)
                                    for n, p, f, e in systems],
                     input_format=fmt, precision=precision)
@@ -110,7 +133,7 @@ def test_first_search_runs_eagerly(gpu):
     assert graph_launches(plan) == 1
 
 
-@pytest.mark.parametrize("case", ["fused", "split4", "generic_folded"])
+@pytest.mark.parametrize("case", ["fused", "split4", "fourstep4", "generic_folded"])
 def test_timing_with_graph(gpu, case):
     # timing: a graph with event-record nodes around the timed stages; its replays set the
     # timers like eager launches do, and switching timing off captures the plain graph again

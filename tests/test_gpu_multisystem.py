@@ -110,8 +110,10 @@ def test_all_systems_one_plan_matches_oracle(gpu, fmt):
     plan = make_plan(2, L, fmt)
     desc = plan.describe()
     print(desc, flush=True)
-    for path in ("fused16368 ", "split4x16368 ", "split10x16368 "):
-        assert path in desc
+    # the split groups run the sub-row form or the four-step form, whichever the autotuner picks
+    assert "fused16368 " in desc
+    assert "split4x16368 " in desc or "split132x496 " in desc
+    assert "split10x16368 " in desc or "split330x496 " in desc
     assert plan.n_peaks == 2 * sum(len(PRNS[s]) for s in ORDER)
     peaks = plan.search(packed)
     # the peak array is laid out system by system, antenna-major inside each system
@@ -202,7 +204,7 @@ def test_stream_of_10ms_observations_with_tones(gpu):
     plan = tga.Plan(FS, n_ant, L, [tga.SystemSearch(s, fc_of(s), PRNS[s]) for s in ORDER],
                     input_format=fmt)
     desc = plan.describe()
-    assert "split10x16368" in desc and "N=163680" in desc
+    assert ("split10x16368" in desc or "split330x496" in desc) and "N=163680" in desc
     dev = torch.from_numpy(np.concatenate([o for _, _, o in stream])).cuda()
     torch.cuda.synchronize()
     row_bytes = stream[0][2].shape[1]

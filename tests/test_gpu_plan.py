@@ -59,11 +59,19 @@ def fused_variant(request, monkeypatch):
     return request.param
 
 
-@pytest.fixture(params=[0, 1], ids=["split-valu31", "split-mfma31"])
+@pytest.fixture(params=[0, 1, 3], ids=["split-valu31", "split-mfma31", "split-fourstep"])
 def split_variant(request, monkeypatch):
-    """Both radix-31 implementations of the long rows' sub-row transforms."""
+    """Every FP32 form of the long rows: R0 sub-rows of 16368 with either radix-31
+    implementation, and the four-step N1 x 496 form."""
     monkeypatch.setenv("TGA_SPLIT_VARIANT", str(request.param))
     return request.param
+
+
+def split_path(variant, r0):
+    """The describe() path of an FP32 split group of R0 x 16368-sample periods."""
+    if variant == 3:
+        return "split132x496" if r0 == 4 else "split330x496"
+    return f"split{r0}x16368"
 
 
 @pytest.mark.parametrize("fmt", [1, 2])
@@ -191,7 +199,7 @@ def test_split_e1c_at_16368khz(gpu, split_variant):
     fc = grid(IF, 1500.0, 500.0)
     systems = [("galileo", [3, 11, 27], fc)]
     plan, peaks, packed = run_case(fs, L, 2, 1, systems, signals, seed=21)
-    assert "split4x16368" in plan.describe()
+    assert split_path(split_variant, 4) in plan.describe()
     assert f"rows_variant={split_variant}" in plan.describe()
     check_against_oracle(plan, peaks, packed, fs, L, 1, systems)
     assert int(peaks[0][0, 1]["best_codephase"]) == (65472 - 40000) % 65472
@@ -207,7 +215,7 @@ def test_split_row_tiles_ragged(gpu, split_variant):
     fc = grid(IF, 1000.0, 500.0)
     systems = [("galileo", list(range(1, 11)), fc)]
     plan, peaks, packed = run_case(fs, L, 1, 1, systems, signals, seed=23)
-    assert "split4x16368" in plan.describe()
+    assert split_path(split_variant, 4) in plan.describe()
     assert check_against_oracle(plan, peaks, packed, fs, L, 1, systems) == 10
     assert int(peaks[0][0, 8]["best_codephase"]) == (65472 - 31000) % 65472
 
@@ -234,12 +242,26 @@ def test_split_tile_shapes_give_identical_peaks(gpu, monkeypatch):
             assert check_against_oracle(plan, [pk], packed, fs, L, 1, systems) == 20
         for f in pk.dtype.names:
             assert np.array_equal(pk[f], base[f]), (t, f)
-    # unpinned, the autotuner times both radix-31 implementations x 5 shapes
+    # the four-step form on every tile shape: bit-identical among themselves, and the oracle's
+    monkeypatch.setenv("TGA_SPLIT_VARIANT", "3")
+    base4 = None
+    for t in ("8x8", "2x32", "3x7"):
+        monkeypatch.setenv("TGA_SPLIT_TILE", t)
+        plan = tga.Plan(fs, 2, L, [tga.SystemSearch(n, f, p) for n, p, f in systems])
+        assert f"tile={t}" in plan.describe() and "split132x496" in plan.describe()
+        pk = plan.search(packed)[0]
+        if base4 is None:
+            base4 = pk
+            assert check_against_oracle(plan, [pk], packed, fs, L, 1, systems) == 20
+        for f in pk.dtype.names:
+            assert np.array_equal(pk[f], base4[f]), (t, f)
+    # unpinned, the autotuner times the three FP32 forms x 5 shapes
     monkeypatch.delenv("TGA_SPLIT_TILE")
     monkeypatch.delenv("TGA_SPLIT_VARIANT")
     desc = tga.Plan(fs, 2, L, [tga.SystemSearch(n, f, p) for n, p, f in systems]).describe()
-    for c in ("v1t8x8:", "v1t4x16:", "v0t16x4:", "v0t2x32:", "v1t32x2:"):
+    for c in ("v1t8x8:", "v1t4x16:", "v0t16x4:", "v0t2x32:", "v1t32x2:", "v3t8x8:", "v3t4x16:"):
         assert c in desc, desc
+    assert "v3t2x32:" not in desc  # the four-step form is timed on two shapes
 
 
 @pytest.mark.parametrize("system,prn", [("beidou", 19), ("l1c", 7)])
@@ -251,7 +273,7 @@ def test_split_b1c_l1c_at_16368khz(gpu, split_variant, system, prn):
     fc = grid(IF, 1000.0, 500.0)
     systems = [(system, [prn, prn + 1], fc)]
     plan, peaks, packed = run_case(fs, L, 1, 2, systems, signals, seed=prn)
-    assert "split10x16368" in plan.describe()
+    assert split_path(split_variant, 10) in plan.describe()
     check_against_oracle(plan, peaks, packed, fs, L, 2, systems)
     assert int(peaks[0][0, 0]["best_codephase"]) == (163680 - 99999) % 163680
 
@@ -340,6 +362,6 @@ def test_split_unfolded_10ms_b1c(gpu, split_variant):
     fc = grid(IF, 1000.0, 500.0)
     systems = [("beidou", [30, 31], fc)]
     plan, peaks, packed = run_case(fs, L, 2, 1, systems, signals, seed=12)
-    assert "split10x16368" in plan.describe()
+    assert split_path(split_variant, 10) in plan.describe()
     check_against_oracle(plan, peaks, packed, fs, L, 1, systems)
     assert int(peaks[0][0, 0]["best_codephase"]) == (163680 - 12345) % 163680
