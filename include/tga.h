@@ -272,6 +272,14 @@ int tga_plan_describe(const tga_plan* plan, char* buf, size_t cap);
  * split (P = 4 or 10 x 16368) kernels, else TGA_ERR_UNSUPPORTED; cap < the size
  * tga_plan_surface_size reports is TGA_ERR_INVALID_ARGUMENT.  Allocates a staging buffer of the
  * same size. */
+/* The carrier table a fused (16368-sample) group's forward kernel multiplies the samples by,
+ * copied to host memory: (cos, sin) pairs [bin][sample] of phi = f32(f64(pp[n]) * fc) through
+ * f64 sincos rounded to f32 (correlate.rs:110-115), or of the f64 phase in the FP16 form; n_floats
+ * receives 2 x bins x samples (samples = 2P, or P x epochs in the non-coherent form); out may be
+ * NULL to query the size.  A diagnostic with no counterpart in the reference.  Other groups:
+ * TGA_ERR_UNSUPPORTED. */
+int tga_plan_carrier(const tga_plan* plan, int32_t sys_index, float* out, size_t cap,
+                     size_t* n_floats);
 /* Floats of system sys_index's surface, or of the whole plan's for sys_index = -1. */
 int tga_plan_surface_size(const tga_plan* plan, int32_t sys_index, size_t* n_floats);
 int tga_plan_surface(tga_plan* plan, const void* d_packed, void* stream, float* d_surface,

@@ -362,7 +362,10 @@ void Plan::build(const tga_plan_desc& d) {
     const double per_row = g.nc > 0 ? 0.5 * g.nc : (g.big && !g.folded) ? 0.5 : 1.0;
     g.row_bytes_alg =
         static_cast<double>(g.n_rows) * per_row * 2.0 * static_cast<double>(M) * elem;
-    g.fwd_bytes_alg = static_cast<double>(g.n_xrow) * static_cast<double>(M) * elem;
+    // forward: each (antenna, bin, fold) row writes its spectrum; the fused kernels also read
+    // the carrier table of the row's samples (2P values, one period per non-coherent epoch)
+    const double car_row = g.fused ? (g.nc > 0 ? 1.0 : 2.0) * static_cast<double>(g.P) * 8.0 : 0.0;
+    g.fwd_bytes_alg = static_cast<double>(g.n_xrow) * (static_cast<double>(M) * elem + car_row);
   }
   for (auto& g : groups)
     if (fft_plan_bluestein(fft_plan(g.M))) graphable = false;
@@ -912,6 +915,24 @@ int tga_plan_surface(tga_plan* plan, const void* d_packed, void* stream, float* 
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : p->stream;
     p->surface(static_cast<const uint8_t*>(d_packed), d_surface, st);
     if (out) p->finish(out, st);
+  });
+}
+
+int tga_plan_carrier(const tga_plan* plan, int32_t sys_index, float* out, size_t cap,
+                     size_t* n_floats) {
+  return plan_guard([&] {
+    if (!plan || !n_floats) fail(TGA_ERR_INVALID_ARGUMENT, "null argument");
+    const Plan* p = reinterpret_cast<const Plan*>(plan);
+    if (sys_index < 0 || sys_index >= static_cast<int32_t>(p->systems.size()))
+      fail(TGA_ERR_INVALID_ARGUMENT, "system index out of range");
+    const Group& g = p->groups[p->systems[sys_index].group];
+    if (!g.fused || !g.d_carrier.p)
+      fail(TGA_ERR_UNSUPPORTED, "only fused (16368-sample) groups keep a carrier table");
+    *n_floats = g.d_carrier.bytes / sizeof(float);
+    if (out) {
+      if (cap < *n_floats) fail(TGA_ERR_INVALID_ARGUMENT, "carrier buffer too small");
+      TGA_HIP(hipMemcpy(out, g.d_carrier.p, g.d_carrier.bytes, hipMemcpyDeviceToHost));
+    }
   });
 }
 

@@ -103,6 +103,8 @@ PROTOTYPES = {
     "tga_plan_kernel_name": (C.c_char_p, [C.c_void_p, C.c_int32]),
     "tga_plan_describe": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
     "tga_plan_surface_size": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_size_t)]),
+    "tga_plan_carrier": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_float), C.c_size_t,
+                                   C.POINTER(C.c_size_t)]),
     "tga_plan_surface": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float),
                                    C.c_size_t, C.POINTER(Peak)]),
     "tga_estimate_cn0": (C.c_int, [C.c_double, C.c_int32, _dp, C.POINTER(C.c_int32)]),

@@ -329,6 +329,17 @@ class Plan:
                                       out.ctypes.data_as(C.POINTER(L.Peak))))
         return self._split(out)
 
+    def carrier(self, sys_index=0):
+        """The carrier table of system sys_index's fused group (tga_plan_carrier): complex64
+        [bins, samples]."""
+        n = C.c_size_t(0)
+        check(lib().tga_plan_carrier(self._h, int(sys_index), None, 0, C.byref(n)))
+        out = np.empty(n.value, dtype=np.float32)
+        check(lib().tga_plan_carrier(self._h, int(sys_index),
+                                     out.ctypes.data_as(C.POINTER(C.c_float)), n.value,
+                                     C.byref(n)))
+        return out.view(np.complex64).reshape(len(self.systems[sys_index].fc), -1)
+
     def surface_size(self, sys_index=-1):
         """Floats of one system's correlation surface (all systems' for -1)."""
         n = C.c_size_t(0)
