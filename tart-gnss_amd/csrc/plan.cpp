@@ -390,7 +390,7 @@ void Plan::build(const tga_plan_desc& d) {
 }
 
 // Rows-kernel autotuner: times every candidate shape once (after one warm-up launch) on the
-// group's real sizes, with zeroed spectra, and keeps the fastest.  Candidates: the radix-31
+// group's real sizes, with pseudo-random spectra, and keeps the fastest.  Candidates: the radix-31
 // implementation of the f32 kernel (VALU codelet or matrix cores) and the number of XCD-local
 // PRN groups (8, 16 or 32: more groups give more, shorter workgroups, each over fewer PRNs).
 // TGA_FUSED_VARIANT=0|1 pins the radix-31 implementation (tests use it to cover both) and
@@ -429,7 +429,7 @@ void Plan::autotune(Group& g) {
   g.tune.clear();
   if (cands.size() == 1) return;
   DevBuf& xs = g.fp16 ? g.d_X16 : g.d_X;
-  TGA_HIP(hipMemsetAsync(xs.p, 0, xs.bytes, stream));
+  launch_fill_tune(xs.p, xs.bytes, g.fp16, stream);
   auto launch = [&](const FusedRowsConfig& c) {
     if (g.nc > 0)
       launch_fused_rows_nc(g.d_X.as<float2>(), g.d_rep.as<float2>(), g.d_tw.as<float2>(), n_ant,
@@ -464,8 +464,8 @@ void Plan::autotune(Group& g) {
 }
 
 // Split-path autotuner: times the first chunk of rows (both kernels of the form, after one
-// warm-up) for every candidate on the group's real sizes, with zeroed spectra, and keeps the
-// fastest.  Candidates: the FP32 form (variants 0 and 1: R0 sub-rows of 16368 through the LDS
+// warm-up) for every candidate on the group's real sizes, with pseudo-random spectra, and keeps
+// the fastest.  Candidates: the FP32 form (variants 0 and 1: R0 sub-rows of 16368 through the LDS
 // transform with the VALU or the matrix-core radix 31, the next sub-row prefetched in 1;
 // variant 3: the four-step N1 x 496 form; the FP16 form has one) and the tile shape of the row
 // order (TX forward spectra x TP replica spectra whose rows run together, so their spectra are
@@ -508,7 +508,7 @@ void Plan::autotune_split(Group& g) {
   g.rows_cfg = cands[0];
   g.tune.clear();
   if (cands.size() == 1) return;
-  TGA_HIP(hipMemsetAsync(g.d_X.p, 0, g.d_X.bytes, stream));
+  launch_fill_tune(g.d_X.p, g.d_X.bytes, false, stream);
   const int hs = g.folded ? 1 : 2;
   const long long r1 = std::min(g.n_rows / hs, g.chunk_rows);
   auto launch = [&](const FusedRowsConfig& c) {

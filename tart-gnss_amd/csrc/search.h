@@ -93,6 +93,9 @@ void launch_fused_forward16(const uint8_t* packed, size_t row_bytes, int fmt,
 void launch_fused_rows16(const uint32_t* X16, const uint32_t* rep16, const float2* tw,
                          int n_ant, int n_bin, int n_prn, int cw, RowOut* rows, int groups,
                          hipStream_t st, float* dump = nullptr);
+// Fills `bytes` of autotuner spectra with a fixed pseudo-random pattern of values in [-1, 1)
+// (f32 words, or f16 pairs with f16 = true).
+void launch_fill_tune(void* p, size_t bytes, bool f16, hipStream_t st);
 // n values = n / 16368 natural-order rows -> f16 pairs times scale in the prime-factor order
 void launch_to_f16(const float2* in, uint32_t* out, long long n, float scale, hipStream_t st);
 // Twiddle table of the fused kernels: W_P^i (i < 128) then W_P^{128 i} (i < 128).

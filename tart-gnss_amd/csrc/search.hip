@@ -814,11 +814,17 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
   }
   // Sample index of output r of pass-C butterfly j: the CRT number of (j mod 33, j / 33, r).
   auto out_idx = [](int j, int r) { return pfa_wrap(pfa_base(j) + NBC * r); };
+  // Sub-rows with the matrix-core radix 31 keep only each mirrored output pair's larger power:
+  // with A = C a and B = S b of output row kk, |Y_kk|^2 = S + 2D and |Y_{31-kk}|^2 = S - 2D for
+  // S = |A|^2 + |B|^2 and D = Re A Im B - Im A Re B, so the pair's maximum is S + 2|D| (one FMA
+  // with an absolute-value operand) and its sample follows from the sign of D.
+  constexpr bool kPairs = SUB > 0 && VARIANT == 1;
   // One transform of a correlation row (or sub-row): `operand(ta, r)` gives pass A's input r of
   // butterfly ta; every output power goes to `sink(slot, p, idx)` with slot < 31 for the
-  // thread's main set (outputs more than 32 samples apart) and slot 31 / 32 for its extra set.
+  // thread's main set (outputs more than 32 samples apart) and slot 31 / 32 for its extra set,
+  // or, with kPairs, every output pair to `pair(ts, e, q, kk, S, D)` (tile slot ts, GEMM row e).
   // `after_a()` runs once pass A has consumed its operands (prefetches for the next row).
-  auto transform = [&](auto operand, auto sink, auto after_a) {
+  auto transform = [&](auto operand, auto sink, auto after_a, auto pair) {
     // ---- pass A
     int ta = tid;
     asm volatile("" : "+v"(ta));
@@ -964,13 +970,20 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
               const int kk = 4 * gc + e;  // output row of the GEMM tile
               const float ar = acc[u][0][e], ai = acc[u][1][e];
               const float br = acc[u][2][e], bi = acc[u][3][e];
-              const float y1r = ar + bi, y1i = ai - br;  // Y_kk
-              const float p1 = y1r * y1r + y1i * y1i;
-              sink(slot, p1, out_idx(q, kk));
-              if (kk > 0) {
-                const float y2r = ar - bi, y2i = ai + br;  // Y_{31-kk}
-                const float p2 = y2r * y2r + y2i * y2i;
-                sink(slot, p2, out_idx(q, 31 - kk));
+              if constexpr (kPairs) {
+                // kk = 0: B = 0 (row 0 of S is zero), so D = 0 and S = |Y_0|^2
+                const float s2 = fmaf(ar, ar, fmaf(ai, ai, fmaf(br, br, bi * bi)));
+                const float d = fmaf(ar, bi, -(ai * br));
+                pair(ts, e, q, kk, s2, d);
+              } else {
+                const float y1r = ar + bi, y1i = ai - br;  // Y_kk
+                const float p1 = y1r * y1r + y1i * y1i;
+                sink(slot, p1, out_idx(q, kk));
+                if (kk > 0) {
+                  const float y2r = ar - bi, y2i = ai + br;  // Y_{31-kk}
+                  const float p2 = y2r * y2r + y2i * y2i;
+                  sink(slot, p2, out_idx(q, 31 - kk));
+                }
               }
             }
           }
@@ -1002,7 +1015,8 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
               return cconjmul(buffer_load_f2(xv, ta * 8, r * NBA * 8),
                               buffer_load_f2(rv, ta * 8, r * NBA * 8));
             },
-            [&](int slot, float p, int) { acc[slot] += p; }, [] {});
+            [&](int slot, float p, int) { acc[slot] += p; }, [] {},
+            [](int, int, int, int, float, float) {});
         lds_barrier();  // the next epoch's pass A overwrites the LDS row
       }
       // the outputs of each slot (same indices in every epoch, see pass C)
@@ -1096,7 +1110,8 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
             if constexpr (kKeepX) {
               if (prn + 1 < p1) load_rep(prn + 1);
             }
-          });
+          },
+          [](int, int, int, int, float, float) {});
       // the two-barrier finish: deferring its second half to the next row's first barrier
       // measured 6% slower here (wave 0 then carries the flush into the next pass)
       row_finish<T>(ms, es, accn != accn, a.cw, red, a.rows + orow);
@@ -1129,6 +1144,24 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
       // the sub-row's argmax only: a running 64-bit key maximum per thread (a NaN power has
       // the largest key, so it reaches the row's key and is flagged there)
       unsigned long long km = 0ull;
+      // kPairs: each output pair's larger power (its f32 bits: powers are >= 0, so the bits
+      // order like the values and a NaN has the largest) with the sign of its D, and the
+      // thread's maximum over them
+      unsigned int pmx = 0u;
+      unsigned int pmb[kPairs ? 5 : 1][4];
+      float pdd[kPairs ? 5 : 1][4];
+      if constexpr (kPairs) {
+#pragma unroll
+        for (int ts = 0; ts < 5; ++ts)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            pmb[ts][e] = 0u;
+            pdd[ts][e] = 0.0f;
+          }
+      }
+      auto dump_at = [&](int idx, float p) {
+        if constexpr (DUMP) a.dump[static_cast<size_t>(r) * (SUB * P) + k0 + SUB * idx] = p;
+      };
       transform(
           [&](int ta, int r) {
             if constexpr (kPrefetch)
@@ -1139,15 +1172,73 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
           [&](int, float p, int idx) {
             const unsigned long long k = pkey(p, idx);
             km = k > km ? k : km;
-            if constexpr (DUMP) a.dump[static_cast<size_t>(r) * (SUB * P) + k0 + SUB * idx] = p;
+            dump_at(idx, p);
           },
           [&] {
             if constexpr (kPrefetch) {
               if (sidx + gridDim.x < a.n_sub) load_sub(sidx + gridDim.x);
             }
+          },
+          [&](int ts, int e, int q, int kk, float s2, float d) {
+            if constexpr (kPairs) {
+              if (ts >= 5) return;  // tile slots 0..4 (wave 0 takes 5 tiles, the others 4)
+              const unsigned int b = __float_as_uint(fmaf(2.0f, fabsf(d), s2));
+              pmb[ts][e] = b;
+              pdd[ts][e] = d;
+              pmx = max(pmx, b);
+              if constexpr (DUMP) {
+                dump_at(out_idx(q, kk), fmaf(2.0f, d, s2));
+                if (kk > 0) dump_at(out_idx(q, 31 - kk), fmaf(-2.0f, d, s2));
+              }
+            }
           });
-      row_finish_key<T>(km, (km >> 32) > 0x7f800000ull, red, k0, SUB, a.rowkey + r,
-                        a.rownan + r);
+      if constexpr (kPairs) {
+        // block maximum M of the pair maxima (one barrier, which also orders this sub-row's
+        // pass-C reads before the next sub-row's pass-A writes), then each wave holding M finds
+        // the largest sample with power M among its pairs (D > 0: Y_kk, D < 0: Y_{31-kk},
+        // D = 0: both) and folds (M, sample) into the row's key: several waves holding M fold
+        // their own, and the atomic maximum keeps the last maximum
+        const unsigned int wmax = wave_umax(pmx);
+        if (lane == 0) red.pk[wave] = wmax;
+        lds_barrier();
+        unsigned int M;
+        {
+          const unsigned int wv = lane < NW ? red.pk[lane] : 0u;
+          M = static_cast<unsigned int>(__builtin_amdgcn_readlane(
+              static_cast<int>(row16_umax(wv)), 0));
+        }
+        const bool mine = pmx == M;
+        if (__ballot(mine) != 0) {
+          unsigned int cand = 0u;  // 1 + sample, 0 for none
+          if (mine) {
+            const int ntile = wave == 0 ? 5 : 4;
+#pragma unroll
+            for (int ts = 0; ts < 5; ++ts) {
+#pragma unroll
+              for (int e = 0; e < 4; ++e) {
+                if (ts < ntile && pmb[ts][e] == M) {
+                  const int q = 16 * (wave + 8 * ts) + ln16;
+                  const int kk = 4 * grp + e;
+                  const float d = pdd[ts][e];
+                  const int i1 = out_idx(q, kk);
+                  const int i2 = kk > 0 ? out_idx(q, 31 - kk) : i1;
+                  const int id = d > 0.0f ? i1 : d < 0.0f ? i2 : max(i1, i2);
+                  cand = max(cand, static_cast<unsigned int>(id + 1));
+                }
+              }
+            }
+          }
+          const unsigned int iw = wave_umax(cand);
+          if (lane == 0) {
+            atomicMax(a.rowkey + r, (static_cast<unsigned long long>(M) << 32) |
+                                        static_cast<unsigned int>(k0 + SUB * (static_cast<int>(iw) - 1)));
+            if (M > 0x7f800000u) atomicOr(a.rownan + r, 1);
+          }
+        }
+      } else {
+        row_finish_key<T>(km, (km >> 32) > 0x7f800000ull, red, k0, SUB, a.rowkey + r,
+                          a.rownan + r);
+      }
     }
   }
 }
@@ -2240,6 +2331,28 @@ void launch_to_pfa(const float2* in, float2* out, long long n, hipStream_t st) {
 void launch_to_f16(const float2* in, uint32_t* out, long long n, float scale, hipStream_t st) {
   hipLaunchKernelGGL(fused::k_to_f16, dim3(grid_for(n, 256)), dim3(256), 0, st, in, out, n,
                      scale);
+  TGA_HIP(hipGetLastError());
+}
+
+// Autotuner spectra: value i is a hash of i mapped to [-1, 1) (f32 words, or f16 pairs), so the
+// timed kernels see data-dependent work like a real search rather than all-zero rows.
+__global__ void k_fill_tune(uint32_t* __restrict__ p, long long n, int f16) {
+  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  uint32_t h = static_cast<uint32_t>(i) * 0x9E3779B1u ^ static_cast<uint32_t>(i >> 32);
+  h ^= h >> 15;
+  h *= 0x85EBCA77u;
+  h ^= h >> 13;
+  const float a = static_cast<float>(h >> 8) * (2.0f / 16777216.0f) - 1.0f;
+  const float b = static_cast<float>(h & 0xffu) * (2.0f / 256.0f) - 1.0f;
+  p[i] = f16 ? fused::pack_f16(make_float2(a, b), 1.0f) : __float_as_uint(a);
+}
+
+void launch_fill_tune(void* p, size_t bytes, bool f16, hipStream_t st) {
+  const long long n = static_cast<long long>(bytes / 4);
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_tune, dim3(grid_for(n, 256)), dim3(256), 0, st,
+                     static_cast<uint32_t*>(p), n, f16 ? 1 : 0);
   TGA_HIP(hipGetLastError());
 }
 
