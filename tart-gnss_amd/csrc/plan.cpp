@@ -49,6 +49,7 @@ std::string group_kernels(const Group& g, int slot) {
              : g.fused ? "k_rows"
              : g.big16 ? "k_big_pre<" + r0 + ",f16>+k_rows16<" + r0 + ">"
              : g.natural() ? "k_4s_pre<" + r0 + ">+k_4s_rows<" + r0 + ">"
+             : g.big && g.rows_cfg.variant == 2 ? "k_big_pre<" + r0 + ">+k_subrows<" + r0 + ">"
              : g.big   ? "k_big_pre<" + r0 + ">+k_rows<" + std::to_string(g.rows_cfg.variant) +
                              "," + r0 + ">"
                        : "k_mul_rows+k_pass+k_row_reduce";
@@ -336,6 +337,9 @@ void Plan::build(const tga_plan_desc& d) {
       g.d_Y.alloc(static_cast<size_t>(g.chunk_rows) * M * y_elem);
       g.d_rowkey.alloc(static_cast<size_t>(g.n_rows) * sizeof(unsigned long long));
       g.d_rownan.alloc(static_cast<size_t>(g.n_rows) * sizeof(int));
+      g.d_rowmap.alloc(static_cast<size_t>(g.chunk_rows) * sizeof(int));
+      if (g.n_rows > (1LL << 31) - 1)
+        fail(TGA_ERR_UNSUPPORTED, "a long-period group holds more than 2^31 - 1 rows");
       const size_t nw = static_cast<size_t>(n_ant) * n_prn;
       g.d_win.alloc(nw * sizeof(int));
       g.d_Zw.alloc(nw * M * sizeof(float2));
@@ -465,11 +469,12 @@ void Plan::autotune(Group& g) {
 
 // Split-path autotuner: times the first chunk of rows (both kernels of the form, after one
 // warm-up) for every candidate on the group's real sizes, with pseudo-random spectra, and keeps
-// the fastest.  Candidates: the FP32 form (variants 0 and 1: R0 sub-rows of 16368 through the LDS
-// transform with the VALU or the matrix-core radix 31, the next sub-row prefetched in 1;
-// variant 3: the four-step N1 x 496 form; the FP16 form has one) and the tile shape of the row
-// order (TX forward spectra x TP replica spectra whose rows run together, so their spectra are
-// fetched once into L2).  TGA_SPLIT_VARIANT=0|1|3 and TGA_SPLIT_TILE=TXxTP pin them.  The
+// the fastest.  Candidates: the FP32 form (variants 0, 1 and 2: R0 sub-rows of 16368 through the
+// LDS transform with the VALU or the matrix-core radix 31 (k_rows: 0, 1), or with the
+// matrix-core radix 31, pass B in place and only each lane's best output pair kept (k_subrows:
+// 2); variant 3: the four-step N1 x 496 form; the FP16 form has one) and the tile shape of the
+// row order (TX forward spectra x TP replica spectra whose rows run together, so their spectra
+// are fetched once into L2).  TGA_SPLIT_VARIANT=0..3 and TGA_SPLIT_TILE=TXxTP pin them.  The
 // choice fixes the spectra's layout: natural order for variant 3, else the segment-wise
 // prime-factor order (the replicas are reordered after tuning).
 void Plan::autotune_split(Group& g) {
@@ -479,9 +484,9 @@ void Plan::autotune_split(Group& g) {
   if (g.big16)
     variants = {0};
   else if (pin_v)
-    variants = {std::atoi(pin_v) == 3 ? 3 : std::atoi(pin_v) != 0 ? 1 : 0};
+    variants = {std::min(3, std::max(0, std::atoi(pin_v)))};
   else
-    variants = {1, 0, 3};
+    variants = {2, 1, 0, 3};
   std::vector<int> tiles;
   if (pin_t) {
     int tx = 0, tp = 0;
@@ -521,10 +526,11 @@ void Plan::autotune_split(Group& g) {
     }
     launch_big_pre(g.d_X.as<float2>(), g.d_rep.as<float2>(), g.d_twb.as<float2>(), g.big_r0,
                    g.n_prn, g.n_bin, n_ant, hs, c.tile, 0, r1, g.d_Y.p, g.big16,
-                   g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(), stream);
+                   g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(),
+                   g.d_rowmap.as<int>(), stream);
     launch_big_sub(g.d_tw.as<float2>(), g.big_r0, g.n_prn, g.n_bin, n_ant, hs, c.tile, 0, r1,
                    g.d_Y.p, g.big16, g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(),
-                   g.d_rows.as<RowOut>(), stream, c.variant);
+                   g.d_rowmap.as<int>(), g.d_rows.as<RowOut>(), stream, c.variant);
   };
   hipEvent_t e0, e1;
   TGA_HIP(hipEventCreate(&e0));
@@ -674,12 +680,13 @@ void Plan::search(const uint8_t* d_in, hipStream_t st, float* dump) {
         const long long r1 = std::min(n_enum, r0 + g.chunk_rows);
         launch_big_pre(g.X, g.d_rep.as<float2>(), g.d_twb.as<float2>(), g.big_r0, g.n_prn,
                        g.n_bin, n_ant, hs, g.rows_cfg.tile, r0, r1, g.d_Y.p, g.big16,
-                       g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(), st);
+                       g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(),
+                       g.d_rowmap.as<int>(), st);
         launch_big_sub(g.d_tw.as<float2>(), g.big_r0, g.n_prn, g.n_bin, n_ant, hs,
                        g.rows_cfg.tile, r0, r1,
                        g.d_Y.p, g.big16, g.d_rowkey.as<unsigned long long>(),
-                       g.d_rownan.as<int>(), g.d_rows.as<RowOut>(), st, g.rows_cfg.variant,
-                       dump ? dump + g.dump_off : nullptr);
+                       g.d_rownan.as<int>(), g.d_rowmap.as<int>(), g.d_rows.as<RowOut>(), st,
+                       g.rows_cfg.variant, dump ? dump + g.dump_off : nullptr);
       }
     } else {
       const FftPlan* fp = fft_plan(g.M);

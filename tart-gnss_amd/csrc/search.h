@@ -156,13 +156,16 @@ std::vector<float2> big_twiddles(size_t P);
 // tile: TX | TP << 16, rows enumerated in tiles of TX forward spectra x TP replica spectra
 void launch_big_pre(const float2* X, const float2* rep, const float2* twb, int R0, int n_prn,
                     int n_bin, int n_ant, int hs, int tile, long long r0, long long r1, void* Y,
-                    bool f16, unsigned long long* rowkey, int* rownan, hipStream_t st);
-// variant: the radix-31 implementation of the FP32 sub-row transforms (0 VALU codelet, 1 matrix
-// cores with the next sub-row prefetched into registers); the FP16 form has one.
+                    bool f16, unsigned long long* rowkey, int* rownan, int* rowmap,
+                    hipStream_t st);
+// variant: the FP32 sub-row transforms (0: k_rows with the VALU radix-31 codelet, 1: k_rows with
+// the matrix-core radix 31 and the next sub-row prefetched into registers, 2: k_subrows, the
+// matrix-core form with pass B in place and each lane's best output pair); the FP16 form has one.
+// rowmap: the row index of each of the chunk's rows, written by launch_big_pre (r1 - r0 ints).
 void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, int hs,
                     int tile, long long r0, long long r1, const void* Y, bool f16,
-                    unsigned long long* rowkey, int* rownan, RowOut* rows, hipStream_t st,
-                    int variant, float* dump = nullptr);
+                    unsigned long long* rowkey, int* rownan, const int* rowmap, RowOut* rows,
+                    hipStream_t st, int variant, float* dump = nullptr);
 // surface form: v[i] = (sqrt(v[i]) * gain) * scale in place (the row values of k_select)
 void launch_surface_finish(float* v, long long n, float gain, float scale, hipStream_t st);
 // second peaks of the winning rows; Z, Zs hold n_ant * n_prn * M complex values; natural: X and

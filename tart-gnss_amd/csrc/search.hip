@@ -13,6 +13,8 @@
 //   rows kernel     (antenna, bin, h) x PRN group: conj(X) . R -> FFT -> |.|^2 -> argmax and
 //                   masked second maximum, X kept in registers across the group's PRNs.
 // |IFFT(Z)| = |FFT(conj Z)| so every transform here is a forward one.
+#include <type_traits>
+
 #include "fft_device.h"
 #include "search.h"
 
@@ -441,7 +443,8 @@ __device__ __forceinline__ void big_pre_positions(const float2* __restrict__ X,
                                                   int p_end, int lt,
                                                   void* __restrict__ Yv,
                                                   unsigned long long* __restrict__ rowkey,
-                                                  int* __restrict__ rownan) {
+                                                  int* __restrict__ rownan,
+                                                  int* __restrict__ rowmap) {
   constexpr int P1 = kFusedP;
   constexpr int P = R0 * P1;
   const int t = p_begin + 2 * lt;
@@ -503,6 +506,7 @@ __device__ __forceinline__ void big_pre_positions(const float2* __restrict__ X,
     const long long r = big_row(I, n_prn, n_bin, n_ant, hs, tile);
     rowkey[r] = 0ull;
     rownan[r] = 0;
+    rowmap[rr] = static_cast<int>(r);  // the chunk's row index for its transforms
   }
 }
 
@@ -529,6 +533,8 @@ struct RowArgs {
   // surface form (DUMP): every output power also goes to dump[row * period + sample], row the
   // RowOut index of the correlation row, period 16368 (rows) or SUB x 16368 (sub-rows)
   float* dump;
+  // sub-row mode: the row index of each of the chunk's rows (written by the pre-pass)
+  const int* rowmap;
 };
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -814,15 +820,18 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
   }
   // Sample index of output r of pass-C butterfly j: the CRT number of (j mod 33, j / 33, r).
   auto out_idx = [](int j, int r) { return pfa_wrap(pfa_base(j) + NBC * r); };
-  // Sub-rows with the matrix-core radix 31 keep only each mirrored output pair's larger power:
-  // with A = C a and B = S b of output row kk, |Y_kk|^2 = S + 2D and |Y_{31-kk}|^2 = S - 2D for
-  // S = |A|^2 + |B|^2 and D = Re A Im B - Im A Re B, so the pair's maximum is S + 2|D| (one FMA
-  // with an absolute-value operand) and its sample follows from the sign of D.
+  // Lane offset of pass A's operand loads: threads beyond the 496 butterflies load at an offset
+  // past the end of the view, which the buffer load returns as 0 (no branch per load).
+  const int voa = tid < NBA ? tid * 8 : P * 8;
+  // Sub-rows with the matrix-core radix 31 keep only each mirrored output pair's larger power,
+  // max(|Y_kk|^2, |Y_{31-kk}|^2) with Y = A -/+ iB, A = C a and B = S b of output row kk, and the
+  // sign of the difference, which names its sample.
   constexpr bool kPairs = SUB > 0 && VARIANT == 1;
   // One transform of a correlation row (or sub-row): `operand(ta, r)` gives pass A's input r of
   // butterfly ta; every output power goes to `sink(slot, p, idx)` with slot < 31 for the
   // thread's main set (outputs more than 32 samples apart) and slot 31 / 32 for its extra set,
-  // or, with kPairs, every output pair to `pair(ts, e, q, kk, S, D)` (tile slot ts, GEMM row e).
+  // or, with kPairs, every output pair to `pair(ts, e, q, kk, p1, p2)` (tile slot ts, GEMM row
+  // e, powers p1 = |Y_kk|^2, p2 = |Y_{31-kk}|^2).
   // `after_a()` runs once pass A has consumed its operands (prefetches for the next row).
   auto transform = [&](auto operand, auto sink, auto after_a, auto pair) {
     // ---- pass A
@@ -971,10 +980,9 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
               const float ar = acc[u][0][e], ai = acc[u][1][e];
               const float br = acc[u][2][e], bi = acc[u][3][e];
               if constexpr (kPairs) {
-                // kk = 0: B = 0 (row 0 of S is zero), so D = 0 and S = |Y_0|^2
-                const float s2 = fmaf(ar, ar, fmaf(ai, ai, fmaf(br, br, bi * bi)));
-                const float d = fmaf(ar, bi, -(ai * br));
-                pair(ts, e, q, kk, s2, d);
+                // kk = 0: B = 0 (row 0 of S is zero), so p2 = p1 = |Y_0|^2
+                const float y1r = ar + bi, y1i = ai - br, y2r = ar - bi, y2i = ai + br;
+                pair(ts, e, q, kk, fmaf(y1r, y1r, y1i * y1i), fmaf(y2r, y2r, y2i * y2i));
               } else {
                 const float y1r = ar + bi, y1i = ai - br;  // Y_kk
                 const float p1 = y1r * y1r + y1i * y1i;
@@ -1065,7 +1073,7 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
     if constexpr (kKeepX) {
 #pragma unroll
       for (int r = 0; r < 33; ++r)
-        xr[r] = tid < NBA ? buffer_load_f2(xv, tid * 8, r * NBA * 8) : make_float2(0.f, 0.f);
+        xr[r] = buffer_load_f2(xv, voa, r * NBA * 8);
     }
     // With X in registers the next PRN's replica inputs are loaded right after pass A, so
     // their latency hides behind passes B and C.
@@ -1077,7 +1085,7 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
       const __amdgpu_buffer_rsrc_t rv = rep_view(prn);
 #pragma unroll
       for (int r = 0; r < 33; ++r)
-        rn[r] = tid < NBA ? buffer_load_f2(rv, tid * 8, r * NBA * 8) : make_float2(0.f, 0.f);
+        rn[r] = buffer_load_f2(rv, voa, r * NBA * 8);
     };
     if constexpr (kKeepX) {
       if (p0 < p1) load_rep(p0);
@@ -1126,7 +1134,7 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
       const __amdgpu_buffer_rsrc_t yv = buffer_view(a.Y + sidx * P, P * 8);
 #pragma unroll
       for (int r = 0; r < 33; ++r)
-        yn[r] = tid < NBA ? buffer_load_f2(yv, tid * 8, r * NBA * 8) : make_float2(0.f, 0.f);
+        yn[r] = buffer_load_f2(yv, voa, r * NBA * 8);
     };
     if constexpr (kPrefetch) {
       if (static_cast<long long>(blockIdx.x) < a.n_sub) load_sub(blockIdx.x);
@@ -1145,8 +1153,8 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
       // the largest key, so it reaches the row's key and is flagged there)
       unsigned long long km = 0ull;
       // kPairs: each output pair's larger power (its f32 bits: powers are >= 0, so the bits
-      // order like the values and a NaN has the largest) with the sign of its D, and the
-      // thread's maximum over them
+      // order like the values and a NaN has the largest) with the difference of the pair's two
+      // powers, and the thread's maximum over them
       unsigned int pmx = 0u;
       unsigned int pmb[kPairs ? 5 : 1][4];
       float pdd[kPairs ? 5 : 1][4];
@@ -1179,24 +1187,24 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
               if (sidx + gridDim.x < a.n_sub) load_sub(sidx + gridDim.x);
             }
           },
-          [&](int ts, int e, int q, int kk, float s2, float d) {
+          [&](int ts, int e, int q, int kk, float p1, float p2) {
             if constexpr (kPairs) {
               if (ts >= 5) return;  // tile slots 0..4 (wave 0 takes 5 tiles, the others 4)
-              const unsigned int b = __float_as_uint(fmaf(2.0f, fabsf(d), s2));
+              const unsigned int b = max(__float_as_uint(p1), __float_as_uint(p2));
               pmb[ts][e] = b;
-              pdd[ts][e] = d;
+              pdd[ts][e] = p1 - p2;
               pmx = max(pmx, b);
               if constexpr (DUMP) {
-                dump_at(out_idx(q, kk), fmaf(2.0f, d, s2));
-                if (kk > 0) dump_at(out_idx(q, 31 - kk), fmaf(-2.0f, d, s2));
+                dump_at(out_idx(q, kk), p1);
+                if (kk > 0) dump_at(out_idx(q, 31 - kk), p2);
               }
             }
           });
       if constexpr (kPairs) {
         // block maximum M of the pair maxima (one barrier, which also orders this sub-row's
         // pass-C reads before the next sub-row's pass-A writes), then each wave holding M finds
-        // the largest sample with power M among its pairs (D > 0: Y_kk, D < 0: Y_{31-kk},
-        // D = 0: both) and folds (M, sample) into the row's key: several waves holding M fold
+        // the largest sample with power M among its pairs (p1 > p2: Y_kk, p1 < p2: Y_{31-kk},
+        // equal: both) and folds (M, sample) into the row's key: several waves holding M fold
         // their own, and the atomic maximum keeps the last maximum
         const unsigned int wmax = wave_umax(pmx);
         if (lane == 0) red.pk[wave] = wmax;
@@ -1243,6 +1251,243 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
   }
 }
 
+
+// ---- Sub-row transform of the long rows (FP32 split variant 2) ----------------------------------
+// The sub-row transforms of k_rows<1, SUB> restructured for fewer barriers and less per-output
+// work, with the same prime-factor passes (radix 33 from the prefetched registers, radix 16,
+// radix 31 on the matrix cores):
+//  * pass B runs in place (a butterfly's 16 outputs go back to its inputs' slots), so a
+//    sub-row needs three workgroup barriers, not four;
+//  * each lane keeps only its best output pair (the larger of |Y_kk|^2 and |Y_{31-kk}|^2, its
+//    slot and which of the two), not a 64-bit key per output; an exact rescan runs only when a
+//    lane's maximum occurs twice;
+//  * the row index of each sub-row comes from the pre-pass's row map (no 64-bit divisions).
+// LDS positions (sample-space indices n1, n2, n3 -> k1, k2, k3 of 16368 = 33 x 16 x 31):
+//   after pass A   1023 n2 + 33 n3 + k1   (pass-A butterfly ta = 31 n2 + n3, output k1)
+//   after pass B   1023 k2 + 33 n3 + k1   (pass-B butterfly j = 33 n3 + k1, output k2)
+//   pass C         butterfly q = 33 k2 + k1 reads n3 = 0..30 at c31_pos(q) + 33 n3
+// In pass C the four 16-lane groups of a wave take the symmetric input pairs (j, 31 - j) as
+// j = 2s, 15 - 2s, 2s + 1, 14 - 2s at GEMM step s, the odd groups loading v_{31-j} first (their
+// sine column negated), so the two groups of each half-wave load addresses 16 x 33 x 8 B apart,
+// on disjoint LDS banks.
+__device__ __forceinline__ int c31_pos(int q) { return q + 990 * (q / 33); }
+
+template <int SUB, bool DUMP = false>
+__global__ __launch_bounds__(T) void k_subrows(RowArgs a) {
+  constexpr int NBA = P / 33;  // 496
+  constexpr int NBB = P / 16;  // 1023
+  constexpr int NBC = P / 31;  // 528
+  __shared__ float2 buf[P];
+  __shared__ float2 stw[256];
+  __shared__ RowRed<NW> red;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < 256; i += T) stw[i] = a.tw[i];
+  __syncthreads();
+  const int wave = tid >> 6, lane = tid & 63, ln16 = lane & 15, grp = lane >> 4;
+  // this lane's GEMM constants (A row k = ln16) and LDS offsets of its two loads per step
+  float cA[4], sA[4];
+  int off1[4], off2[4];
+  const bool odd = (grp & 1) != 0;
+#pragma unroll
+  for (int st = 0; st < 4; ++st) {
+    const int j = grp == 0 ? 2 * st : grp == 1 ? 15 - 2 * st : grp == 2 ? 2 * st + 1 : 14 - 2 * st;
+    const float2 w = tw_lds(stw, ((j * ln16) % 31) * NBC);  // exp(-2 pi i jk/31)
+    const int m = j == 0 ? 0 : 31 - j;
+    cA[st] = j == 0 ? 1.0f : w.x;
+    sA[st] = j == 0 ? 0.0f : (odd ? w.y : -w.y);
+    off1[st] = 33 * (odd ? m : j);
+    off2[st] = 33 * (odd ? j : m);
+  }
+  const bool z0 = grp == 0;  // step 0 of group 0 is j = 0: a_0 = v_0 alone
+  auto out_idx = [](int q, int r) { return pfa_wrap(pfa_base(q) + NBC * r); };
+  // sample of pair (tile slot ts, GEMM row e) of this lane with d = p1 - p2: Y_kk for d > 0,
+  // Y_{31-kk} for d < 0, the larger of the two samples for d = 0 (equal powers: last maximum)
+  auto pair_sample = [&](int ts, int e, float d) {
+    const int q = 16 * (wave + 8 * ts) + ln16;
+    const int kk = 4 * grp + e;
+    const int i1 = out_idx(q, kk);
+    const int i2 = kk > 0 ? out_idx(q, 31 - kk) : i1;
+    return d > 0.0f ? i1 : d < 0.0f ? i2 : max(i1, i2);
+  };
+  // lanes beyond the 496 pass-A butterflies load past the end of the view: 0, no branch
+  const int voa = tid < NBA ? tid * 8 : P * 8;
+  auto load_sub = [&](long long s, float2 (&y)[33]) {
+    const __amdgpu_buffer_rsrc_t yv = buffer_view(a.Y + s * P, P * 8);
+#pragma unroll
+    for (int r = 0; r < 33; ++r) y[r] = buffer_load_f2(yv, voa, r * NBA * 8);
+  };
+  const long long G = gridDim.x;
+  long long sidx = blockIdx.x;
+  if (sidx >= a.n_sub) return;
+  // `cur` holds the sub-row's pass-A inputs (loaded during the previous sub-row); they are
+  // transformed and stored at the top, and the registers reloaded with the next sub-row's
+  // inputs, which then have a whole sub-row to arrive.
+  float2 cur[33];
+  load_sub(sidx, cur);
+  for (;;) {
+    const long long s = sidx;
+    const long long rq = s / SUB;
+    const int k0 = static_cast<int>(s - rq * SUB);
+    const long long r = a.rowmap[rq];  // big_row(a.r0 + rq, ...), from the pre-pass
+    // ---- pass A (radix 33) -> LDS; the next sub-row's inputs in flight
+    Dft<33>::run(cur);
+    {
+      int ta = tid;
+      asm volatile("" : "+v"(ta));
+      if (ta < NBA) {
+#pragma unroll
+        for (int q = 0; q < 33; ++q) buf[ta * 33 + q] = cur[q];
+      }
+    }
+    if (s + G < a.n_sub) load_sub(s + G, cur);
+    lds_barrier();
+    // ---- pass B (radix 16) in place
+    {
+      int tb = tid;
+      asm volatile("" : "+v"(tb));
+      // one butterfly after the other (the memory clobber keeps the second one's loads after
+      // the first one's stores): 32 VGPRs of values instead of 64
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int j = tb + b * T;
+        if (b == 0 || j < NBB) {
+          float2 v[16];
+#pragma unroll
+          for (int q = 0; q < 16; ++q) v[q] = buf[j + q * NBB];
+          Dft<16>::run(v);
+#pragma unroll
+          for (int q = 0; q < 16; ++q) buf[j + q * NBB] = v[q];
+        }
+        asm volatile("" ::: "memory");
+      }
+    }
+    lds_barrier();
+    // ---- pass C (radix 31 on the matrix cores).  Each lane keeps its largest pair maximum max(p1, p2) (f32 bits: powers are >= 0, so the
+    // bits order like the values and a NaN has the largest), the pair's slot 4 ts + e and
+    // p1 - p2; `tmask` marks the lanes whose maximum another of their pairs equals (no branch here:
+    // such a tie is resolved at the end of the sub-row by an exact rescan).
+    unsigned int pmx = 0u;
+    int psl = 0;
+    float pdv = 0.0f;
+    unsigned long long tmask = 0ull;
+    int gc = grp;
+    asm volatile("" : "+v"(gc));
+    auto dump_at = [&](int idx, float p) {
+      if constexpr (DUMP) a.dump[static_cast<size_t>(r) * (SUB * P) + k0 + SUB * idx] = p;
+    };
+    // the 16 MFMAs of tile slot TS (tile wave + 8 TS) into acc
+    auto tile_mfma = [&](auto ts_c, floatx4 (&acc)[4]) {
+      constexpr int TS = decltype(ts_c)::value;
+      const int base = c31_pos(16 * (wave + 8 * TS) + ln16);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[c] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int st = 0; st < 4; ++st) {
+        const float2 x1 = buf[base + off1[st]];
+        float2 x2 = buf[base + off2[st]];
+        if (st == 0 && z0) x2 = make_float2(0.f, 0.f);
+        acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(cA[st], x1.x + x2.x, acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(cA[st], x1.y + x2.y, acc[1], 0, 0, 0);
+        acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(sA[st], x1.x - x2.x, acc[2], 0, 0, 0);
+        acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(sA[st], x1.y - x2.y, acc[3], 0, 0, 0);
+      }
+    };
+    // powers p1 = |Y_kk|^2 = |A - iB|^2 and p2 = |Y_{31-kk}|^2 = |A + iB|^2 of pair e of a tile's
+    // accumulators (each from its own output, so a small power carries no cancellation error)
+    auto pair_pw = [](const floatx4 (&acc)[4], int e, float& p1, float& p2) {
+      const float ar = acc[0][e], ai = acc[1][e];
+      const float br = acc[2][e], bi = acc[3][e];
+      const float y1r = ar + bi, y1i = ai - br, y2r = ar - bi, y2i = ai + br;
+      p1 = fmaf(y1r, y1r, y1i * y1i);
+      p2 = fmaf(y2r, y2r, y2i * y2i);
+    };
+    // one tile: its MFMAs, then its outputs' pair maxima
+    auto tile = [&](auto ts_c) {
+      constexpr int TS = decltype(ts_c)::value;
+      floatx4 acc[4];
+      tile_mfma(ts_c, acc);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        float p1, p2;
+        pair_pw(acc, e, p1, p2);
+        // the pair's maximum (bits: a NaN has the largest); d > 0: Y_kk, d < 0: Y_{31-kk}
+        const unsigned int bits = max(__float_as_uint(p1), __float_as_uint(p2));
+        const float d = p1 - p2;
+        const bool up = bits > pmx;
+        tmask = (tmask | __ballot(bits == pmx)) & ~__ballot(up);
+        pmx = up ? bits : pmx;
+        psl = up ? 4 * TS + e : psl;
+        pdv = up ? d : pdv;
+        if constexpr (DUMP) {
+          const int q = 16 * (wave + 8 * TS) + ln16;
+          const int kk = 4 * gc + e;
+          dump_at(out_idx(q, kk), p1);
+          if (kk > 0) dump_at(out_idx(q, 31 - kk), p2);
+        }
+      }
+    };
+    tile(std::integral_constant<int, 0>{});
+    tile(std::integral_constant<int, 1>{});
+    tile(std::integral_constant<int, 2>{});
+    tile(std::integral_constant<int, 3>{});
+    if (wave == 0) tile(std::integral_constant<int, 4>{});  // tile 32
+    // ---- end of the sub-row: block maximum M of the lanes' maxima (the barrier also orders
+    // this sub-row's pass-C loads before the next sub-row's pass-A store); the waves holding M
+    // fold (M, largest sample with power M) into the row's key (several waves: several atomic
+    // maxima, the largest sample wins)
+    const unsigned int wmax = wave_umax(pmx);
+    const bool wtie = (__ballot(pmx == wmax) & tmask) != 0ull;
+    if (lane == 0) {
+      red.pk[wave] = wmax;
+      red.bad[wave] = wtie ? 1 : 0;
+    }
+    lds_barrier();
+    const bool mine = lane < NW;
+    const unsigned int M = static_cast<unsigned int>(__builtin_amdgcn_readlane(
+        static_cast<int>(row16_umax(mine ? red.pk[lane] : 0u)), 0));
+    const bool tie = __ballot(mine && red.pk[mine ? lane : 0] == M &&
+                              red.bad[mine ? lane : 0] != 0) != 0ull;
+    unsigned int cand = 0u;  // 1 + sample, 0 for none
+    if (tie) {
+      // Rare (block-uniform): a lane holding M reached it in more than one pair.  Recompute the
+      // wave's tiles (the pass-B values are still in LDS: no wave stores the next sub-row's
+      // pass A before the barrier below) and take every pair of power M.
+      if (wmax == M) {
+        auto rescan = [&](auto ts_c) {
+          constexpr int TS = decltype(ts_c)::value;
+          floatx4 acc[4];
+          tile_mfma(ts_c, acc);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            float p1, p2;
+            pair_pw(acc, e, p1, p2);
+            if (max(__float_as_uint(p1), __float_as_uint(p2)) == M)
+              cand = max(cand, static_cast<unsigned int>(pair_sample(TS, e, p1 - p2) + 1));
+          }
+        };
+        rescan(std::integral_constant<int, 0>{});
+        rescan(std::integral_constant<int, 1>{});
+        rescan(std::integral_constant<int, 2>{});
+        rescan(std::integral_constant<int, 3>{});
+        if (wave == 0) rescan(std::integral_constant<int, 4>{});
+      }
+      lds_barrier();
+    } else if (pmx == M) {
+      cand = static_cast<unsigned int>(pair_sample(psl >> 2, psl & 3, pdv) + 1);
+    }
+    if (wmax == M) {
+      const unsigned int iw = wave_umax(cand);
+      if (lane == 0) {
+        atomicMax(a.rowkey + r,
+                  (static_cast<unsigned long long>(M) << 32) |
+                      static_cast<unsigned int>(k0 + SUB * (static_cast<int>(iw) - 1)));
+        if (M > 0x7f800000u) atomicOr(a.rownan + r, 1);
+      }
+    }
+    sidx += G;
+    if (sidx >= a.n_sub) break;
+  }
+}
 
 // ---- FP16 form of the rows kernel ------------------------------------------------------------
 // Operands X and R are stored as f16 (re, im) pairs scaled by kF16Scale, conj(X) . R is formed
@@ -2047,14 +2292,15 @@ __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
                                                  long long r0, long long n_rows,
                                                  void* __restrict__ Yv,
                                                  unsigned long long* __restrict__ rowkey,
-                                                 int* __restrict__ rownan) {
+                                                 int* __restrict__ rownan,
+                                                 int* __restrict__ rowmap) {
   const long long rr = blockIdx.x >> 5;
   if (rr >= n_rows) return;
   const int chunk = 4 * static_cast<int>(blockIdx.x & 7) + static_cast<int>((blockIdx.x >> 3) & 3);
   const int p0 = 512 * chunk;
   fused::big_pre_positions<R0, F16>(X, rep, twb, n_prn, n_bin, n_ant, hs, tile, r0, rr, p0,
                              p0 + 512 < kFusedP ? p0 + 512 : kFusedP,
-                             static_cast<int>(threadIdx.x), Yv, rowkey, rownan);
+                             static_cast<int>(threadIdx.x), Yv, rowkey, rownan, rowmap);
 }
 
 // Row keys of long rows -> RowOut (p2 is filled later from the winning row only).
@@ -2433,30 +2679,40 @@ std::vector<float2> big_twiddles(size_t P) {
 
 void launch_big_pre(const float2* X, const float2* rep, const float2* twb, int R0, int n_prn,
                     int n_bin, int n_ant, int hs, int tile, long long r0, long long r1, void* Y,
-                    bool f16, unsigned long long* rowkey, int* rownan, hipStream_t st) {
+                    bool f16, unsigned long long* rowkey, int* rownan, int* rowmap,
+                    hipStream_t st) {
   const long long nr = r1 - r0;
   const long long total = nr;  // rows of this chunk, 32 blocks each
   const dim3 grid(static_cast<unsigned>(nr * 32));
   if (R0 == 4 && !f16)
     hipLaunchKernelGGL((k_big_pre<4, false>), grid, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
-                       n_ant, hs, tile, r0, total, Y, rowkey, rownan);
+                       n_ant, hs, tile, r0, total, Y, rowkey, rownan, rowmap);
   else if (R0 == 4)
     hipLaunchKernelGGL((k_big_pre<4, true>), grid, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
-                       n_ant, hs, tile, r0, total, Y, rowkey, rownan);
+                       n_ant, hs, tile, r0, total, Y, rowkey, rownan, rowmap);
   else if (R0 == 10 && !f16)
     hipLaunchKernelGGL((k_big_pre<10, false>), grid, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
-                       n_ant, hs, tile, r0, total, Y, rowkey, rownan);
+                       n_ant, hs, tile, r0, total, Y, rowkey, rownan, rowmap);
   else if (R0 == 10)
     hipLaunchKernelGGL((k_big_pre<10, true>), grid, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
-                       n_ant, hs, tile, r0, total, Y, rowkey, rownan);
+                       n_ant, hs, tile, r0, total, Y, rowkey, rownan, rowmap);
   else
     fail(TGA_ERR_UNSUPPORTED, "long-period split supports 4 and 10 sub-rows");
   TGA_HIP(hipGetLastError());
 }
 
+template <int R0>
+void launch_subrows(bool dump, unsigned blocks, hipStream_t st, const fused::RowArgs& a) {
+  if (dump)
+    hipLaunchKernelGGL((fused::k_subrows<R0, true>), dim3(blocks), dim3(fused::T), 0, st, a);
+  else
+    hipLaunchKernelGGL((fused::k_subrows<R0, false>), dim3(blocks), dim3(fused::T), 0, st, a);
+}
+
 void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, int hs, int tile,
                     long long r0, long long r1, const void* Y, bool f16, unsigned long long* rowkey,
-                    int* rownan, RowOut* rows, hipStream_t st, int variant, float* dump) {
+                    int* rownan, const int* rowmap, RowOut* rows, hipStream_t st, int variant,
+                    float* dump) {
   const long long nr = r1 - r0;
   const long long n_sub = nr * R0;
   const unsigned blocks = static_cast<unsigned>(n_sub < 2048 ? n_sub : 2048);
@@ -2498,9 +2754,14 @@ void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, i
     a.r0 = r0;
     a.rowkey = rowkey;
     a.rownan = rownan;
+    a.rowmap = rowmap;
     a.dump = dump;
     // the surface form runs the plan's radix-31 implementation, so its search is the same
-    if (dump && R0 == 4 && variant == 0)
+    if (variant == 2 && R0 == 4)
+      launch_subrows<4>(dump != nullptr, blocks, st, a);
+    else if (variant == 2)
+      launch_subrows<10>(dump != nullptr, blocks, st, a);
+    else if (dump && R0 == 4 && variant == 0)
       hipLaunchKernelGGL((fused::k_rows<0, 4, false, true>), dim3(blocks), dim3(fused::T), 0, st,
                          a);
     else if (dump && R0 == 4)

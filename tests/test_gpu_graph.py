@@ -38,6 +38,12 @@ CASES = {
                "split4x16368"),
     "split10_unfolded": (FS, 163680, 1, 1, [("beidou", [30], grid(IF, 500.0, 500.0), 0)],
                          "fp32", "split10x16368"),
+    "subrows4": (FS, 2 * 65472, 1, 1, [("galileo", [3, 11], grid(IF, 500.0, 500.0), 0)],
+                   "fp32", "split4x16368"),
+    "subrows10_unfolded": (FS, 163680, 1, 1, [("beidou", [30], grid(IF, 500.0, 500.0), 0)],
+                             "fp32", "split10x16368"),
+    "subrows10": (FS, 2 * 163680, 1, 2, [("l1c", [5, 9], grid(IF, 500.0, 500.0), 0)],
+                  "fp32", "split10x16368"),
     "fourstep4": (FS, 2 * 65472, 1, 1, [("galileo", [3, 11], grid(IF, 500.0, 500.0), 0)],
                   "fp32", "split132x496"),
     "fourstep10_unfolded": (FS, 163680, 1, 1, [("beidou", [30], grid(IF, 500.0, 500.0), 0)],
@@ -55,15 +61,17 @@ def signals_for(systems, fs, seed):
     out = []
     for name, prns, fc, _ in systems:
         P = int(round(fs * {"gps": 1e-3, "sbas": 1e-3, "qzss": 1e-3, "galileo": 4e-3,
-                            "beidou": 1e-2}[name]))
+                            "beidou": 1e-2, "l1c": 1e-2}[name]))
         out.append(dict(system=name, prn=prns[0], cn0=52.0, doppler=float(rng.choice(fc)) - IF,
                         delay=int(rng.integers(0, P)), phase=float(rng.uniform(0, 6.28))))
     return out
 
 
-# the split cases pin their FP32 form: the sub-row transforms (matrix-core radix 31) or the
-# four-step form
-SPLIT_FORM = {"split4": "1", "split10_unfolded": "1", "fourstep4": "3", "fourstep10_unfolded": "3"}
+# the split cases pin their FP32 form: the sub-row transforms of k_rows (matrix-core radix 31)
+# or k_subrows, or the four-step form
+SPLIT_FORM = {"split4": "1", "split10_unfolded": "1", "subrows4": "2",
+              "subrows10_unfolded": "2", "subrows10": "2", "fourstep4": "3",
+              "fourstep10_unfolded": "3"}
 
 
 def make_plan(case):

@@ -59,10 +59,12 @@ def fused_variant(request, monkeypatch):
     return request.param
 
 
-@pytest.fixture(params=[0, 1, 3], ids=["split-valu31", "split-mfma31", "split-fourstep"])
+@pytest.fixture(params=[0, 1, 2, 3],
+                ids=["split-valu31", "split-mfma31", "split-subrows", "split-fourstep"])
 def split_variant(request, monkeypatch):
     """Every FP32 form of the long rows: R0 sub-rows of 16368 with either radix-31
-    implementation, and the four-step N1 x 496 form."""
+    implementation (k_rows), the sub-row transform that keeps each lane's best output pair
+    (k_subrows), and the four-step N1 x 496 form."""
     monkeypatch.setenv("TGA_SPLIT_VARIANT", str(request.param))
     return request.param
 
@@ -255,11 +257,12 @@ def test_split_tile_shapes_give_identical_peaks(gpu, monkeypatch):
             assert check_against_oracle(plan, [pk], packed, fs, L, 1, systems) == 20
         for f in pk.dtype.names:
             assert np.array_equal(pk[f], base4[f]), (t, f)
-    # unpinned, the autotuner times the three FP32 forms x 5 shapes
+    # unpinned, the autotuner times the four FP32 forms x 5 shapes
     monkeypatch.delenv("TGA_SPLIT_TILE")
     monkeypatch.delenv("TGA_SPLIT_VARIANT")
     desc = tga.Plan(fs, 2, L, [tga.SystemSearch(n, f, p) for n, p, f in systems]).describe()
-    for c in ("v1t8x8:", "v1t4x16:", "v0t16x4:", "v0t2x32:", "v1t32x2:", "v3t8x8:", "v3t4x16:"):
+    for c in ("v2t8x8:", "v2t32x2:", "v1t8x8:", "v1t4x16:", "v0t16x4:", "v0t2x32:", "v1t32x2:",
+              "v3t8x8:", "v3t4x16:"):
         assert c in desc, desc
     assert "v3t2x32:" not in desc  # the four-step form is timed on two shapes
 
@@ -365,3 +368,25 @@ def test_split_unfolded_10ms_b1c(gpu, split_variant):
     assert split_path(split_variant, 10) in plan.describe()
     check_against_oracle(plan, peaks, packed, fs, L, 1, systems)
     assert int(peaks[0][0, 0]["best_codephase"]) == (163680 - 12345) % 163680
+
+
+@pytest.mark.parametrize("system", ["gps", "galileo", "beidou"])
+def test_constant_observation_every_power_ties(gpu, split_variant, system):
+    # A constant record is all zeros after the whole-record de-mean, so every correlation power
+    # is exactly 0 on both sides: the reference's last-maximum rules then fix the answer (last
+    # sample of the row, last bin, [c,c]) with no tolerance, and the sub-row transforms take
+    # their exact tie path for every row.
+    fs = FS
+    P = ocodes.samples_per_period(system, fs)
+    L = 2 * P
+    fc = grid(IF, 500.0, 500.0)
+    systems = [(system, [3, 4], fc)]
+    plan = tga.Plan(fs, 1, L, [tga.SystemSearch(n, f, p) for n, p, f in systems], input_format=1)
+    packed = np.full((1, plan.row_bytes), 0xFF, dtype=np.uint8)
+    peaks = plan.search(packed)
+    levels = synth.unpack_levels(packed, L, 1)
+    for pi, prn in enumerate(plan.prns[0]):
+        code = ocodes.code_resampled(system, prn, float(P), L)
+        ref, *_ = oracle_search(levels[0], fs, P, L, fc, code)
+        assert_peak_matches(peaks[0][0, pi], ref, P, (False, False), what=f"{system} {prn}")
+        assert int(peaks[0][0, pi]["best_freq_idx"]) == len(fc) - 1
