@@ -347,8 +347,22 @@ void Plan::build(const tga_plan_desc& d) {
       autotune_split(g);
       // the sub-row forms read their spectra segment by segment in the prime-factor order of
       // 16368; the four-step form reads them in natural order
-      if (!g.natural())
+      if (!g.natural()) {
         rep_to_pfa(g, static_cast<long long>(n_prn) * 2 * static_cast<long long>(M));
+        // their forward spectra come from the two-step LDS forward (launch_fwd_long): the
+        // carrier in its decimated order (f64 phase in the FP16 form), tau as [r][m]
+        g.d_carrier.alloc(static_cast<size_t>(n_bin) * (g.folded ? 2 : 1) * g.P * sizeof(float2));
+        launch_carrier_dec(g.d_fc.as<double>(), n_bin, g.big_r0, g.folded, phase_const, g.big16,
+                           g.d_carrier.as<float2>(), stream);
+        if (g.folded) {
+          std::vector<float2> td(g.P);
+          for (int r = 0; r < g.big_r0; ++r)
+            for (size_t m = 0; m < kFusedP; ++m) td[r * kFusedP + m] = tau[g.big_r0 * m + r];
+          upload(g.d_tau, td, stream);
+        }
+        upload(g.d_twl, long_twiddles(g.P), stream);
+        TGA_HIP(hipStreamSynchronize(stream));
+      }
     } else {
       g.d_X.alloc(g.n_xrow * M * sizeof(float2));
       g.d_Xs.alloc(g.n_xrow * M * sizeof(float2));
@@ -631,6 +645,11 @@ void Plan::search(const uint8_t* d_in, hipStream_t st, float* dump) {
       launch_fused_forward(d_in, row_bytes, fmt, d_means.as<double>(), g.d_fc.as<double>(),
                            n_ant, g.n_bin, phase_const, g.d_tau.as<float2>(),
                            g.d_tw.as<float2>(), g.d_carrier.as<float2>(), g.d_X.as<float2>(), st);
+      g.X = g.d_X.as<float2>();
+    } else if (g.big && !g.natural()) {
+      launch_fwd_long(d_in, row_bytes, fmt, d_means.as<double>(), g.d_carrier.as<float2>(),
+                      g.d_tau.as<float2>(), g.d_tw.as<float2>(), g.d_twl.as<float2>(), g.big_r0,
+                      g.folded, n_ant, g.n_bin, g.d_Xs.as<float2>(), g.d_X.as<float2>(), st);
       g.X = g.d_X.as<float2>();
     } else {
       if (g.folded)

@@ -29,10 +29,13 @@ struct Group {
   float scale = 1.f;
   int cw = 1;
   DevBuf d_fc, d_slots, d_tau, d_rep, d_X, d_Xs, d_rows, d_Z, d_Zs, d_tw;
-  DevBuf d_carrier;  // fused groups: carrier (cos, sin) per (bin, sample), built at plan creation
+  // carrier (cos, sin) per (bin, sample), built at plan creation: fused groups [bin][sample],
+  // sub-row split groups in the decimated order of launch_fwd_long
+  DevBuf d_carrier;
   DevBuf d_rep16, d_X16;  // FP16 form: f16 (re, im) pairs scaled by kF16Scale
   DevBuf d_Y, d_rowkey, d_rownan, d_rowmap, d_twb, d_win, d_Zw, d_Zws;  // long-period split
   DevBuf d_tw4, d_c31;  // four-step split: W_P^{n2 k1} as [k1][n2], radix-31 cos / sin
+  DevBuf d_twl;  // sub-row split: W_P^e two-level table of the long-period forward
   float2* X = nullptr;  // where the forward spectra of the last search landed
   FusedRowsConfig rows_cfg;  // rows-kernel shape picked by the autotuner
   std::vector<std::pair<FusedRowsConfig, double>> tune;  // candidates timed, ms per launch

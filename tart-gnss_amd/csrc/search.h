@@ -96,6 +96,18 @@ void launch_fused_rows16(const uint32_t* X16, const uint32_t* rep16, const float
 // Fills `bytes` of autotuner spectra with a fixed pseudo-random pattern of values in [-1, 1)
 // (f32 words, or f16 pairs with f16 = true).
 void launch_fill_tune(void* p, size_t bytes, bool f16, hipStream_t st);
+// Forward spectra of long code periods P = R0 x 16368 (R0 = 4 or 10), folded (N = 2P: rows
+// (ant, bin, h)) or not (N = P: rows (ant, bin)), in two steps: one 16368-point LDS transform
+// per (row, r) of the samples R0 m + r into Y, then the twiddles and R0-point DFTs into X, in
+// the segment-wise prime-factor order of the split.  carrier: launch_carrier_dec's table; tau:
+// tau_n = exp(-i pi n / P) in the order [r][m] (n = R0 m + r); twl: long_twiddles(P).
+void launch_carrier_dec(const double* fc, int n_bin, int R0, bool fold, double phase_const,
+                        bool phase64, float2* out, hipStream_t st);
+std::vector<float2> long_twiddles(size_t P);
+void launch_fwd_long(const uint8_t* packed, size_t row_bytes, int fmt, const double* means,
+                     const float2* carrier, const float2* tau, const float2* tw,
+                     const float2* twl, int R0, bool fold, int n_ant, int n_bin, float2* Y,
+                     float2* X, hipStream_t st);
 // n values = n / 16368 natural-order rows -> f16 pairs times scale in the prime-factor order
 void launch_to_f16(const float2* in, uint32_t* out, long long n, float scale, hipStream_t st);
 // Twiddle table of the fused kernels: W_P^i (i < 128) then W_P^{128 i} (i < 128).

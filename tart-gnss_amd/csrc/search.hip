@@ -327,6 +327,28 @@ __device__ __forceinline__ void passes_to_last(float2* buf, const float2* stw, i
   lds_pass<3, 341>(buf, stw, tid);
 }
 
+// The 16368-point forward DFT of buf in place (Stockham passes 31, 11, 3 and a last radix 16
+// after 1023 points), output in natural order; the caller synchronises before and after.
+__device__ __forceinline__ void lds_fft_natural(float2* buf, const float2* stw, int tid) {
+  passes_to_last(buf, stw, tid);
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    const int j = tid + b * T;
+    if (j < NB4) {
+      float2 v[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) v[r] = buf[j + r * NB4];
+#pragma unroll
+      for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], tw_lds(stw, r * j));
+      Dft<16>::run(v);
+      // back into LDS in natural order: slots j + r NB4 are read and written by this thread
+      // only, so no barrier is needed before the writes
+#pragma unroll
+      for (int r = 0; r < 16; ++r) buf[j + r * NB4] = v[r];
+    }
+  }
+}
+
 struct FwdArgs {
   const uint8_t* packed;
   size_t row_bytes;
@@ -403,24 +425,7 @@ __global__ __launch_bounds__(T) void k_forward(FwdArgs a) {
     }
   }
   __syncthreads();
-  passes_to_last(buf, stw, tid);
-  // last pass: radix 16 after 1023, output in natural order
-#pragma unroll
-  for (int b = 0; b < 2; ++b) {
-    const int j = tid + b * T;
-    if (j < NB4) {
-      float2 v[16];
-#pragma unroll
-      for (int r = 0; r < 16; ++r) v[r] = buf[j + r * NB4];
-#pragma unroll
-      for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], tw_lds(stw, r * j));
-      Dft<16>::run(v);
-      // back into LDS in natural order: slots j + r NB4 are read and written by this thread
-      // only, so no barrier is needed before the writes
-#pragma unroll
-      for (int r = 0; r < 16; ++r) buf[j + r * NB4] = v[r];
-    }
-  }
+  lds_fft_natural(buf, stw, tid);
   // X is stored in the prime-factor order of the rows kernels, written coalesced
   __syncthreads();
   if constexpr (F16) {
@@ -429,6 +434,135 @@ __global__ __launch_bounds__(T) void k_forward(FwdArgs a) {
   } else {
     float2* out = a.X + static_cast<size_t>(xrow) * P;
     for (int s = tid; s < P; s += T) out[s] = buf[pfa_nat(s)];
+  }
+}
+
+// ---- Forward spectra of long code periods (P = R0 x 16368), four-step through LDS ------------
+// With n = R0 m + r (m < 16368, r < R0) and k = t + 16368 k0 (t < 16368, k0 < R0),
+//   X[t + 16368 k0] = sum_r W_R0^{r k0} (W_P^{r t} Y_r[t]),   Y_r = FFT_16368(x[R0 m + r]),
+// so step 1 (k_fwd_long) runs one LDS transform per (antenna, bin, fold, r) of the decimated,
+// decoded, wiped and folded samples, and step 2 (k_fwd_twist) the twiddles and the R0-point
+// DFTs, writing X segment by segment in the prime-factor order the split's pre-pass reads
+// (position 16368 k0 + s holds natural index 16368 k0 + pfa_nat(s)).  The carrier comes from
+// a plan-time table in the decimated order, [bin][half][r][m] for sample n = half P + R0 m + r
+// (same rounding as k_carrier), and the twist tau_n = exp(-i pi n / P) from [r][m].
+struct LongFwdArgs {
+  const uint8_t* packed;
+  size_t row_bytes;
+  int fmt;
+  const double* means;
+  const float2* carrier;  // [n_bin][nh][R0][16368]
+  const float2* tau;      // [R0][16368] (folded rows)
+  const float2* tw;       // fused twiddles of the LDS passes
+  int n_ant;
+  int n_bin;
+  float2* Y;  // [xrow][R0][16368] in the prime-factor order of 16368
+};
+
+template <int R0, bool FOLD>
+__global__ __launch_bounds__(T) void k_fwd_long(LongFwdArgs a) {
+  constexpr int NH = FOLD ? 2 : 1;
+  __shared__ float2 buf[P];
+  __shared__ float2 stw[256];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < 256; i += T) stw[i] = a.tw[i];
+  // blocks (bin, r, ant, h), ant and h fastest so that the blocks reading one slice of the
+  // carrier table run together; each XCD takes one contiguous share of them
+  const int nblk = a.n_bin * R0 * a.n_ant * NH;
+  const int per_xcd = (nblk + 7) / 8;
+  const int lb = static_cast<int>(blockIdx.x & 7) * per_xcd + static_cast<int>(blockIdx.x >> 3);
+  if (lb >= nblk) return;
+  const int h = FOLD ? lb % NH : 0;
+  const int ant = (lb / NH) % a.n_ant;
+  const int br = lb / (NH * a.n_ant);
+  const int r = br % R0;
+  const int bin = br / R0;
+  const uint8_t* row = a.packed + static_cast<size_t>(ant) * a.row_bytes;
+  const double mean = a.means[ant];
+  const float2* c0 = a.carrier + (static_cast<size_t>(bin) * NH * R0 + r) * P;
+  const float2* tau = a.tau + static_cast<size_t>(r) * P;
+  auto level = [&](long long n) {
+    return __double2float_rn(static_cast<double>(decode_level(row, a.fmt, n)) - mean);
+  };
+  for (int m = tid; m < P; m += T) {
+    const long long n = static_cast<long long>(R0) * m + r;
+    const float l0 = level(n);
+    const float2 w0 = c0[m];
+    const float2 x0 = make_float2(w0.x * l0, w0.y * l0);
+    if constexpr (FOLD) {
+      const float l1 = level(n + static_cast<long long>(R0) * P);
+      const float2 w1 = c0[static_cast<size_t>(R0) * P + m];
+      const float2 x1 = make_float2(w1.x * l1, w1.y * l1);
+      buf[m] = h == 0 ? cadd(x0, x1) : cmul(csub(x0, x1), tau[m]);
+    } else {
+      buf[m] = x0;
+    }
+  }
+  __syncthreads();
+  lds_fft_natural(buf, stw, tid);
+  __syncthreads();
+  const size_t xrow = (static_cast<size_t>(ant) * a.n_bin + bin) * NH + h;
+  float2* out = a.Y + (xrow * R0 + r) * P;
+  for (int s = tid; s < P; s += T) out[s] = buf[pfa_nat(s)];
+}
+
+// Step 2: per position s of a row (natural index t = pfa_nat(s)), the R0 values Y_r[s] times
+// W_P^{r t} (from the two-level table twl: W_P^e = twl[e & 1023] twl[1024 + (e >> 10)]), an
+// R0-point DFT, and X[xrow][k0][s]; two positions per thread with 16-byte accesses.
+template <int R0>
+__global__ __launch_bounds__(256) void k_fwd_twist(const float2* __restrict__ Y,
+                                                   const float2* __restrict__ twl,
+                                                   long long n_pairs, float2* __restrict__ X) {
+  constexpr int PL = R0 * P;
+  const long long gid = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (gid >= n_pairs) return;
+  const long long xrow = gid / (P / 2);
+  const int s = 2 * static_cast<int>(gid - xrow * (P / 2));
+  const float4* y = reinterpret_cast<const float4*>(Y + xrow * PL + s);
+  const int t0 = pfa_nat(s), t1 = pfa_nat(s + 1);
+  float2 v[R0], u[R0];
+  int e0 = 0, e1 = 0;  // r t mod PL
+#pragma unroll
+  for (int r = 0; r < R0; ++r) {
+    const float4 q = y[static_cast<long long>(r) * (P / 2)];
+    v[r] = make_float2(q.x, q.y);
+    u[r] = make_float2(q.z, q.w);
+    if (r > 0) {
+      v[r] = cmul(v[r], cmul(twl[e0 & 1023], twl[1024 + (e0 >> 10)]));
+      u[r] = cmul(u[r], cmul(twl[e1 & 1023], twl[1024 + (e1 >> 10)]));
+    }
+    e0 += t0;
+    e0 = e0 >= PL ? e0 - PL : e0;
+    e1 += t1;
+    e1 = e1 >= PL ? e1 - PL : e1;
+  }
+  Dft<R0>::run(v);
+  Dft<R0>::run(u);
+  float4* x = reinterpret_cast<float4*>(X + xrow * PL + s);
+#pragma unroll
+  for (int k0 = 0; k0 < R0; ++k0)
+    x[static_cast<long long>(k0) * (P / 2)] = make_float4(v[k0].x, v[k0].y, u[k0].x, u[k0].y);
+}
+
+// Carrier (cos, sin) of sample n = half P + R0 m + r in the decimated order [bin][half][r][m]
+// of k_fwd_long, rounded like k_carrier (f32 phase, or the f64 phase of the FP16 form).
+template <bool PHASE64>
+__global__ void k_carrier_dec(const double* __restrict__ fc, double pc, int R0, int nh,
+                              long long total, float2* __restrict__ out) {
+  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= total) return;
+  const long long per_bin = static_cast<long long>(nh) * R0 * P;
+  const long long bin = i / per_bin;
+  const long long j = i - bin * per_bin;
+  const long long half = j / (static_cast<long long>(R0) * P);
+  const long long k = j - half * R0 * P;
+  const long long r = k / P, m = k - r * P;
+  const long long n = half * R0 * P + R0 * m + r;
+  const double f = fc[bin];
+  if constexpr (PHASE64) {
+    out[i] = cossin_f64(__dmul_rn(__dmul_rn(pc, static_cast<double>(n)), f));
+  } else {
+    out[i] = wipe_value(1.0f, __dmul_rn(pc, static_cast<double>(n)), f);
   }
 }
 
@@ -1272,6 +1406,64 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
 // on disjoint LDS banks.
 __device__ __forceinline__ int c31_pos(int q) { return q + 990 * (q / 33); }
 
+// Pass C (radix 31) of k_subrows and k_rowsx, per lane: the GEMM constants of A row k = ln16
+// and the LDS offsets of the lane's two loads at each of the 4 GEMM steps (input pairs in the
+// bank-disjoint order above; step 0 of group 0 is j = 0, a_0 = v_0 alone).
+struct C31Lane {
+  float cA[4], sA[4];
+  int off1[4], off2[4];
+  bool z0;
+  __device__ __forceinline__ void init(const float2* stw, int ln16, int grp) {
+    const bool odd = (grp & 1) != 0;
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      const int j = grp == 0 ? 2 * st : grp == 1 ? 15 - 2 * st : grp == 2 ? 2 * st + 1 : 14 - 2 * st;
+      const float2 w = tw_lds(stw, ((j * ln16) % 31) * (P / 31));  // exp(-2 pi i jk/31)
+      const int m = j == 0 ? 0 : 31 - j;
+      cA[st] = j == 0 ? 1.0f : w.x;
+      sA[st] = j == 0 ? 0.0f : (odd ? w.y : -w.y);
+      off1[st] = 33 * (odd ? m : j);
+      off2[st] = 33 * (odd ? j : m);
+    }
+    z0 = grp == 0;
+  }
+  // the 16 MFMAs of the tile of butterflies q0..q0+15 (this lane: q = q0 + ln16): acc[0], acc[1]
+  // = C a (re, im), acc[2], acc[3] = S b (re, im); row kk = 4 grp + e of the tile in element e
+  __device__ __forceinline__ void tile(const float2* buf, int q, floatx4 (&acc)[4]) const {
+    const int base = c31_pos(q);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[c] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      const float2 x1 = buf[base + off1[st]];
+      float2 x2 = buf[base + off2[st]];
+      if (st == 0 && z0) x2 = make_float2(0.f, 0.f);
+      acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(cA[st], x1.x + x2.x, acc[0], 0, 0, 0);
+      acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(cA[st], x1.y + x2.y, acc[1], 0, 0, 0);
+      acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(sA[st], x1.x - x2.x, acc[2], 0, 0, 0);
+      acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(sA[st], x1.y - x2.y, acc[3], 0, 0, 0);
+    }
+  }
+};
+
+// powers p1 = |Y_kk|^2 = |A - iB|^2 and p2 = |Y_{31-kk}|^2 = |A + iB|^2 of row e of a tile's
+// accumulators (each from its own output, so a small power carries no cancellation error)
+__device__ __forceinline__ void c31_pair(const floatx4 (&acc)[4], int e, float& p1, float& p2) {
+  const float ar = acc[0][e], ai = acc[1][e];
+  const float br = acc[2][e], bi = acc[3][e];
+  const float y1r = ar + bi, y1i = ai - br, y2r = ar - bi, y2i = ai + br;
+  p1 = fmaf(y1r, y1r, y1i * y1i);
+  p2 = fmaf(y2r, y2r, y2i * y2i);
+}
+
+// sample of the larger output of pair (butterfly q, row kk) with d = p1 - p2: Y_kk for d > 0,
+// Y_{31-kk} for d < 0, the larger of the two samples for d = 0 (equal powers: last maximum)
+__device__ __forceinline__ int c31_pair_sample(int q, int kk, float d) {
+  const int i1 = pfa_wrap(pfa_base(q) + (P / 31) * kk);
+  const int i2 = kk > 0 ? pfa_wrap(pfa_base(q) + (P / 31) * (31 - kk)) : i1;
+  return d > 0.0f ? i1 : d < 0.0f ? i2 : max(i1, i2);
+}
+
 template <int SUB, bool DUMP = false>
 __global__ __launch_bounds__(T) void k_subrows(RowArgs a) {
   constexpr int NBA = P / 33;  // 496
@@ -1284,30 +1476,12 @@ __global__ __launch_bounds__(T) void k_subrows(RowArgs a) {
   for (int i = tid; i < 256; i += T) stw[i] = a.tw[i];
   __syncthreads();
   const int wave = tid >> 6, lane = tid & 63, ln16 = lane & 15, grp = lane >> 4;
-  // this lane's GEMM constants (A row k = ln16) and LDS offsets of its two loads per step
-  float cA[4], sA[4];
-  int off1[4], off2[4];
-  const bool odd = (grp & 1) != 0;
-#pragma unroll
-  for (int st = 0; st < 4; ++st) {
-    const int j = grp == 0 ? 2 * st : grp == 1 ? 15 - 2 * st : grp == 2 ? 2 * st + 1 : 14 - 2 * st;
-    const float2 w = tw_lds(stw, ((j * ln16) % 31) * NBC);  // exp(-2 pi i jk/31)
-    const int m = j == 0 ? 0 : 31 - j;
-    cA[st] = j == 0 ? 1.0f : w.x;
-    sA[st] = j == 0 ? 0.0f : (odd ? w.y : -w.y);
-    off1[st] = 33 * (odd ? m : j);
-    off2[st] = 33 * (odd ? j : m);
-  }
-  const bool z0 = grp == 0;  // step 0 of group 0 is j = 0: a_0 = v_0 alone
+  C31Lane c31;
+  c31.init(stw, ln16, grp);
   auto out_idx = [](int q, int r) { return pfa_wrap(pfa_base(q) + NBC * r); };
-  // sample of pair (tile slot ts, GEMM row e) of this lane with d = p1 - p2: Y_kk for d > 0,
-  // Y_{31-kk} for d < 0, the larger of the two samples for d = 0 (equal powers: last maximum)
+  // sample of pair (tile slot ts, GEMM row e) of this lane with d = p1 - p2
   auto pair_sample = [&](int ts, int e, float d) {
-    const int q = 16 * (wave + 8 * ts) + ln16;
-    const int kk = 4 * grp + e;
-    const int i1 = out_idx(q, kk);
-    const int i2 = kk > 0 ? out_idx(q, 31 - kk) : i1;
-    return d > 0.0f ? i1 : d < 0.0f ? i2 : max(i1, i2);
+    return c31_pair_sample(16 * (wave + 8 * ts) + ln16, 4 * grp + e, d);
   };
   // lanes beyond the 496 pass-A butterflies load past the end of the view: 0, no branch
   const int voa = tid < NBA ? tid * 8 : P * 8;
@@ -1377,29 +1551,7 @@ __global__ __launch_bounds__(T) void k_subrows(RowArgs a) {
     };
     // the 16 MFMAs of tile slot TS (tile wave + 8 TS) into acc
     auto tile_mfma = [&](auto ts_c, floatx4 (&acc)[4]) {
-      constexpr int TS = decltype(ts_c)::value;
-      const int base = c31_pos(16 * (wave + 8 * TS) + ln16);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) acc[c] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int st = 0; st < 4; ++st) {
-        const float2 x1 = buf[base + off1[st]];
-        float2 x2 = buf[base + off2[st]];
-        if (st == 0 && z0) x2 = make_float2(0.f, 0.f);
-        acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(cA[st], x1.x + x2.x, acc[0], 0, 0, 0);
-        acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(cA[st], x1.y + x2.y, acc[1], 0, 0, 0);
-        acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(sA[st], x1.x - x2.x, acc[2], 0, 0, 0);
-        acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(sA[st], x1.y - x2.y, acc[3], 0, 0, 0);
-      }
-    };
-    // powers p1 = |Y_kk|^2 = |A - iB|^2 and p2 = |Y_{31-kk}|^2 = |A + iB|^2 of pair e of a tile's
-    // accumulators (each from its own output, so a small power carries no cancellation error)
-    auto pair_pw = [](const floatx4 (&acc)[4], int e, float& p1, float& p2) {
-      const float ar = acc[0][e], ai = acc[1][e];
-      const float br = acc[2][e], bi = acc[3][e];
-      const float y1r = ar + bi, y1i = ai - br, y2r = ar - bi, y2i = ai + br;
-      p1 = fmaf(y1r, y1r, y1i * y1i);
-      p2 = fmaf(y2r, y2r, y2i * y2i);
+      c31.tile(buf, 16 * (wave + 8 * decltype(ts_c)::value) + ln16, acc);
     };
     // one tile: its MFMAs, then its outputs' pair maxima
     auto tile = [&](auto ts_c) {
@@ -1409,7 +1561,7 @@ __global__ __launch_bounds__(T) void k_subrows(RowArgs a) {
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         float p1, p2;
-        pair_pw(acc, e, p1, p2);
+        c31_pair(acc, e, p1, p2);
         // the pair's maximum (bits: a NaN has the largest); d > 0: Y_kk, d < 0: Y_{31-kk}
         const unsigned int bits = max(__float_as_uint(p1), __float_as_uint(p2));
         const float d = p1 - p2;
@@ -1460,7 +1612,7 @@ __global__ __launch_bounds__(T) void k_subrows(RowArgs a) {
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             float p1, p2;
-            pair_pw(acc, e, p1, p2);
+            c31_pair(acc, e, p1, p2);
             if (max(__float_as_uint(p1), __float_as_uint(p2)) == M)
               cand = max(cand, static_cast<unsigned int>(pair_sample(TS, e, p1 - p2) + 1));
           }
@@ -2554,6 +2706,57 @@ void launch_fused_rows16(const uint32_t* X16, const uint32_t* rep16, const float
   else
     hipLaunchKernelGGL(fused::k_rows16<0>, dim3(static_cast<unsigned>(blocks)),
                        dim3(fused::T16), 0, st, a);
+  TGA_HIP(hipGetLastError());
+}
+
+void launch_carrier_dec(const double* fc, int n_bin, int R0, bool fold, double phase_const,
+                        bool phase64, float2* out, hipStream_t st) {
+  const int nh = fold ? 2 : 1;
+  const long long total = static_cast<long long>(n_bin) * nh * R0 * kFusedP;
+  if (phase64)
+    hipLaunchKernelGGL(fused::k_carrier_dec<true>, dim3(grid_for(total, 256)), dim3(256), 0, st,
+                       fc, phase_const, R0, nh, total, out);
+  else
+    hipLaunchKernelGGL(fused::k_carrier_dec<false>, dim3(grid_for(total, 256)), dim3(256), 0, st,
+                       fc, phase_const, R0, nh, total, out);
+  TGA_HIP(hipGetLastError());
+}
+
+std::vector<float2> long_twiddles(size_t P) {
+  std::vector<float2> t(1024 + 160);
+  for (int j = 0; j < 1024 + 160; ++j) {
+    const long long e = j < 1024 ? j : 1024LL * (j - 1024);
+    const double a = -2.0 * kPiD * static_cast<double>(e) / static_cast<double>(P);
+    t[j] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+  }
+  return t;
+}
+
+void launch_fwd_long(const uint8_t* packed, size_t row_bytes, int fmt, const double* means,
+                     const float2* carrier, const float2* tau, const float2* tw,
+                     const float2* twl, int R0, bool fold, int n_ant, int n_bin, float2* Y,
+                     float2* X, hipStream_t st) {
+  fused::LongFwdArgs a{packed, row_bytes, fmt, means, carrier, tau, tw, n_ant, n_bin, Y};
+  const int nblk = n_bin * R0 * n_ant * (fold ? 2 : 1);
+  const dim3 g1(static_cast<unsigned>(8 * ((nblk + 7) / 8)));
+  if (R0 == 4 && fold)
+    hipLaunchKernelGGL((fused::k_fwd_long<4, true>), g1, dim3(fused::T), 0, st, a);
+  else if (R0 == 4)
+    hipLaunchKernelGGL((fused::k_fwd_long<4, false>), g1, dim3(fused::T), 0, st, a);
+  else if (R0 == 10 && fold)
+    hipLaunchKernelGGL((fused::k_fwd_long<10, true>), g1, dim3(fused::T), 0, st, a);
+  else if (R0 == 10)
+    hipLaunchKernelGGL((fused::k_fwd_long<10, false>), g1, dim3(fused::T), 0, st, a);
+  else
+    fail(TGA_ERR_UNSUPPORTED, "long-period forward supports 4 and 10 segments");
+  TGA_HIP(hipGetLastError());
+  const long long n_pairs = static_cast<long long>(n_ant) * n_bin * (fold ? 2 : 1) * (kFusedP / 2);
+  if (R0 == 4)
+    hipLaunchKernelGGL(fused::k_fwd_twist<4>, dim3(grid_for(n_pairs, 256)), dim3(256), 0, st, Y,
+                       twl, n_pairs, X);
+  else
+    hipLaunchKernelGGL(fused::k_fwd_twist<10>, dim3(grid_for(n_pairs, 256)), dim3(256), 0, st, Y,
+                       twl, n_pairs, X);
   TGA_HIP(hipGetLastError());
 }
 
