@@ -342,9 +342,10 @@ void Plan::build(const tga_plan_desc& d) {
         fail(TGA_ERR_UNSUPPORTED, "a long-period group holds more than 2^31 - 1 rows");
       const size_t nw = static_cast<size_t>(n_ant) * n_prn;
       g.d_win.alloc(nw * sizeof(int));
+      g.d_m2.alloc(nw * sizeof(unsigned int));
       g.d_Zw.alloc(nw * M * sizeof(float2));
-      g.d_Zws.alloc(nw * M * sizeof(float2));
       autotune_split(g);
+      if (g.natural()) g.d_Zws.alloc(nw * M * sizeof(float2));  // the generic transforms' ping-pong
       // the sub-row forms read their spectra segment by segment in the prime-factor order of
       // 16368; the four-step form reads them in natural order
       if (!g.natural()) {
@@ -729,7 +730,13 @@ void Plan::search(const uint8_t* d_in, hipStream_t st, float* dump) {
     launch_select(g.d_rows.as<RowOut>(), g.d_slots.as<PeakSlot>(), n_ant, g.n_prn, g.n_bin,
                   gain, g.scale, d_peaks.as<tga_peak>(), d_nan.as<int>(), st,
                   g.big ? g.d_win.as<int>() : nullptr);
-    if (g.big)
+    if (g.big && !g.natural())
+      launch_big_second_sub(g.X, g.d_rep.as<float2>(), g.d_tw.as<float2>(),
+                            g.d_twb.as<float2>(), g.d_slots.as<PeakSlot>(), g.big_r0, g.cw,
+                            n_ant, g.n_prn, g.n_bin, g.folded, g.d_win.as<int>(), fold, g.scale,
+                            g.d_Zw.as<float2>(), g.d_m2.as<unsigned int>(),
+                            d_peaks.as<tga_peak>(), st);
+    else if (g.big)
       launch_big_second(g.X, g.d_rep.as<float2>(), g.d_slots.as<PeakSlot>(),
                         static_cast<int>(g.M), static_cast<int>(g.P), g.cw, n_ant, g.n_prn,
                         g.n_bin, g.folded, g.d_win.as<int>(), fold, g.scale, g.d_Zw.as<float2>(),

@@ -566,30 +566,17 @@ __global__ void k_carrier_dec(const double* __restrict__ fc, double pc, int R0, 
   }
 }
 
-// Pre-pass of positions [p_begin, p_end) of row rr of a chunk (enumerated row r0 + rr): thread
-// lt takes the two consecutive positions p_begin + 2 lt (+1) with 16-byte loads and stores.
+// Pre-pass of positions t, t + 1 of one long row: conj(X) . R at the R0 positions
+// t + 16368 n0 (16-byte loads), an R0-point DFT, the twiddle W_P^{n1 k0}, and the R0 sub-rows
+// at Yrow + k0 * 16368 + t (non-temporal stores; f16 pairs scaled by kSplitF16Scale with F16).
 template <int R0, bool F16>
-__device__ __forceinline__ void big_pre_positions(const float2* __restrict__ X,
-                                                  const float2* __restrict__ rep,
-                                                  const float2* __restrict__ twb, int n_prn,
-                                                  int n_bin, int n_ant, int hs, int tile,
-                                                  long long r0, long long rr, int p_begin,
-                                                  int p_end, int lt,
-                                                  void* __restrict__ Yv,
-                                                  unsigned long long* __restrict__ rowkey,
-                                                  int* __restrict__ rownan,
-                                                  int* __restrict__ rowmap) {
+__device__ __forceinline__ void big_pre_core(const float2* __restrict__ xrow_p,
+                                             const float2* __restrict__ crow_p,
+                                             const float2* __restrict__ twb, int t,
+                                             void* __restrict__ Yrow) {
   constexpr int P1 = kFusedP;
-  constexpr int P = R0 * P1;
-  const int t = p_begin + 2 * lt;
-  if (t >= p_end) return;
-  const long long I = r0 + rr;  // tiled order (big_item)
-  const BigItem it = big_item(I, n_prn, n_bin, n_ant, hs, tile);
-  const long long xrow = it.xrow;
-  const int prn = it.prn;
-  const int h = static_cast<int>(xrow & 1);
-  const float4* x = reinterpret_cast<const float4*>(X + (hs == 1 ? xrow : xrow >> 1) * P);
-  const float4* c = reinterpret_cast<const float4*>(rep + (static_cast<long long>(prn) * 2 + h) * P);
+  const float4* x = reinterpret_cast<const float4*>(xrow_p);
+  const float4* c = reinterpret_cast<const float4*>(crow_p);
   float2 v[R0], u[R0];
 #pragma unroll
   for (int n0 = 0; n0 < R0; ++n0) {
@@ -618,7 +605,7 @@ __device__ __forceinline__ void big_pre_positions(const float2* __restrict__ X,
   // the tile's spectra from L2 (the float stores merge into one non-temporal dwordx4 per
   // sub-row; a 64-bit integer store through the same builtin would lose the non-temporal bit).
   if constexpr (F16) {
-    uint32_t* y = static_cast<uint32_t*>(Yv) + rr * P;
+    uint32_t* y = static_cast<uint32_t*>(Yrow);
 #pragma unroll
     for (int k0 = 0; k0 < R0; ++k0) {
       uint32_t* d = y + static_cast<long long>(k0) * P1 + t;
@@ -626,7 +613,7 @@ __device__ __forceinline__ void big_pre_positions(const float2* __restrict__ X,
       __builtin_nontemporal_store(pack_f16(u[k0], kSplitF16Scale), d + 1);
     }
   } else {
-    float2* y = static_cast<float2*>(Yv) + rr * P;
+    float2* y = static_cast<float2*>(Yrow);
 #pragma unroll
     for (int k0 = 0; k0 < R0; ++k0) {
       float* d = reinterpret_cast<float*>(y + static_cast<long long>(k0) * P1 + t);
@@ -636,6 +623,33 @@ __device__ __forceinline__ void big_pre_positions(const float2* __restrict__ X,
       __builtin_nontemporal_store(u[k0].y, d + 3);
     }
   }
+}
+
+// Pre-pass of positions [p_begin, p_end) of row rr of a chunk (enumerated row r0 + rr): thread
+// lt takes the two consecutive positions p_begin + 2 lt (+1).
+template <int R0, bool F16>
+__device__ __forceinline__ void big_pre_positions(const float2* __restrict__ X,
+                                                  const float2* __restrict__ rep,
+                                                  const float2* __restrict__ twb, int n_prn,
+                                                  int n_bin, int n_ant, int hs, int tile,
+                                                  long long r0, long long rr, int p_begin,
+                                                  int p_end, int lt,
+                                                  void* __restrict__ Yv,
+                                                  unsigned long long* __restrict__ rowkey,
+                                                  int* __restrict__ rownan,
+                                                  int* __restrict__ rowmap) {
+  constexpr int P = R0 * kFusedP;
+  const int t = p_begin + 2 * lt;
+  if (t >= p_end) return;
+  const long long I = r0 + rr;  // tiled order (big_item)
+  const BigItem it = big_item(I, n_prn, n_bin, n_ant, hs, tile);
+  const long long xrow = it.xrow;
+  const int prn = it.prn;
+  const int h = static_cast<int>(xrow & 1);
+  void* yrow = F16 ? static_cast<void*>(static_cast<uint32_t*>(Yv) + rr * P)
+                   : static_cast<void*>(static_cast<float2*>(Yv) + rr * P);
+  big_pre_core<R0, F16>(X + (hs == 1 ? xrow : xrow >> 1) * P,
+                        rep + (static_cast<long long>(prn) * 2 + h) * P, twb, t, yrow);
   if (t == 0) {
     const long long r = big_row(I, n_prn, n_bin, n_ant, hs, tile);
     rowkey[r] = 0ull;
@@ -669,6 +683,12 @@ struct RowArgs {
   float* dump;
   // sub-row mode: the row index of each of the chunk's rows (written by the pre-pass)
   const int* rowmap;
+  // masked mode of k_subrows (second peaks of the winning rows): sub-rows of winner w =
+  // ant * n_prn + prn, whose peak (slot of prn) names the window centre; the largest power
+  // outside it goes to m2key[w]
+  const tga_peak* peaks;
+  const PeakSlot* slots;
+  unsigned int* m2key;
 };
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -1464,7 +1484,7 @@ __device__ __forceinline__ int c31_pair_sample(int q, int kk, float d) {
   return d > 0.0f ? i1 : d < 0.0f ? i2 : max(i1, i2);
 }
 
-template <int SUB, bool DUMP = false>
+template <int SUB, bool DUMP = false, bool MASK = false>
 __global__ __launch_bounds__(T) void k_subrows(RowArgs a) {
   constexpr int NBA = P / 33;  // 496
   constexpr int NBB = P / 16;  // 1023
@@ -1502,7 +1522,17 @@ __global__ __launch_bounds__(T) void k_subrows(RowArgs a) {
     const long long s = sidx;
     const long long rq = s / SUB;
     const int k0 = static_cast<int>(s - rq * SUB);
-    const long long r = a.rowmap[rq];  // big_row(a.r0 + rq, ...), from the pre-pass
+    // the row index (big_row(a.r0 + rq, ...), from the pre-pass), or in the masked mode the
+    // winner and the centre of its exclusion window
+    const long long r = MASK ? rq : a.rowmap[rq];
+    int centre = 0;
+    if constexpr (MASK) {
+      const int ant = static_cast<int>(rq / a.n_prn);
+      const PeakSlot sl = a.slots[rq - static_cast<long long>(ant) * a.n_prn];
+      centre = static_cast<int>(
+          a.peaks[sl.base + static_cast<long long>(ant) * sl.stride + sl.local].best_codephase %
+          static_cast<uint64_t>(SUB * P));
+    }
     // ---- pass A (radix 33) -> LDS; the next sub-row's inputs in flight
     Dft<33>::run(cur);
     {
@@ -1563,6 +1593,18 @@ __global__ __launch_bounds__(T) void k_subrows(RowArgs a) {
         float p1, p2;
         c31_pair(acc, e, p1, p2);
         // the pair's maximum (bits: a NaN has the largest); d > 0: Y_kk, d < 0: Y_{31-kk}
+        if constexpr (MASK) {
+          // the largest power outside the window of `centre` (ring distance > cw)
+          const int q = 16 * (wave + 8 * TS) + ln16;
+          const int kk = 4 * grp + e;
+          const int sa = k0 + SUB * out_idx(q, kk);
+          const int sb = k0 + SUB * out_idx(q, 31 - kk);
+          const unsigned int ua = ring_dist(sa, centre, SUB * P) > a.cw ? __float_as_uint(p1) : 0u;
+          const unsigned int ub =
+              kk > 0 && ring_dist(sb, centre, SUB * P) > a.cw ? __float_as_uint(p2) : 0u;
+          pmx = max(pmx, max(ua, ub));
+          continue;
+        }
         const unsigned int bits = max(__float_as_uint(p1), __float_as_uint(p2));
         const float d = p1 - p2;
         const bool up = bits > pmx;
@@ -1597,6 +1639,12 @@ __global__ __launch_bounds__(T) void k_subrows(RowArgs a) {
     const bool mine = lane < NW;
     const unsigned int M = static_cast<unsigned int>(__builtin_amdgcn_readlane(
         static_cast<int>(row16_umax(mine ? red.pk[lane] : 0u)), 0));
+    if constexpr (MASK) {
+      if (tid == 0) atomicMax(a.m2key + r, M);
+      sidx += G;
+      if (sidx >= a.n_sub) break;
+      continue;
+    }
     const bool tie = __ballot(mine && red.pk[mine ? lane : 0] == M &&
                               red.bad[mine ? lane : 0] != 0) != 0ull;
     unsigned int cand = 0u;  // 1 + sample, 0 for none
@@ -2455,6 +2503,46 @@ __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
                              static_cast<int>(threadIdx.x), Yv, rowkey, rownan, rowmap);
 }
 
+// Pre-pass of the winning row of every (antenna, PRN) pair w (win[w] = 2 bin + h): its R0
+// sub-rows into Zw[w], 32 blocks of 512 positions per row; clears the masked-maximum key.
+template <int R0>
+__global__ __launch_bounds__(256) void k_win_pre(const float2* __restrict__ X,
+                                                 const float2* __restrict__ rep,
+                                                 const float2* __restrict__ twb, int n_prn,
+                                                 int n_bin, int folded,
+                                                 const int* __restrict__ win, int nw,
+                                                 float2* __restrict__ Zw,
+                                                 unsigned int* __restrict__ m2key) {
+  constexpr int PL = R0 * kFusedP;
+  const int w = static_cast<int>(blockIdx.x >> 5);
+  if (w >= nw) return;
+  const int t = 512 * static_cast<int>(blockIdx.x & 31) + 2 * static_cast<int>(threadIdx.x);
+  if (t >= kFusedP) return;
+  const int ant = w / n_prn;
+  const int prn = w - ant * n_prn;
+  const int bh = win[w];
+  const long long xrow = static_cast<long long>(ant) * n_bin * 2 + bh;
+  fused::big_pre_core<R0, false>(X + (folded ? xrow : xrow >> 1) * PL,
+                                 rep + (static_cast<long long>(prn) * 2 + (bh & 1)) * PL, twb, t,
+                                 Zw + static_cast<long long>(w) * PL);
+  if (t == 0) m2key[w] = 0u;
+}
+
+// Second peak of every (antenna, PRN) pair from its masked maximum (k_second_win's rounding).
+__global__ void k_win_second(const unsigned int* __restrict__ m2key, int nw, int n_prn,
+                             const PeakSlot* __restrict__ slots, float gain, float scale,
+                             tga_peak* __restrict__ out) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nw) return;
+  const int ant = w / n_prn;
+  const int prn = w - ant * n_prn;
+  const PeakSlot sl = slots[prn];
+  const float m2 = __uint_as_float(m2key[w]);
+  float second = (__fsqrt_rn(m2) * gain) * scale;
+  if (!(second > 0.0f)) second = 1e-6f;  // correlate.rs:172-174
+  out[sl.base + static_cast<long long>(ant) * sl.stride + sl.local].second_peak = second;
+}
+
 // Row keys of long rows -> RowOut (p2 is filled later from the winning row only).
 __global__ void k_big_rowout(const unsigned long long* __restrict__ rowkey,
                              const int* __restrict__ rownan, long long r0, long long n_rows,
@@ -3058,6 +3146,42 @@ void launch_surface_finish(float* v, long long n, float gain, float scale, hipSt
   if (n <= 0) return;
   hipLaunchKernelGGL(k_surface_finish, dim3(grid_for(n, 256)), dim3(256), 0, st, v, n, gain,
                      scale);
+  TGA_HIP(hipGetLastError());
+}
+
+void launch_big_second_sub(const float2* X, const float2* rep, const float2* tw,
+                           const float2* twb, const PeakSlot* slots, int R0, int cw, int n_ant,
+                           int n_prn, int n_bin, bool folded, const int* win, float gain,
+                           float scale, float2* Zw, unsigned int* m2key, tga_peak* out,
+                           hipStream_t st) {
+  const int nw = n_ant * n_prn;
+  const dim3 g1(static_cast<unsigned>(nw) * 32);
+  if (R0 == 4)
+    hipLaunchKernelGGL(k_win_pre<4>, g1, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
+                       folded ? 1 : 0, win, nw, Zw, m2key);
+  else
+    hipLaunchKernelGGL(k_win_pre<10>, g1, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
+                       folded ? 1 : 0, win, nw, Zw, m2key);
+  TGA_HIP(hipGetLastError());
+  fused::RowArgs a{};
+  a.tw = tw;
+  a.cw = cw;
+  a.n_prn = n_prn;
+  a.n_bin = n_bin;
+  a.Y = Zw;
+  a.n_sub = static_cast<long long>(nw) * R0;
+  a.peaks = out;
+  a.slots = slots;
+  a.m2key = m2key;
+  const unsigned blocks = static_cast<unsigned>(a.n_sub < 2048 ? a.n_sub : 2048);
+  if (R0 == 4)
+    hipLaunchKernelGGL((fused::k_subrows<4, false, true>), dim3(blocks), dim3(fused::T), 0, st, a);
+  else
+    hipLaunchKernelGGL((fused::k_subrows<10, false, true>), dim3(blocks), dim3(fused::T), 0, st,
+                       a);
+  TGA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_win_second, dim3(grid_for(nw, 256)), dim3(256), 0, st, m2key, nw, n_prn,
+                     slots, gain, scale, out);
   TGA_HIP(hipGetLastError());
 }
 
