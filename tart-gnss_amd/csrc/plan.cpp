@@ -362,6 +362,7 @@ void Plan::build(const tga_plan_desc& d) {
           upload(g.d_tau, td, stream);
         }
         upload(g.d_twl, long_twiddles(g.P), stream);
+        g.d_lev.alloc(static_cast<size_t>(n_ant) * (g.folded ? 2 : 1) * g.P * sizeof(float));
         TGA_HIP(hipStreamSynchronize(stream));
       }
     } else {
@@ -650,7 +651,8 @@ void Plan::search(const uint8_t* d_in, hipStream_t st, float* dump) {
     } else if (g.big && !g.natural()) {
       launch_fwd_long(d_in, row_bytes, fmt, d_means.as<double>(), g.d_carrier.as<float2>(),
                       g.d_tau.as<float2>(), g.d_tw.as<float2>(), g.d_twl.as<float2>(), g.big_r0,
-                      g.folded, n_ant, g.n_bin, g.d_Xs.as<float2>(), g.d_X.as<float2>(), st);
+                      g.folded, n_ant, g.n_bin, g.d_lev.as<float>(), g.d_Xs.as<float2>(),
+                      g.d_X.as<float2>(), st);
       g.X = g.d_X.as<float2>();
     } else {
       if (g.folded)

@@ -37,6 +37,7 @@ struct Group {
   DevBuf d_tw4, d_c31;  // four-step split: W_P^{n2 k1} as [k1][n2], radix-31 cos / sin
   DevBuf d_twl;  // sub-row split: W_P^e two-level table of the long-period forward
   DevBuf d_m2;   // sub-row split: masked maximum (f32 bits) of each winning row
+  DevBuf d_lev;  // sub-row split: decoded levels of the long-period forward
   float2* X = nullptr;  // where the forward spectra of the last search landed
   FusedRowsConfig rows_cfg;  // rows-kernel shape picked by the autotuner
   std::vector<std::pair<FusedRowsConfig, double>> tune;  // candidates timed, ms per launch

@@ -104,10 +104,11 @@ void launch_fill_tune(void* p, size_t bytes, bool f16, hipStream_t st);
 void launch_carrier_dec(const double* fc, int n_bin, int R0, bool fold, double phase_const,
                         bool phase64, float2* out, hipStream_t st);
 std::vector<float2> long_twiddles(size_t P);
+// lev: scratch for the decoded levels, n_ant * (fold ? 2 : 1) * P floats.
 void launch_fwd_long(const uint8_t* packed, size_t row_bytes, int fmt, const double* means,
                      const float2* carrier, const float2* tau, const float2* tw,
-                     const float2* twl, int R0, bool fold, int n_ant, int n_bin, float2* Y,
-                     float2* X, hipStream_t st);
+                     const float2* twl, int R0, bool fold, int n_ant, int n_bin, float* lev,
+                     float2* Y, float2* X, hipStream_t st);
 // n values = n / 16368 natural-order rows -> f16 pairs times scale in the prime-factor order
 void launch_to_f16(const float2* in, uint32_t* out, long long n, float scale, hipStream_t st);
 // Twiddle table of the fused kernels: W_P^i (i < 128) then W_P^{128 i} (i < 128).

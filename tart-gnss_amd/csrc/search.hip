@@ -416,6 +416,8 @@ __global__ __launch_bounds__(T) void k_forward(FwdArgs a) {
     const float2 c = car[n];
     return make_float2(c.x * l, c.y * l);
   };
+  // eight samples per thread at a time, so that their carrier loads are in flight together
+#pragma unroll 8
   for (int n = tid; n < P; n += T) {
     if constexpr (NC) {
       buf[n] = wipe(n);
@@ -437,134 +439,6 @@ __global__ __launch_bounds__(T) void k_forward(FwdArgs a) {
   }
 }
 
-// ---- Forward spectra of long code periods (P = R0 x 16368), four-step through LDS ------------
-// With n = R0 m + r (m < 16368, r < R0) and k = t + 16368 k0 (t < 16368, k0 < R0),
-//   X[t + 16368 k0] = sum_r W_R0^{r k0} (W_P^{r t} Y_r[t]),   Y_r = FFT_16368(x[R0 m + r]),
-// so step 1 (k_fwd_long) runs one LDS transform per (antenna, bin, fold, r) of the decimated,
-// decoded, wiped and folded samples, and step 2 (k_fwd_twist) the twiddles and the R0-point
-// DFTs, writing X segment by segment in the prime-factor order the split's pre-pass reads
-// (position 16368 k0 + s holds natural index 16368 k0 + pfa_nat(s)).  The carrier comes from
-// a plan-time table in the decimated order, [bin][half][r][m] for sample n = half P + R0 m + r
-// (same rounding as k_carrier), and the twist tau_n = exp(-i pi n / P) from [r][m].
-struct LongFwdArgs {
-  const uint8_t* packed;
-  size_t row_bytes;
-  int fmt;
-  const double* means;
-  const float2* carrier;  // [n_bin][nh][R0][16368]
-  const float2* tau;      // [R0][16368] (folded rows)
-  const float2* tw;       // fused twiddles of the LDS passes
-  int n_ant;
-  int n_bin;
-  float2* Y;  // [xrow][R0][16368] in the prime-factor order of 16368
-};
-
-template <int R0, bool FOLD>
-__global__ __launch_bounds__(T) void k_fwd_long(LongFwdArgs a) {
-  constexpr int NH = FOLD ? 2 : 1;
-  __shared__ float2 buf[P];
-  __shared__ float2 stw[256];
-  const int tid = threadIdx.x;
-  for (int i = tid; i < 256; i += T) stw[i] = a.tw[i];
-  // blocks (bin, r, ant, h), ant and h fastest so that the blocks reading one slice of the
-  // carrier table run together; each XCD takes one contiguous share of them
-  const int nblk = a.n_bin * R0 * a.n_ant * NH;
-  const int per_xcd = (nblk + 7) / 8;
-  const int lb = static_cast<int>(blockIdx.x & 7) * per_xcd + static_cast<int>(blockIdx.x >> 3);
-  if (lb >= nblk) return;
-  const int h = FOLD ? lb % NH : 0;
-  const int ant = (lb / NH) % a.n_ant;
-  const int br = lb / (NH * a.n_ant);
-  const int r = br % R0;
-  const int bin = br / R0;
-  const uint8_t* row = a.packed + static_cast<size_t>(ant) * a.row_bytes;
-  const double mean = a.means[ant];
-  const float2* c0 = a.carrier + (static_cast<size_t>(bin) * NH * R0 + r) * P;
-  const float2* tau = a.tau + static_cast<size_t>(r) * P;
-  auto level = [&](long long n) {
-    return __double2float_rn(static_cast<double>(decode_level(row, a.fmt, n)) - mean);
-  };
-  for (int m = tid; m < P; m += T) {
-    const long long n = static_cast<long long>(R0) * m + r;
-    const float l0 = level(n);
-    const float2 w0 = c0[m];
-    const float2 x0 = make_float2(w0.x * l0, w0.y * l0);
-    if constexpr (FOLD) {
-      const float l1 = level(n + static_cast<long long>(R0) * P);
-      const float2 w1 = c0[static_cast<size_t>(R0) * P + m];
-      const float2 x1 = make_float2(w1.x * l1, w1.y * l1);
-      buf[m] = h == 0 ? cadd(x0, x1) : cmul(csub(x0, x1), tau[m]);
-    } else {
-      buf[m] = x0;
-    }
-  }
-  __syncthreads();
-  lds_fft_natural(buf, stw, tid);
-  __syncthreads();
-  const size_t xrow = (static_cast<size_t>(ant) * a.n_bin + bin) * NH + h;
-  float2* out = a.Y + (xrow * R0 + r) * P;
-  for (int s = tid; s < P; s += T) out[s] = buf[pfa_nat(s)];
-}
-
-// Step 2: per position s of a row (natural index t = pfa_nat(s)), the R0 values Y_r[s] times
-// W_P^{r t} (from the two-level table twl: W_P^e = twl[e & 1023] twl[1024 + (e >> 10)]), an
-// R0-point DFT, and X[xrow][k0][s]; two positions per thread with 16-byte accesses.
-template <int R0>
-__global__ __launch_bounds__(256) void k_fwd_twist(const float2* __restrict__ Y,
-                                                   const float2* __restrict__ twl,
-                                                   long long n_pairs, float2* __restrict__ X) {
-  constexpr int PL = R0 * P;
-  const long long gid = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-  if (gid >= n_pairs) return;
-  const long long xrow = gid / (P / 2);
-  const int s = 2 * static_cast<int>(gid - xrow * (P / 2));
-  const float4* y = reinterpret_cast<const float4*>(Y + xrow * PL + s);
-  const int t0 = pfa_nat(s), t1 = pfa_nat(s + 1);
-  float2 v[R0], u[R0];
-  int e0 = 0, e1 = 0;  // r t mod PL
-#pragma unroll
-  for (int r = 0; r < R0; ++r) {
-    const float4 q = y[static_cast<long long>(r) * (P / 2)];
-    v[r] = make_float2(q.x, q.y);
-    u[r] = make_float2(q.z, q.w);
-    if (r > 0) {
-      v[r] = cmul(v[r], cmul(twl[e0 & 1023], twl[1024 + (e0 >> 10)]));
-      u[r] = cmul(u[r], cmul(twl[e1 & 1023], twl[1024 + (e1 >> 10)]));
-    }
-    e0 += t0;
-    e0 = e0 >= PL ? e0 - PL : e0;
-    e1 += t1;
-    e1 = e1 >= PL ? e1 - PL : e1;
-  }
-  Dft<R0>::run(v);
-  Dft<R0>::run(u);
-  float4* x = reinterpret_cast<float4*>(X + xrow * PL + s);
-#pragma unroll
-  for (int k0 = 0; k0 < R0; ++k0)
-    x[static_cast<long long>(k0) * (P / 2)] = make_float4(v[k0].x, v[k0].y, u[k0].x, u[k0].y);
-}
-
-// Carrier (cos, sin) of sample n = half P + R0 m + r in the decimated order [bin][half][r][m]
-// of k_fwd_long, rounded like k_carrier (f32 phase, or the f64 phase of the FP16 form).
-template <bool PHASE64>
-__global__ void k_carrier_dec(const double* __restrict__ fc, double pc, int R0, int nh,
-                              long long total, float2* __restrict__ out) {
-  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= total) return;
-  const long long per_bin = static_cast<long long>(nh) * R0 * P;
-  const long long bin = i / per_bin;
-  const long long j = i - bin * per_bin;
-  const long long half = j / (static_cast<long long>(R0) * P);
-  const long long k = j - half * R0 * P;
-  const long long r = k / P, m = k - r * P;
-  const long long n = half * R0 * P + R0 * m + r;
-  const double f = fc[bin];
-  if constexpr (PHASE64) {
-    out[i] = cossin_f64(__dmul_rn(__dmul_rn(pc, static_cast<double>(n)), f));
-  } else {
-    out[i] = wipe_value(1.0f, __dmul_rn(pc, static_cast<double>(n)), f);
-  }
-}
 
 // Pre-pass of positions t, t + 1 of one long row: conj(X) . R at the R0 positions
 // t + 16368 n0 (16-byte loads), an R0-point DFT, the twiddle W_P^{n1 k0}, and the R0 sub-rows
@@ -1686,6 +1560,248 @@ __global__ __launch_bounds__(T) void k_subrows(RowArgs a) {
     }
     sidx += G;
     if (sidx >= a.n_sub) break;
+  }
+}
+
+// ---- Forward spectra of long code periods (P = R0 x 16368), four-step through LDS ------------
+// With n = R0 m + r (m < 16368, r < R0) and k = t + 16368 k0 (t < 16368, k0 < R0),
+//   X[t + 16368 k0] = sum_r W_R0^{r k0} (W_P^{r t} Y_r[t]),   Y_r = FFT_16368(x[R0 m + r]),
+// so step 1 (k_fwd_long) runs one LDS transform per (antenna, bin, fold, r) of the decimated,
+// decoded, wiped and folded samples, and step 2 (k_fwd_twist) the twiddles and the R0-point
+// DFTs, writing X segment by segment in the prime-factor order the split's pre-pass reads
+// (position 16368 k0 + s holds natural index 16368 k0 + pfa_nat(s)).  The carrier comes from
+// a plan-time table in the decimated order, [bin][half][r][m] for sample n = half P + R0 m + r
+// (same rounding as k_carrier), and the twist tau_n = exp(-i pi n / P) from [r][m].
+struct LongFwdArgs {
+  const float* lev;       // [n_ant][nh][R0][16368] f32(level - mean) (k_levels_dec)
+  const float2* carrier;  // [n_bin][nh][R0][16368]
+  const float2* tau;      // [R0][16368] (folded rows)
+  const float2* tw;       // fused twiddles of the LDS passes
+  int n_ant;
+  int n_bin;
+  float2* Y;  // [xrow][R0][16368] in the prime-factor order of 16368
+};
+
+template <int R0, bool FOLD>
+__global__ __launch_bounds__(T) void k_fwd_long(LongFwdArgs a) {
+  constexpr int NH = FOLD ? 2 : 1;
+  constexpr int NBA = P / 33;  // 496
+  constexpr int NBB = P / 16;  // 1023
+  __shared__ float2 buf[P];
+  __shared__ float2 stw[256];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < 256; i += T) stw[i] = a.tw[i];
+  // blocks (bin, r, ant, h), ant and h fastest so that the blocks reading one slice of the
+  // carrier table run together; each XCD takes one contiguous share of them
+  const int nblk = a.n_bin * R0 * a.n_ant * NH;
+  const int per_xcd = (nblk + 7) / 8;
+  const int lb = static_cast<int>(blockIdx.x & 7) * per_xcd + static_cast<int>(blockIdx.x >> 3);
+  if (lb >= nblk) return;
+  const int h = FOLD ? lb % NH : 0;
+  const int ant = (lb / NH) % a.n_ant;
+  const int br = lb / (NH * a.n_ant);
+  const int r = br % R0;
+  const int bin = br / R0;
+  const float* l0p = a.lev + (static_cast<size_t>(ant) * NH * R0 + r) * P;
+  const float2* c0 = a.carrier + (static_cast<size_t>(bin) * NH * R0 + r) * P;
+  const float2* tau = a.tau + static_cast<size_t>(r) * P;
+  // ---- wipe, fold / twist: the decimated samples in natural order, eight per thread at a
+  // time so that their loads are in flight together
+  for (int m0 = tid; m0 < P; m0 += 8 * T) {
+    float2 w0[8], w1[8], tw[8];
+    float v0[8], v1[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int m = m0 + u * T;
+      const int mm = m < P ? m : 0;
+      v0[u] = l0p[mm];
+      w0[u] = c0[mm];
+      if constexpr (FOLD) {
+        v1[u] = l0p[static_cast<size_t>(R0) * P + mm];
+        w1[u] = c0[static_cast<size_t>(R0) * P + mm];
+        tw[u] = tau[mm];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int m = m0 + u * T;
+      if (m < P) {
+        const float l0 = v0[u];
+        const float2 x0 = make_float2(w0[u].x * l0, w0[u].y * l0);
+        if constexpr (FOLD) {
+          const float l1 = v1[u];
+          const float2 x1 = make_float2(w1[u].x * l1, w1[u].y * l1);
+          buf[m] = h == 0 ? cadd(x0, x1) : cmul(csub(x0, x1), tw[u]);
+        } else {
+          buf[m] = x0;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // ---- the 16368-point DFT in the prime-factor order of the rows kernels (33 x 16 x 31, no
+  // twiddles).  Pass A: butterfly ta = 31 n2 + n3 gathers its inputs at natural indices
+  // (496 n1 + 1023 n2 + 528 n3) mod P, all before any thread stores its outputs at
+  // 1023 n2 + 33 n3 + k1 (the layout of k_subrows after pass A).
+  C31Lane c31;
+  c31.init(stw, tid & 15, (tid & 63) >> 4);
+  {
+    int ta = tid;
+    asm volatile("" : "+v"(ta));
+    float2 v[33];
+    if (ta < NBA) {
+      const int n2 = ta / 31, n3 = ta - 31 * n2;
+      int m = (1023 * n2 + 528 * n3) % P;
+#pragma unroll
+      for (int n1 = 0; n1 < 33; ++n1) {
+        v[n1] = buf[m];
+        m += 496;
+        m = m >= P ? m - P : m;
+      }
+    }
+    __syncthreads();
+    if (ta < NBA) {
+      Dft<33>::run(v);
+#pragma unroll
+      for (int q = 0; q < 33; ++q) buf[ta * 33 + q] = v[q];
+    }
+  }
+  __syncthreads();
+  // pass B (radix 16) in place
+  {
+    int tb = tid;
+    asm volatile("" : "+v"(tb));
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int j = tb + b * T;
+      if (b == 0 || j < NBB) {
+        float2 v[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = buf[j + q * NBB];
+        Dft<16>::run(v);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) buf[j + q * NBB] = v[q];
+      }
+      asm volatile("" ::: "memory");
+    }
+  }
+  __syncthreads();
+  // pass C (radix 31 on the matrix cores): the lane's outputs Y_kk = A - iB and
+  // Y_{31-kk} = A + iB of its tiles (tile wave + 8 ts), kept until every pass-C load is done
+  const int wave = tid >> 6, lane = tid & 63, ln16 = lane & 15, grp = lane >> 4;
+  const int ntile = wave == 0 ? 5 : 4;
+  float2 yo[5][4][2];
+#pragma unroll
+  for (int ts = 0; ts < 5; ++ts) {
+    if (ts < ntile) {
+      floatx4 acc[4];
+      c31.tile(buf, 16 * (wave + 8 * ts) + ln16, acc);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float ar = acc[0][e], ai = acc[1][e], br_ = acc[2][e], bi = acc[3][e];
+        yo[ts][e][0] = make_float2(ar + bi, ai - br_);
+        yo[ts][e][1] = make_float2(ar - bi, ai + br_);
+      }
+    }
+  }
+  __syncthreads();
+  // output k (k mod 33 = q mod 33, k mod 16 = q / 33, k mod 31 = kk) goes to its prime-factor
+  // position pfa_pos(k) = 496 (k mod 33) + 31 ((-k) mod 16) + (k mod 31)
+#pragma unroll
+  for (int ts = 0; ts < 5; ++ts) {
+    if (ts < ntile) {
+      const int q = 16 * (wave + 8 * ts) + ln16;
+      const int k2 = q / 33, k1 = q - 33 * k2;
+      const int base = 496 * k1 + 31 * ((16 - k2) & 15);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int kk = 4 * grp + e;
+        buf[base + kk] = yo[ts][e][0];
+        if (kk > 0) buf[base + 31 - kk] = yo[ts][e][1];
+      }
+    }
+  }
+  __syncthreads();
+  const size_t xrow = (static_cast<size_t>(ant) * a.n_bin + bin) * NH + h;
+  float2* out = a.Y + (xrow * R0 + r) * P;
+  for (int s2 = tid; s2 < P; s2 += T) out[s2] = buf[s2];
+}
+
+// Step 2: per position s of a row (natural index t = pfa_nat(s)), the R0 values Y_r[s] times
+// W_P^{r t} (from the two-level table twl: W_P^e = twl[e & 1023] twl[1024 + (e >> 10)]), an
+// R0-point DFT, and X[xrow][k0][s]; two positions per thread with 16-byte accesses.
+template <int R0>
+__global__ __launch_bounds__(256) void k_fwd_twist(const float2* __restrict__ Y,
+                                                   const float2* __restrict__ twl,
+                                                   long long n_pairs, float2* __restrict__ X) {
+  constexpr int PL = R0 * P;
+  const long long gid = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (gid >= n_pairs) return;
+  const long long xrow = gid / (P / 2);
+  const int s = 2 * static_cast<int>(gid - xrow * (P / 2));
+  const float4* y = reinterpret_cast<const float4*>(Y + xrow * PL + s);
+  const int t0 = pfa_nat(s), t1 = pfa_nat(s + 1);
+  float2 v[R0], u[R0];
+  int e0 = 0, e1 = 0;  // r t mod PL
+#pragma unroll
+  for (int r = 0; r < R0; ++r) {
+    const float4 q = y[static_cast<long long>(r) * (P / 2)];
+    v[r] = make_float2(q.x, q.y);
+    u[r] = make_float2(q.z, q.w);
+    if (r > 0) {
+      v[r] = cmul(v[r], cmul(twl[e0 & 1023], twl[1024 + (e0 >> 10)]));
+      u[r] = cmul(u[r], cmul(twl[e1 & 1023], twl[1024 + (e1 >> 10)]));
+    }
+    e0 += t0;
+    e0 = e0 >= PL ? e0 - PL : e0;
+    e1 += t1;
+    e1 = e1 >= PL ? e1 - PL : e1;
+  }
+  Dft<R0>::run(v);
+  Dft<R0>::run(u);
+  float4* x = reinterpret_cast<float4*>(X + xrow * PL + s);
+#pragma unroll
+  for (int k0 = 0; k0 < R0; ++k0)
+    x[static_cast<long long>(k0) * (P / 2)] = make_float4(v[k0].x, v[k0].y, u[k0].x, u[k0].y);
+}
+
+// f32(level - mean) of sample n = half P + R0 m + r of every antenna in the decimated order
+// [ant][half][r][m] of k_fwd_long, decoded once per search (every bin's blocks read it).
+__global__ void k_levels_dec(const uint8_t* __restrict__ packed, size_t row_bytes, int fmt,
+                             const double* __restrict__ means, int R0, int nh, long long total,
+                             float* __restrict__ out) {
+  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= total) return;
+  const long long per_ant = static_cast<long long>(nh) * R0 * P;
+  const long long ant = i / per_ant;
+  const long long j = i - ant * per_ant;
+  const long long half = j / (static_cast<long long>(R0) * P);
+  const long long k = j - half * R0 * P;
+  const long long r = k / P, m = k - r * P;
+  const long long n = half * R0 * P + R0 * m + r;
+  out[i] = __double2float_rn(
+      static_cast<double>(decode_level(packed + ant * row_bytes, fmt, n)) - means[ant]);
+}
+
+// Carrier (cos, sin) of sample n = half P + R0 m + r in the decimated order [bin][half][r][m]
+// of k_fwd_long, rounded like k_carrier (f32 phase, or the f64 phase of the FP16 form).
+template <bool PHASE64>
+__global__ void k_carrier_dec(const double* __restrict__ fc, double pc, int R0, int nh,
+                              long long total, float2* __restrict__ out) {
+  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= total) return;
+  const long long per_bin = static_cast<long long>(nh) * R0 * P;
+  const long long bin = i / per_bin;
+  const long long j = i - bin * per_bin;
+  const long long half = j / (static_cast<long long>(R0) * P);
+  const long long k = j - half * R0 * P;
+  const long long r = k / P, m = k - r * P;
+  const long long n = half * R0 * P + R0 * m + r;
+  const double f = fc[bin];
+  if constexpr (PHASE64) {
+    out[i] = cossin_f64(__dmul_rn(__dmul_rn(pc, static_cast<double>(n)), f));
+  } else {
+    out[i] = wipe_value(1.0f, __dmul_rn(pc, static_cast<double>(n)), f);
   }
 }
 
@@ -2822,9 +2938,13 @@ std::vector<float2> long_twiddles(size_t P) {
 
 void launch_fwd_long(const uint8_t* packed, size_t row_bytes, int fmt, const double* means,
                      const float2* carrier, const float2* tau, const float2* tw,
-                     const float2* twl, int R0, bool fold, int n_ant, int n_bin, float2* Y,
-                     float2* X, hipStream_t st) {
-  fused::LongFwdArgs a{packed, row_bytes, fmt, means, carrier, tau, tw, n_ant, n_bin, Y};
+                     const float2* twl, int R0, bool fold, int n_ant, int n_bin, float* lev,
+                     float2* Y, float2* X, hipStream_t st) {
+  const long long nlev = static_cast<long long>(n_ant) * (fold ? 2 : 1) * R0 * kFusedP;
+  hipLaunchKernelGGL(fused::k_levels_dec, dim3(grid_for(nlev, 256)), dim3(256), 0, st, packed,
+                     row_bytes, fmt, means, R0, fold ? 2 : 1, nlev, lev);
+  TGA_HIP(hipGetLastError());
+  fused::LongFwdArgs a{lev, carrier, tau, tw, n_ant, n_bin, Y};
   const int nblk = n_bin * R0 * n_ant * (fold ? 2 : 1);
   const dim3 g1(static_cast<unsigned>(8 * ((nblk + 7) / 8)));
   if (R0 == 4 && fold)
