@@ -409,8 +409,46 @@ void Plan::build(const tga_plan_desc& d) {
   TGA_HIP(hipStreamSynchronize(stream));
 }
 
-// Rows-kernel autotuner: times every candidate shape once (after one warm-up launch) on the
-// group's real sizes, with pseudo-random spectra, and keeps the fastest.  Candidates: the radix-31
+// Times every candidate once after a warm-up launch, then the candidates within 3 % of the
+// best twice more (one timing varies by about 1 %), and keeps the fastest: g.tune holds each
+// candidate's best time, g.rows_cfg the winner.
+template <typename Launch>
+void tune_pick(Group& g, const std::vector<FusedRowsConfig>& cands, hipStream_t stream,
+               Launch&& launch) {
+  hipEvent_t e0, e1;
+  TGA_HIP(hipEventCreate(&e0));
+  TGA_HIP(hipEventCreate(&e1));
+  auto time_one = [&](const FusedRowsConfig& c) {
+    TGA_HIP(hipEventRecord(e0, stream));
+    launch(c);
+    TGA_HIP(hipEventRecord(e1, stream));
+    TGA_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    TGA_HIP(hipEventElapsedTime(&ms, e0, e1));
+    return static_cast<double>(ms);
+  };
+  double best = 1e30;
+  for (const FusedRowsConfig& c : cands) {
+    launch(c);
+    const double ms = time_one(c);
+    g.tune.push_back({c, ms});
+    best = std::min(best, ms);
+  }
+  for (auto& t : g.tune)
+    if (t.second <= 1.03 * best)
+      for (int k = 0; k < 2; ++k) t.second = std::min(t.second, time_one(t.first));
+  best = 1e30;
+  for (const auto& t : g.tune)
+    if (t.second < best) {
+      best = t.second;
+      g.rows_cfg = t.first;
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+}
+
+// Rows-kernel autotuner: times every candidate shape on the group's real sizes, with
+// pseudo-random spectra, and keeps the fastest (tune_pick).  Candidates: the radix-31
 // implementation of the f32 kernel (VALU codelet or matrix cores) and the number of XCD-local
 // PRN groups (8, 16 or 32: more groups give more, shorter workgroups, each over fewer PRNs).
 // TGA_FUSED_VARIANT=0|1 pins the radix-31 implementation (tests use it to cover both) and
@@ -461,26 +499,7 @@ void Plan::autotune(Group& g) {
       launch_fused_rows(g.d_X.as<float2>(), g.d_rep.as<float2>(), g.d_tw.as<float2>(), n_ant,
                         g.n_bin, g.n_prn, g.cw, g.d_rows.as<RowOut>(), stream, c);
   };
-  hipEvent_t e0, e1;
-  TGA_HIP(hipEventCreate(&e0));
-  TGA_HIP(hipEventCreate(&e1));
-  float best = 1e30f;
-  for (const FusedRowsConfig& c : cands) {
-    launch(c);
-    TGA_HIP(hipEventRecord(e0, stream));
-    launch(c);
-    TGA_HIP(hipEventRecord(e1, stream));
-    TGA_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    TGA_HIP(hipEventElapsedTime(&ms, e0, e1));
-    g.tune.push_back({c, static_cast<double>(ms)});
-    if (ms < best) {
-      best = ms;
-      g.rows_cfg = c;
-    }
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  tune_pick(g, cands, stream, launch);
 }
 
 // Split-path autotuner: times the first chunk of rows (both kernels of the form, after one
@@ -548,26 +567,7 @@ void Plan::autotune_split(Group& g) {
                    g.d_Y.p, g.big16, g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(),
                    g.d_rowmap.as<int>(), g.d_rows.as<RowOut>(), stream, c.variant);
   };
-  hipEvent_t e0, e1;
-  TGA_HIP(hipEventCreate(&e0));
-  TGA_HIP(hipEventCreate(&e1));
-  float best = 1e30f;
-  for (const FusedRowsConfig& c : cands) {
-    launch(c);
-    TGA_HIP(hipEventRecord(e0, stream));
-    launch(c);
-    TGA_HIP(hipEventRecord(e1, stream));
-    TGA_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    TGA_HIP(hipEventElapsedTime(&ms, e0, e1));
-    g.tune.push_back({c, static_cast<double>(ms)});
-    if (ms < best) {
-      best = ms;
-      g.rows_cfg = c;
-    }
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  tune_pick(g, cands, stream, launch);
 }
 
 // While a graph is being captured the records become event-record nodes of the graph, and

@@ -1839,6 +1839,7 @@ struct Row16Args {
   int tile = 0;   // sub-row mode: tile shape of the row order (big_item)
   long long n_items = 0;  // SUB == 0: (xrow, PRN group) items, n_xrow * groups
   float* dump = nullptr;  // surface form: as RowArgs::dump
+  const int* rowmap = nullptr;  // sub-row mode: as RowArgs::rowmap
 };
 
 // conj(x) * r on f16 pairs (two packed instructions)
@@ -1965,7 +1966,7 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
         dbase = ((((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) * 2 + h)) * P;
       } else {
         const long long rq = it / SUB;
-        const long long r = big_row(a.r0 + rq, a.n_prn, a.n_bin, a.n_ant, a.hs, a.tile);
+        const long long r = a.rowmap[rq];
         dbase = static_cast<size_t>(r) * (SUB * P) + static_cast<size_t>(it - rq * SUB);
         dstride = SUB;
       }
@@ -2077,12 +2078,15 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
     // Top2 set may hold them.
     Top2 main_set, extra_set;
     float accn = 0.0f;  // sum of powers: NaN / overflow detector
-    // sub-rows keep only their argmax: a running key maximum (inf or NaN powers have the
-    // largest keys and are flagged at the end)
-    unsigned long long km = 0ull;
-    auto keep = [&](float p, int idx) {
-      const unsigned long long k = pkey(p, idx);
-      km = k > km ? k : km;
+    // sub-rows keep only their argmax: the lane keeps its powers (outputs 8 t and 8 t + 1 of
+    // butterflies j0 and j1, and of tile 32 on waves 0-3) and the maximum of their f32 bits
+    // (powers are >= 0: the bits order like the values, inf and NaN the largest); the samples
+    // are found only where the block maximum is (end of the sub-row)
+    float pj0[4][2], pj1[4][2], pex[2];
+    unsigned int pmx = 0u;
+    auto keep = [&](float& slot, float p, int idx) {
+      slot = p;
+      pmx = max(pmx, __float_as_uint(p));
       dump_at(p, idx);
     };
     {
@@ -2153,14 +2157,16 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
       } else {
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-          keep(c0[t][0] * c0[t][0] + c0[t][1] * c0[t][1], sample(base0, 8 * t));
-          keep(c1[t][0] * c1[t][0] + c1[t][1] * c1[t][1], sample(base1, 8 * t));
+          keep(pj0[t][0], c0[t][0] * c0[t][0] + c0[t][1] * c0[t][1], sample(base0, 8 * t));
+          keep(pj1[t][0], c1[t][0] * c1[t][0] + c1[t][1] * c1[t][1], sample(base1, 8 * t));
+          pj0[t][1] = pj1[t][1] = 0.0f;
           if (t < 3 || grp < 3) {  // output 8 t + 2 grp + 1 = 31 does not exist
-            keep(c0[t][2] * c0[t][2] + c0[t][3] * c0[t][3], sample(base0, 8 * t + 1));
-            keep(c1[t][2] * c1[t][2] + c1[t][3] * c1[t][3], sample(base1, 8 * t + 1));
+            keep(pj0[t][1], c0[t][2] * c0[t][2] + c0[t][3] * c0[t][3], sample(base0, 8 * t + 1));
+            keep(pj1[t][1], c1[t][2] * c1[t][2] + c1[t][3] * c1[t][3], sample(base1, 8 * t + 1));
           }
         }
       }
+      pex[0] = pex[1] = 0.0f;
       if (wave < 4) {
         const int j = 512 + ln16;
         halfx8 b[2];
@@ -2184,8 +2190,8 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
             dump_at(pb, sample(be, 1));
           }
         } else {
-          keep(pa, sample(be, 0));
-          if (okb) keep(pb, sample(be, 1));
+          keep(pex[0], pa, sample(be, 0));
+          if (okb) keep(pex[1], pb, sample(be, 1));
         }
       }
     }
@@ -2194,12 +2200,49 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
           main_set, extra_set, !(accn <= 3.0e38f), a.cw, red,
           a.rows + ((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) * 2 + h, pend);
     } else {
-      // sub-row it of the chunk: row r0 + it / SUB, output samples k0 + SUB m (k0 = it mod SUB)
+      // sub-row it of the chunk: row r0 + it / SUB (the pre-pass's row map), output samples
+      // k0 + SUB m (k0 = it mod SUB).  Block maximum M of the lanes' maxima (one barrier); in
+      // the waves holding M, the lanes holding it take the largest sample of power M among
+      // their outputs, and the wave folds (M, sample) into the row's key (several waves holding
+      // M: several atomic maxima, the largest sample wins).  inf / NaN powers are flagged.
       const long long rq = it / SUB;
       const int k0 = static_cast<int>(it - rq * SUB);
-      const long long r = big_row(a.r0 + rq, a.n_prn, a.n_bin, a.n_ant, a.hs, a.tile);
-      row_finish_key<T16>(km, (km >> 32) >= 0x7f800000ull, red, k0, SUB, a.rowkey + r,
-                          a.rownan + r);
+      const long long r = a.rowmap[rq];
+      const unsigned int wmax = wave_umax(pmx);
+      if (lane == 0) red.pk[wave] = wmax;
+      lds_barrier();
+      const unsigned int M = static_cast<unsigned int>(__builtin_amdgcn_readlane(
+          static_cast<int>(row16_umax(lane < NW16 ? red.pk[lane] : 0u)), 0));
+      if (wmax == M) {
+        unsigned int cand = 0u;  // 1 + sample, 0 for none
+        if (pmx == M) {
+          const int j0 = 16 * (wave < 8 ? wave : wave + 8) + ln16, j1 = j0 + 128;
+          const int base0 = pfa_base(j0) + NBC * 2 * grp, base1 = pfa_base(j1) + NBC * 2 * grp;
+          auto take = [&](float p, int idx) {
+            if (__float_as_uint(p) == M) cand = max(cand, static_cast<unsigned int>(idx + 1));
+          };
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            take(pj0[t][0], pfa_wrap(base0 + NBC * 8 * t));
+            take(pj1[t][0], pfa_wrap(base1 + NBC * 8 * t));
+            if (t < 3 || grp < 3) {
+              take(pj0[t][1], pfa_wrap(base0 + NBC * (8 * t + 1)));
+              take(pj1[t][1], pfa_wrap(base1 + NBC * (8 * t + 1)));
+            }
+          }
+          if (wave < 4) {
+            const int be = pfa_base(512 + ln16) + NBC * (8 * wave + 2 * grp);
+            take(pex[0], pfa_wrap(be));
+            if (wave < 3 || grp < 3) take(pex[1], pfa_wrap(be + NBC));
+          }
+        }
+        const unsigned int iw = wave_umax(cand);
+        if (lane == 0) {
+          atomicMax(a.rowkey + r, (static_cast<unsigned long long>(M) << 32) |
+                                      static_cast<unsigned int>(k0 + SUB * (static_cast<int>(iw) - 1)));
+          if (M >= 0x7f800000u) atomicOr(a.rownan + r, 1);
+        }
+      }
     }
   }
   }
@@ -3143,6 +3186,7 @@ void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, i
     a.n_ant = n_ant;
     a.tile = tile;
     a.dump = dump;
+    a.rowmap = rowmap;
     if (dump && R0 == 4)
       hipLaunchKernelGGL((fused::k_rows16<4, true>), dim3(blocks), dim3(fused::T16), 0, st, a);
     else if (dump)
