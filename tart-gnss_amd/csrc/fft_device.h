@@ -5,6 +5,7 @@
 // |.| unchanged, so only the forward direction exists here.
 //
 // Codelets:
+//   dft<1>               identity (one-segment rows)
 //   dft<2>, dft<4>       butterflies
 //   dft<8>, dft<16>      two-level splits (2x4, 4x4) with constant inner twiddles
 //   dft<3>, dft<5>, dft<7>, dft<11>, dft<13>, dft<31>
@@ -89,6 +90,11 @@ __device__ __forceinline__ float2 rot(float2 v) {
 
 template <int R>
 struct Dft;
+
+template <>
+struct Dft<1> {
+  __device__ __forceinline__ static void run(float2*) {}
+};
 
 template <>
 struct Dft<2> {

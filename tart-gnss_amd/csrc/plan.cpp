@@ -56,7 +56,11 @@ std::string group_kernels(const Group& g, int slot) {
     case 1:
       return g.fused ? "k_forward" : g.folded ? "k_fold_obs+k_pass" : "k_wipe_obs+k_pass";
     default:
-      return g.big ? "k_select+k_second_win" : "k_select";
+      return g.win_sub() ? "k_select+k_win_pre<" + std::string(g.big ? r0 : "1") +
+                               ">+k_subrows<" + std::string(g.big ? r0 : "1") +
+                               ",mask>+k_win_second"
+             : g.big ? "k_select+k_mul_win+k_pass+k_second_win"
+                     : "k_select";
   }
 }
 
@@ -308,6 +312,12 @@ void Plan::build(const tga_plan_desc& d) {
       rep_to_pfa(g, static_cast<long long>(n_prn) * 2 * static_cast<long long>(M));
       g.d_X.alloc(g.n_xrow * M * sizeof(float2));
       upload(g.d_tw, fused_twiddles(), stream);
+      if (g.win_sub()) {
+        const size_t nw = static_cast<size_t>(n_ant) * n_prn;
+        g.d_win.alloc(nw * sizeof(int));
+        g.d_m2.alloc(nw * sizeof(unsigned int));
+        g.d_Zw.alloc(nw * M * sizeof(float2));
+      }
       autotune(g);
     } else if (g.big) {
       g.d_X.alloc(g.n_xrow * M * sizeof(float2));
@@ -726,15 +736,16 @@ void Plan::search(const uint8_t* d_in, hipStream_t st, float* dump) {
   time_end(0, st, rows_bytes);
   time_begin(2, st);
   for (auto& g : groups) {
-    // the winners' second peaks of split groups are recomputed in f32
+    // the winners' second peaks of split groups and fused FP32 rows are recomputed in f32
     const float fold = g.folded ? 2.0f : 1.0f;
     const float gain = select_gain(g);
     launch_select(g.d_rows.as<RowOut>(), g.d_slots.as<PeakSlot>(), n_ant, g.n_prn, g.n_bin,
                   gain, g.scale, d_peaks.as<tga_peak>(), d_nan.as<int>(), st,
-                  g.big ? g.d_win.as<int>() : nullptr);
-    if (g.big && !g.natural())
+                  g.big || g.win_sub() ? g.d_win.as<int>() : nullptr);
+    if (g.win_sub())
       launch_big_second_sub(g.X, g.d_rep.as<float2>(), g.d_tw.as<float2>(),
-                            g.d_twb.as<float2>(), g.d_slots.as<PeakSlot>(), g.big_r0, g.cw,
+                            g.d_twb.as<float2>(), g.d_slots.as<PeakSlot>(),
+                            g.big ? g.big_r0 : 1, g.cw,
                             n_ant, g.n_prn, g.n_bin, g.folded, g.d_win.as<int>(), fold, g.scale,
                             g.d_Zw.as<float2>(), g.d_m2.as<unsigned int>(),
                             d_peaks.as<tga_peak>(), st);

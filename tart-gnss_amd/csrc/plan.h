@@ -36,7 +36,7 @@ struct Group {
   DevBuf d_Y, d_rowkey, d_rownan, d_rowmap, d_twb, d_win, d_Zw, d_Zws;  // long-period split
   DevBuf d_tw4, d_c31;  // four-step split: W_P^{n2 k1} as [k1][n2], radix-31 cos / sin
   DevBuf d_twl;  // sub-row split: W_P^e two-level table of the long-period forward
-  DevBuf d_m2;   // sub-row split: masked maximum (f32 bits) of each winning row
+  DevBuf d_m2;   // masked maximum (f32 bits) of each winning row (win_sub groups)
   DevBuf d_lev;  // sub-row split: decoded levels of the long-period forward
   float2* X = nullptr;  // where the forward spectra of the last search landed
   FusedRowsConfig rows_cfg;  // rows-kernel shape picked by the autotuner
@@ -45,6 +45,9 @@ struct Group {
   size_t dump_off = 0;  // surface form: first float of the group's rows in the staging buffer
   // FP32 split groups on the four-step form (variant 3) keep their spectra in natural order
   bool natural() const { return big && !big16 && rows_cfg.variant == 3; }
+  // second peaks from a masked transform of each winning row (launch_big_second_sub): the
+  // sub-row split groups and the fused FP32 rows of the reference form (one segment)
+  bool win_sub() const { return (big && !natural()) || (fused && !fp16 && nc == 0); }
 };
 
 struct SysEntry {

@@ -463,16 +463,18 @@ __device__ __forceinline__ void big_pre_core(const float2* __restrict__ xrow_p,
   // W_P^{n1 k0} = (W_P^{n1})^k0 by repeated products from one lookup of W_P^{n1} per position
   // in a table kept in the prime-factor order of the positions (at most R0 - 1 = 9 products:
   // relative error below 1e-6)
-  const float4 wq = reinterpret_cast<const float4*>(twb)[t >> 1];
-  const float2 w1 = make_float2(wq.x, wq.y), w2 = make_float2(wq.z, wq.w);
-  float2 wk = w1, wl = w2;
+  if constexpr (R0 > 1) {  // one segment (R0 = 1): the products alone, no table
+    const float4 wq = reinterpret_cast<const float4*>(twb)[t >> 1];
+    const float2 w1 = make_float2(wq.x, wq.y), w2 = make_float2(wq.z, wq.w);
+    float2 wk = w1, wl = w2;
 #pragma unroll
-  for (int k0 = 1; k0 < R0; ++k0) {
-    v[k0] = cmul(v[k0], wk);
-    u[k0] = cmul(u[k0], wl);
-    if (k0 + 1 < R0) {
-      wk = cmul(wk, w1);
-      wl = cmul(wl, w2);
+    for (int k0 = 1; k0 < R0; ++k0) {
+      v[k0] = cmul(v[k0], wk);
+      u[k0] = cmul(u[k0], wl);
+      if (k0 + 1 < R0) {
+        wk = cmul(wk, w1);
+        wl = cmul(wl, w2);
+      }
     }
   }
   // Non-temporal stores: the sub-rows are read once, by the transforms, and should not displace
@@ -851,10 +853,14 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
   // Lane offset of pass A's operand loads: threads beyond the 496 butterflies load at an offset
   // past the end of the view, which the buffer load returns as 0 (no branch per load).
   const int voa = tid < NBA ? tid * 8 : P * 8;
-  // Sub-rows with the matrix-core radix 31 keep only each mirrored output pair's larger power,
-  // max(|Y_kk|^2, |Y_{31-kk}|^2) with Y = A -/+ iB, A = C a and B = S b of output row kk, and the
-  // sign of the difference, which names its sample.
+  // Rows and sub-rows with the matrix-core radix 31 keep only each mirrored output pair's
+  // larger power, max(|Y_kk|^2, |Y_{31-kk}|^2) with Y = A -/+ iB, A = C a and B = S b of output
+  // row kk, and the sign of the difference, which names its sample.
   constexpr bool kPairs = SUB > 0 && VARIANT == 1;
+  // Rows of the matrix-core variant keep only the lane's running maximum of the 64-bit keys
+  // (power bits, sample) of their outputs and end with a one-barrier argmax; their second peak
+  // comes from a masked recomputation of the winning rows.
+  constexpr bool kRowKey = SUB == 0 && VARIANT == 1 && !NC;
   // One transform of a correlation row (or sub-row): `operand(ta, r)` gives pass A's input r of
   // butterfly ta; every output power goes to `sink(slot, p, idx)` with slot < 31 for the
   // thread's main set (outputs more than 32 samples apart) and slot 31 / 32 for its extra set,
@@ -1015,7 +1021,7 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
                 const float y1r = ar + bi, y1i = ai - br;  // Y_kk
                 const float p1 = y1r * y1r + y1i * y1i;
                 sink(slot, p1, out_idx(q, kk));
-                if (kk > 0) {
+                if (e > 0 || kk > 0) {
                   const float y2r = ar - bi, y2i = ai + br;  // Y_{31-kk}
                   const float p2 = y2r * y2r + y2i * y2i;
                   sink(slot, p2, out_idx(q, 31 - kk));
@@ -1123,6 +1129,9 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
       const size_t orow = ((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) * 2 + h;
       Top2 ms, es;
       float accn = 0.0f;  // sum of powers: NaN-propagating detector
+      // kRowKey: the lane's running maximum of the keys (power bits, sample); powers are >= 0,
+      // so their f32 bits order like the values, a NaN above everything
+      unsigned long long km = 0ull;
       float2 rc[kKeepX ? 33 : 1];
       if constexpr (kKeepX) {
 #pragma unroll
@@ -1138,8 +1147,13 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
             }
           },
           [&](int slot, float p, int idx) {
-            accn += p;
-            (slot < 31 ? ms : es).add(p, idx);
+            if constexpr (kRowKey) {
+              const unsigned long long k = pkey(p, idx);
+              km = k > km ? k : km;
+            } else {
+              accn += p;
+              (slot < 31 ? ms : es).add(p, idx);
+            }
             if constexpr (DUMP) a.dump[orow * P + idx] = p;
           },
           [&] {
@@ -1148,9 +1162,38 @@ __global__ __launch_bounds__(T) void k_rows(RowArgs a) {
             }
           },
           [](int, int, int, int, float, float) {});
-      // the two-barrier finish: deferring its second half to the next row's first barrier
-      // measured 6% slower here (wave 0 then carries the flush into the next pass)
-      row_finish<T>(ms, es, accn != accn, a.cw, red, a.rows + orow);
+      if constexpr (kRowKey) {
+        // block argmax with the last-maximum rule: per wave (power bits, 1 + sample) by two
+        // unsigned maxima, across waves in wave 0 after one barrier (which also orders this
+        // row's pass-C reads before the next row's pass-A writes; the slots are next written
+        // three barriers later)
+        const unsigned int kb = static_cast<unsigned int>(km >> 32);
+        const unsigned int pw = wave_umax(kb);
+        const unsigned int iw =
+            wave_umax(kb == pw ? static_cast<unsigned int>(km & 0xffffffffu) + 1u : 0u);
+        if (lane == 0) {
+          red.pk[wave] = pw;
+          red.ik[wave] = iw;
+        }
+        lds_barrier();
+        if (wave == 0) {
+          const bool mine = lane < NW;
+          unsigned int M, I;
+          lanes_argmax(mine ? red.pk[lane] : 0u, mine ? red.ik[lane] : 0u, M, I);
+          if (lane == 0) {
+            RowOut o;
+            o.p1 = __uint_as_float(M);
+            o.p2 = -1.0f;
+            o.i1 = static_cast<int32_t>(I) - 1;
+            o.nan = M > 0x7f800000u ? 1 : 0;
+            a.rows[orow] = o;
+          }
+        }
+      } else {
+        // the two-barrier finish: deferring its second half to the next row's first barrier
+        // measured 6% slower here (wave 0 then carries the flush into the next pass)
+        row_finish<T>(ms, es, accn != accn, a.cw, red, a.rows + orow);
+      }
     }
   } else {
     // sub-row s of the chunk: row r0 + s / SUB, output samples k0 + SUB m (k0 = s mod SUB).
@@ -3320,7 +3363,10 @@ void launch_big_second_sub(const float2* X, const float2* rep, const float2* tw,
                            hipStream_t st) {
   const int nw = n_ant * n_prn;
   const dim3 g1(static_cast<unsigned>(nw) * 32);
-  if (R0 == 4)
+  if (R0 == 1)
+    hipLaunchKernelGGL(k_win_pre<1>, g1, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
+                       folded ? 1 : 0, win, nw, Zw, m2key);
+  else if (R0 == 4)
     hipLaunchKernelGGL(k_win_pre<4>, g1, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
                        folded ? 1 : 0, win, nw, Zw, m2key);
   else
@@ -3338,7 +3384,9 @@ void launch_big_second_sub(const float2* X, const float2* rep, const float2* tw,
   a.slots = slots;
   a.m2key = m2key;
   const unsigned blocks = static_cast<unsigned>(a.n_sub < 2048 ? a.n_sub : 2048);
-  if (R0 == 4)
+  if (R0 == 1)
+    hipLaunchKernelGGL((fused::k_subrows<1, false, true>), dim3(blocks), dim3(fused::T), 0, st, a);
+  else if (R0 == 4)
     hipLaunchKernelGGL((fused::k_subrows<4, false, true>), dim3(blocks), dim3(fused::T), 0, st, a);
   else
     hipLaunchKernelGGL((fused::k_subrows<10, false, true>), dim3(blocks), dim3(fused::T), 0, st,
