@@ -56,8 +56,10 @@ std::string group_kernels(const Group& g, int slot) {
     case 1:
       return g.fused ? "k_forward" : g.folded ? "k_fold_obs+k_pass" : "k_wipe_obs+k_pass";
     default:
-      return g.win_sub() ? "k_select+k_win_pre<" + std::string(g.big ? r0 : "1") +
-                               ">+k_subrows<" + std::string(g.big ? r0 : "1") +
+      return g.win_sub() ? (g.fp16 ? std::string("k_select+k_win_pre16")
+                                   : "k_select+k_win_pre<" + std::string(g.big ? r0 : "1") +
+                                         ">") +
+                               "+k_subrows<" + std::string(g.big ? r0 : "1") +
                                ",mask>+k_win_second"
              : g.big ? "k_select+k_mul_win+k_pass+k_second_win"
                      : "k_select";
@@ -307,17 +309,13 @@ void Plan::build(const tga_plan_desc& d) {
       g.d_rep.release();
       g.d_X16.alloc(g.n_xrow * M * sizeof(uint32_t));
       upload(g.d_tw, fused_twiddles(), stream);
+      alloc_winners(g);
       autotune(g);
     } else if (g.fused) {  // FP32 fused rows, reference or non-coherent form
       rep_to_pfa(g, static_cast<long long>(n_prn) * 2 * static_cast<long long>(M));
       g.d_X.alloc(g.n_xrow * M * sizeof(float2));
       upload(g.d_tw, fused_twiddles(), stream);
-      if (g.win_sub()) {
-        const size_t nw = static_cast<size_t>(n_ant) * n_prn;
-        g.d_win.alloc(nw * sizeof(int));
-        g.d_m2.alloc(nw * sizeof(unsigned int));
-        g.d_Zw.alloc(nw * M * sizeof(float2));
-      }
+      if (g.win_sub()) alloc_winners(g);
       autotune(g);
     } else if (g.big) {
       g.d_X.alloc(g.n_xrow * M * sizeof(float2));
@@ -350,12 +348,9 @@ void Plan::build(const tga_plan_desc& d) {
       g.d_rowmap.alloc(static_cast<size_t>(g.chunk_rows) * sizeof(int));
       if (g.n_rows > (1LL << 31) - 1)
         fail(TGA_ERR_UNSUPPORTED, "a long-period group holds more than 2^31 - 1 rows");
-      const size_t nw = static_cast<size_t>(n_ant) * n_prn;
-      g.d_win.alloc(nw * sizeof(int));
-      g.d_m2.alloc(nw * sizeof(unsigned int));
-      g.d_Zw.alloc(nw * M * sizeof(float2));
+      alloc_winners(g);
       autotune_split(g);
-      if (g.natural()) g.d_Zws.alloc(nw * M * sizeof(float2));  // the generic transforms' ping-pong
+      if (g.natural()) g.d_Zws.alloc(g.d_Zw.bytes);  // the generic transforms' ping-pong
       // the sub-row forms read their spectra segment by segment in the prime-factor order of
       // 16368; the four-step form reads them in natural order
       if (!g.natural()) {
@@ -455,6 +450,15 @@ void tune_pick(Group& g, const std::vector<FusedRowsConfig>& cands, hipStream_t 
     }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+}
+
+// Buffers of the winners' masked second peaks: the winning row of each (antenna, PRN) pair,
+// its masked maximum and its spectrum products (one row of M complex values per pair).
+void Plan::alloc_winners(Group& g) {
+  const size_t nw = static_cast<size_t>(n_ant) * g.n_prn;
+  g.d_win.alloc(nw * sizeof(int));
+  g.d_m2.alloc(nw * sizeof(unsigned int));
+  g.d_Zw.alloc(nw * g.M * sizeof(float2));
 }
 
 // Rows-kernel autotuner: times every candidate shape on the group's real sizes, with
@@ -746,9 +750,12 @@ void Plan::search(const uint8_t* d_in, hipStream_t st, float* dump) {
       launch_big_second_sub(g.X, g.d_rep.as<float2>(), g.d_tw.as<float2>(),
                             g.d_twb.as<float2>(), g.d_slots.as<PeakSlot>(),
                             g.big ? g.big_r0 : 1, g.cw,
-                            n_ant, g.n_prn, g.n_bin, g.folded, g.d_win.as<int>(), fold, g.scale,
+                            n_ant, g.n_prn, g.n_bin, g.folded, g.d_win.as<int>(),
+                            g.fp16 ? gain : fold, g.scale,
                             g.d_Zw.as<float2>(), g.d_m2.as<unsigned int>(),
-                            d_peaks.as<tga_peak>(), st);
+                            d_peaks.as<tga_peak>(), st,
+                            g.fp16 ? g.d_X16.as<uint32_t>() : nullptr,
+                            g.fp16 ? g.d_rep16.as<uint32_t>() : nullptr);
     else if (g.big)
       launch_big_second(g.X, g.d_rep.as<float2>(), g.d_slots.as<PeakSlot>(),
                         static_cast<int>(g.M), static_cast<int>(g.P), g.cw, n_ant, g.n_prn,

@@ -46,8 +46,8 @@ struct Group {
   // FP32 split groups on the four-step form (variant 3) keep their spectra in natural order
   bool natural() const { return big && !big16 && rows_cfg.variant == 3; }
   // second peaks from a masked transform of each winning row (launch_big_second_sub): the
-  // sub-row split groups and the fused FP32 rows of the reference form (one segment)
-  bool win_sub() const { return (big && !natural()) || (fused && !fp16 && nc == 0); }
+  // sub-row split groups and the fused rows of the reference form (one segment)
+  bool win_sub() const { return (big && !natural()) || (fused && nc == 0); }
 };
 
 struct SysEntry {
@@ -113,6 +113,7 @@ struct Plan {
   void autotune(Group& g);
   void autotune_split(Group& g);
   void rep_to_pfa(Group& g, long long n);
+  void alloc_winners(Group& g);
 };
 
 // Error-to-status wrapper shared by the C ABI translation units.

@@ -182,14 +182,16 @@ void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, i
 // surface form: v[i] = (sqrt(v[i]) * gain) * scale in place (the row values of k_select)
 void launch_surface_finish(float* v, long long n, float gain, float scale, hipStream_t st);
 // second peaks of the winning rows of sub-row split groups (spectra in the segment-wise
-// prime-factor order; R0 = 1: the fused 16368-sample rows): their sub-rows into Zw
-// (n_ant * n_prn * R0 * 16368 complex values), the masked sub-row transforms into m2key
-// (n_ant * n_prn), then the peaks' second_peak
+// prime-factor order; R0 = 1: the fused 16368-sample rows, read from the f16 spectra X16 and
+// rep16 when those are given): their sub-rows into Zw (n_ant * n_prn * R0 * 16368 complex
+// values), the masked sub-row transforms into m2key (n_ant * n_prn), then the peaks'
+// second_peak
 void launch_big_second_sub(const float2* X, const float2* rep, const float2* tw,
                            const float2* twb, const PeakSlot* slots, int R0, int cw, int n_ant,
                            int n_prn, int n_bin, bool folded, const int* win, float gain,
                            float scale, float2* Zw, unsigned int* m2key, tga_peak* out,
-                           hipStream_t st);
+                           hipStream_t st, const uint32_t* X16 = nullptr,
+                           const uint32_t* rep16 = nullptr);
 // second peaks of the winning rows; Z, Zs hold n_ant * n_prn * M complex values; natural: X and
 // rep in natural order (the four-step split) instead of the segment-wise prime-factor order
 void launch_big_second(const float2* X, const float2* rep, const PeakSlot* slots, int M, int P,

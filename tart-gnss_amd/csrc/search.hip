@@ -1971,7 +1971,6 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
 #pragma unroll
   for (int t = 0; t < 2; ++t) fb[t] = dft_frag<16>(stw, t, 0, tid & 63);
   __syncthreads();
-  RowPending pend;  // the previous row, written after this row's first barrier
   // SUB == 0: persistent blocks over the items (xrow, PRN group) b, b + gridDim, ... with
   // item = xrow * groups + g; gridDim is a multiple of groups, so a block keeps g = b mod groups
   // and, blocks being dealt round-robin to the 8 XCDs, a group's replicas stay in one XCD's L2.
@@ -2087,7 +2086,6 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
       }
     }
     lds_barrier();
-    if constexpr (SUB == 0) row_flush<T16>(red, pend);
     // ---- pass B: radix 16, 64 tiles (the last holds 15 butterflies): wave + 16 s
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
@@ -2114,13 +2112,17 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
     lds_barrier();
     // ---- pass C: radix 31, 33 tiles: tiles t0 and t0 + 8 (t0 = wave, or wave + 8 for waves
     // 8-15) on every wave, and the four output tiles t of tile 32 on waves 0-3 (wave t takes
-    // output tile t).  All products are issued before the first power is used, and the 16
-    // powers of the two main tiles are merged as a tree, so neither the matrix-core latency nor
-    // the top-2 chain serialises.  Output q of butterfly j is sample pfa_base(j) + 528 q mod P;
-    // the 16 outputs of a lane are pairwise at least 128 samples apart (ring distance), so each
-    // Top2 set may hold them.
-    Top2 main_set, extra_set;
-    float accn = 0.0f;  // sum of powers: NaN / overflow detector
+    // output tile t).  All products are issued before the first power is used.  Output q of
+    // butterfly j is sample pfa_base(j) + 528 q mod P.
+    // Rows keep the lane's running maximum of the 64-bit keys (power bits, sample) of its
+    // outputs (powers are >= 0: the bits order like the values, inf and NaN the largest); the
+    // second peak of the winning rows comes from a masked recomputation.
+    unsigned long long km = 0ull;
+    auto keyed = [&](float p, int idx) {
+      const unsigned long long k = pkey(p, idx);
+      km = k > km ? k : km;
+      dump_at(p, idx);
+    };
     // sub-rows keep only their argmax: the lane keeps its powers (outputs 8 t and 8 t + 1 of
     // butterflies j0 and j1, and of tile 32 on waves 0-3) and the maximum of their f32 bits
     // (powers are >= 0: the bits order like the values, inf and NaN the largest); the samples
@@ -2170,33 +2172,15 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
         c1[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(f, b1[1], c1[t], 0, 0, 0);
       }
       if constexpr (SUB == 0) {
-        Top2 lv[16];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-          const float pa0 = c0[t][0] * c0[t][0] + c0[t][1] * c0[t][1];
-          const float pb0 = c0[t][2] * c0[t][2] + c0[t][3] * c0[t][3];
-          const float pa1 = c1[t][0] * c1[t][0] + c1[t][1] * c1[t][1];
-          const float pb1 = c1[t][2] * c1[t][2] + c1[t][3] * c1[t][3];
-          const bool okb = t < 3 || grp < 3;  // output 8 t + 2 grp + 1 = 31 does not exist
-          accn += (pa0 + pa1) + (okb ? pb0 + pb1 : 0.0f);
-          if constexpr (DUMP) {
-            dump_at(pa0, sample(base0, 8 * t));
-            dump_at(pa1, sample(base1, 8 * t));
-            if (okb) {
-              dump_at(pb0, sample(base0, 8 * t + 1));
-              dump_at(pb1, sample(base1, 8 * t + 1));
-            }
+          keyed(c0[t][0] * c0[t][0] + c0[t][1] * c0[t][1], sample(base0, 8 * t));
+          keyed(c1[t][0] * c1[t][0] + c1[t][1] * c1[t][1], sample(base1, 8 * t));
+          if (t < 3 || grp < 3) {  // output 8 t + 2 grp + 1 = 31 does not exist
+            keyed(c0[t][2] * c0[t][2] + c0[t][3] * c0[t][3], sample(base0, 8 * t + 1));
+            keyed(c1[t][2] * c1[t][2] + c1[t][3] * c1[t][3], sample(base1, 8 * t + 1));
           }
-          lv[4 * t + 0] = Top2::leaf(pa0, sample(base0, 8 * t));
-          lv[4 * t + 1] = Top2::leaf(pa1, sample(base1, 8 * t));
-          lv[4 * t + 2] = okb ? Top2::leaf(pb0, sample(base0, 8 * t + 1)) : Top2();
-          lv[4 * t + 3] = okb ? Top2::leaf(pb1, sample(base1, 8 * t + 1)) : Top2();
         }
-#pragma unroll
-        for (int w = 8; w >= 1; w >>= 1)
-#pragma unroll
-          for (int i = 0; i < w; ++i) lv[i] = Top2::merge(lv[i], lv[i + w]);
-        main_set = lv[0];
       } else {
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
@@ -2224,14 +2208,8 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
         const float pb = acc[2] * acc[2] + acc[3] * acc[3];
         const bool okb = wave < 3 || grp < 3;
         if constexpr (SUB == 0) {
-          accn += pa;
-          extra_set.add(pa, sample(be, 0));
-          dump_at(pa, sample(be, 0));
-          if (okb) {
-            accn += pb;
-            extra_set.add(pb, sample(be, 1));
-            dump_at(pb, sample(be, 1));
-          }
+          keyed(pa, sample(be, 0));
+          if (okb) keyed(pb, sample(be, 1));
         } else {
           keep(pex[0], pa, sample(be, 0));
           if (okb) keep(pex[1], pb, sample(be, 1));
@@ -2239,9 +2217,31 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
       }
     }
     if constexpr (SUB == 0) {
-      row_finish_begin<T16>(
-          main_set, extra_set, !(accn <= 3.0e38f), a.cw, red,
-          a.rows + ((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) * 2 + h, pend);
+      // block argmax with the last-maximum rule: per wave (power bits, 1 + sample) by two
+      // unsigned maxima, across waves in wave 0 after one barrier (which also orders this row's
+      // pass-C reads before the next row's pass-A writes; the slots are next written two
+      // barriers later); inf / NaN powers are flagged
+      const unsigned int kb = static_cast<unsigned int>(km >> 32);
+      const unsigned int pw = wave_umax(kb);
+      const unsigned int iw =
+          wave_umax(kb == pw ? static_cast<unsigned int>(km & 0xffffffffu) + 1u : 0u);
+      if (lane == 0) {
+        red.pk[wave] = pw;
+        red.ik[wave] = iw;
+      }
+      lds_barrier();
+      if (wave == 0) {
+        unsigned int M, I;
+        lanes_argmax(lane < NW16 ? red.pk[lane] : 0u, lane < NW16 ? red.ik[lane] : 0u, M, I);
+        if (lane == 0) {
+          RowOut o;
+          o.p1 = __uint_as_float(M);
+          o.p2 = -1.0f;
+          o.i1 = static_cast<int32_t>(I) - 1;
+          o.nan = M >= 0x7f800000u ? 1 : 0;
+          a.rows[((static_cast<size_t>(ant) * a.n_prn + prn) * a.n_bin + bin) * 2 + h] = o;
+        }
+      }
     } else {
       // sub-row it of the chunk: row r0 + it / SUB (the pre-pass's row map), output samples
       // k0 + SUB m (k0 = it mod SUB).  Block maximum M of the lanes' maxima (one barrier); in
@@ -2288,10 +2288,6 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
       }
     }
   }
-  }
-  if constexpr (SUB == 0) {
-    lds_barrier();
-    row_flush<T16>(red, pend);
   }
 }
 
@@ -2727,6 +2723,40 @@ __global__ __launch_bounds__(256) void k_win_pre(const float2* __restrict__ X,
   fused::big_pre_core<R0, false>(X + (folded ? xrow : xrow >> 1) * PL,
                                  rep + (static_cast<long long>(prn) * 2 + (bh & 1)) * PL, twb, t,
                                  Zw + static_cast<long long>(w) * PL);
+  if (t == 0) m2key[w] = 0u;
+}
+
+// The same for the FP16 fused rows (one segment): Zw[w] = conj(X) R of the winning row in f32
+// from the f16 spectra (both scaled by kF16Scale, as the FP16 rows kernel multiplies them), in
+// their prime-factor order.
+__global__ __launch_bounds__(256) void k_win_pre16(const uint32_t* __restrict__ X16,
+                                                   const uint32_t* __restrict__ rep16,
+                                                   int n_prn, int n_bin,
+                                                   const int* __restrict__ win, int nw,
+                                                   float2* __restrict__ Zw,
+                                                   unsigned int* __restrict__ m2key) {
+  const int w = static_cast<int>(blockIdx.x >> 5);
+  if (w >= nw) return;
+  const int t = 512 * static_cast<int>(blockIdx.x & 31) + 2 * static_cast<int>(threadIdx.x);
+  if (t >= kFusedP) return;
+  const int ant = w / n_prn;
+  const int prn = w - ant * n_prn;
+  const int bh = win[w];
+  const long long xrow = static_cast<long long>(ant) * n_bin * 2 + bh;
+  const uint2 xw = *reinterpret_cast<const uint2*>(X16 + xrow * kFusedP + t);
+  const uint2 rw = *reinterpret_cast<const uint2*>(
+      rep16 + (static_cast<long long>(prn) * 2 + (bh & 1)) * kFusedP + t);
+  auto f2 = [](uint32_t u) {
+    const fused::half2_t h = __builtin_bit_cast(fused::half2_t, u);
+    return make_float2(static_cast<float>(h.x), static_cast<float>(h.y));
+  };
+  float4 z;
+  const float2 z0 = cconjmul(f2(xw.x), f2(rw.x)), z1 = cconjmul(f2(xw.y), f2(rw.y));
+  z.x = z0.x;
+  z.y = z0.y;
+  z.z = z1.x;
+  z.w = z1.y;
+  *reinterpret_cast<float4*>(Zw + static_cast<long long>(w) * kFusedP + t) = z;
   if (t == 0) m2key[w] = 0u;
 }
 
@@ -3360,10 +3390,13 @@ void launch_big_second_sub(const float2* X, const float2* rep, const float2* tw,
                            const float2* twb, const PeakSlot* slots, int R0, int cw, int n_ant,
                            int n_prn, int n_bin, bool folded, const int* win, float gain,
                            float scale, float2* Zw, unsigned int* m2key, tga_peak* out,
-                           hipStream_t st) {
+                           hipStream_t st, const uint32_t* X16, const uint32_t* rep16) {
   const int nw = n_ant * n_prn;
   const dim3 g1(static_cast<unsigned>(nw) * 32);
-  if (R0 == 1)
+  if (X16 != nullptr)
+    hipLaunchKernelGGL(k_win_pre16, g1, dim3(256), 0, st, X16, rep16, n_prn, n_bin, win, nw, Zw,
+                       m2key);
+  else if (R0 == 1)
     hipLaunchKernelGGL(k_win_pre<1>, g1, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
                        folded ? 1 : 0, win, nw, Zw, m2key);
   else if (R0 == 4)
