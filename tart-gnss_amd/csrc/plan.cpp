@@ -54,7 +54,11 @@ std::string group_kernels(const Group& g, int slot) {
                              "," + r0 + ">"
                        : "k_mul_rows+k_pass+k_row_reduce";
     case 1:
-      return g.fused ? "k_forward" : g.folded ? "k_fold_obs+k_pass" : "k_wipe_obs+k_pass";
+      return g.fused ? "k_forward"
+             : g.big && !g.natural()
+                 ? "k_levels_dec+k_fwd_long<" + r0 + ">+k_fwd_twist<" + r0 + ">"
+             : g.folded ? "k_fold_obs+k_pass"
+                        : "k_wipe_obs+k_pass";
     default:
       return g.win_sub() ? (g.fp16 ? std::string("k_select+k_win_pre16")
                                    : "k_select+k_win_pre<" + std::string(g.big ? r0 : "1") +
