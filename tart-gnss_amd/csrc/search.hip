@@ -1630,8 +1630,12 @@ __global__ __launch_bounds__(T) void k_fwd_long(LongFwdArgs a) {
   constexpr int NH = FOLD ? 2 : 1;
   constexpr int NBA = P / 33;  // 496
   constexpr int NBB = P / 16;  // 1023
-  __shared__ float2 buf[P];
+  // the natural-order row and the output scatter use a padded layout, one slot in 17
+  // (pad(m) = m + m / 16), so the gathers at lane strides 528 and 496 and the scatter at 496
+  // fall on distinct banks; the passes in between use the first P slots unpadded
+  __shared__ float2 buf[P + P / 16 + 1];
   __shared__ float2 stw[256];
+  auto pad = [](int m) { return m + (m >> 4); };
   const int tid = threadIdx.x;
   for (int i = tid; i < 256; i += T) stw[i] = a.tw[i];
   // blocks (bin, r, ant, h), ant and h fastest so that the blocks reading one slice of the
@@ -1674,9 +1678,9 @@ __global__ __launch_bounds__(T) void k_fwd_long(LongFwdArgs a) {
         if constexpr (FOLD) {
           const float l1 = v1[u];
           const float2 x1 = make_float2(w1[u].x * l1, w1[u].y * l1);
-          buf[m] = h == 0 ? cadd(x0, x1) : cmul(csub(x0, x1), tw[u]);
+          buf[pad(m)] = h == 0 ? cadd(x0, x1) : cmul(csub(x0, x1), tw[u]);
         } else {
-          buf[m] = x0;
+          buf[pad(m)] = x0;
         }
       }
     }
@@ -1697,7 +1701,7 @@ __global__ __launch_bounds__(T) void k_fwd_long(LongFwdArgs a) {
       int m = (1023 * n2 + 528 * n3) % P;
 #pragma unroll
       for (int n1 = 0; n1 < 33; ++n1) {
-        v[n1] = buf[m];
+        v[n1] = buf[pad(m)];
         m += 496;
         m = m >= P ? m - P : m;
       }
@@ -1759,15 +1763,15 @@ __global__ __launch_bounds__(T) void k_fwd_long(LongFwdArgs a) {
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int kk = 4 * grp + e;
-        buf[base + kk] = yo[ts][e][0];
-        if (kk > 0) buf[base + 31 - kk] = yo[ts][e][1];
+        buf[pad(base + kk)] = yo[ts][e][0];
+        if (kk > 0) buf[pad(base + 31 - kk)] = yo[ts][e][1];
       }
     }
   }
   __syncthreads();
   const size_t xrow = (static_cast<size_t>(ant) * a.n_bin + bin) * NH + h;
   float2* out = a.Y + (xrow * R0 + r) * P;
-  for (int s2 = tid; s2 < P; s2 += T) out[s2] = buf[s2];
+  for (int s2 = tid; s2 < P; s2 += T) out[s2] = buf[pad(s2)];
 }
 
 // Step 2: per position s of a row (natural index t = pfa_nat(s)), the R0 values Y_r[s] times
