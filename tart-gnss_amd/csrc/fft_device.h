@@ -236,9 +236,10 @@ template <> struct Dft<7> { __device__ __forceinline__ static void run(float2* v
 template <> struct Dft<11> { __device__ __forceinline__ static void run(float2* v) { DftOdd<11>::run(v); } };
 template <> struct Dft<13> { __device__ __forceinline__ static void run(float2* v) { DftOdd<13>::run(v); } };
 template <> struct Dft<31> { __device__ __forceinline__ static void run(float2* v) { DftOdd<31>::run(v); } };
-// 33 = 3 x 11 and 10 = 2 x 5 in registers, prime-factor form
+// 33 = 3 x 11, 10 = 2 x 5 and 20 = 4 x 5 in registers, prime-factor form
 template <> struct Dft<33> { __device__ __forceinline__ static void run(float2* v) { DftPfa<3, 11>::run(v); } };
 template <> struct Dft<10> { __device__ __forceinline__ static void run(float2* v) { DftPfa<2, 5>::run(v); } };
+template <> struct Dft<20> { __device__ __forceinline__ static void run(float2* v) { DftPfa<4, 5>::run(v); } };
 
 // Twiddle W_n^e from the two-level table: tw1[e % split] * tw2[e / split].
 __device__ __forceinline__ float2 twiddle2(const float2* tw1, const float2* tw2, int split,

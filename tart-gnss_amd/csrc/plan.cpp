@@ -50,6 +50,8 @@ std::string group_kernels(const Group& g, int slot) {
              : g.big16 ? "k_big_pre<" + r0 + ",f16>+k_rows16<" + r0 + ">"
              : g.natural() ? "k_4s_pre<" + r0 + ">+k_4s_rows<" + r0 + ">"
              : g.big && g.rows_cfg.variant == 2 ? "k_big_pre<" + r0 + ">+k_subrows<" + r0 + ">"
+             : g.big && g.rows_cfg.variant == 5
+                 ? "k_half_pre<" + r0 + ">+k_halfrows<" + std::to_string(2 * g.big_r0) + ">"
              : g.big   ? "k_big_pre<" + r0 + ">+k_rows<" + std::to_string(g.rows_cfg.variant) +
                              "," + r0 + ">"
                        : "k_mul_rows+k_pass+k_row_reduce";
@@ -329,6 +331,7 @@ void Plan::build(const tga_plan_desc& d) {
       if (!g.big16) {
         upload(g.d_tw4, four_step_twiddles(g.P), stream);
         upload(g.d_c31, radix31_table(), stream);
+        upload(g.d_twh, half_twiddles(g.P), stream);
       }
       // sub-rows of a chunk of rows per pre-pass / transform pair, on the plan's stream (a
       // second stream for the pre-pass of the next chunk measured no faster: the two kernels
@@ -525,9 +528,11 @@ void Plan::autotune(Group& g) {
 // the fastest.  Candidates: the FP32 form (variants 0, 1 and 2: R0 sub-rows of 16368 through the
 // LDS transform with the VALU or the matrix-core radix 31 (k_rows: 0, 1), or with the
 // matrix-core radix 31, pass B in place and only each lane's best output pair kept (k_subrows:
-// 2); variant 3: the four-step N1 x 496 form; the FP16 form has one) and the tile shape of the
-// row order (TX forward spectra x TP replica spectra whose rows run together, so their spectra
-// are fetched once into L2).  TGA_SPLIT_VARIANT=0..3 and TGA_SPLIT_TILE=TXxTP pin them.  The
+// 2; 2 R0 half sub-rows of 8184 points with two transforming workgroups per CU: 5); variant 3:
+// the four-step N1 x 496 form; the FP16 form has one) and the
+// tile shape of the row order (TX forward spectra x TP replica spectra whose rows run together,
+// so their spectra are fetched once into L2).  TGA_SPLIT_VARIANT=0..5 and TGA_SPLIT_TILE=TXxTP
+// pin them.  The
 // choice fixes the spectra's layout: natural order for variant 3, else the segment-wise
 // prime-factor order (the replicas are reordered after tuning).
 void Plan::autotune_split(Group& g) {
@@ -536,10 +541,14 @@ void Plan::autotune_split(Group& g) {
   std::vector<int> variants;
   if (g.big16)
     variants = {0};
-  else if (pin_v)
-    variants = {std::min(3, std::max(0, std::atoi(pin_v)))};
-  else
-    variants = {2, 1, 0, 3};
+  else if (pin_v) {
+    const int v = std::atoi(pin_v);
+    if (v < 0 || v > 5 || v == 4)
+      fail(TGA_ERR_INVALID_ARGUMENT, "TGA_SPLIT_VARIANT must be 0, 1, 2, 3 or 5");
+    variants = {v};
+  } else {
+    variants = {5, 2, 1, 0, 3};
+  }
   std::vector<int> tiles;
   if (pin_t) {
     int tx = 0, tp = 0;
@@ -575,6 +584,17 @@ void Plan::autotune_split(Group& g) {
                     g.d_c31.as<float2>(), g.big_r0, g.n_prn, g.n_bin, n_ant, hs, c.tile, 0, r1,
                     g.d_Y.as<float2>(), g.d_rowkey.as<unsigned long long>(),
                     g.d_rownan.as<int>(), g.d_rows.as<RowOut>(), stream);
+      return;
+    }
+    if (c.variant == 5) {
+      launch_half_pre(g.d_X.as<float2>(), g.d_rep.as<float2>(), g.d_twh.as<float2>(), g.big_r0,
+                      g.n_prn, g.n_bin, n_ant, hs, c.tile, 0, r1, g.d_Y.as<float2>(),
+                      g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(),
+                      g.d_rowmap.as<int>(), stream);
+      launch_half_sub(g.d_tw.as<float2>(), g.big_r0, g.n_prn, g.n_bin, n_ant, hs, c.tile, 0, r1,
+                      g.d_Y.as<float2>(), g.d_rowkey.as<unsigned long long>(),
+                      g.d_rownan.as<int>(), g.d_rowmap.as<int>(), g.d_rows.as<RowOut>(),
+                      stream);
       return;
     }
     launch_big_pre(g.d_X.as<float2>(), g.d_rep.as<float2>(), g.d_twb.as<float2>(), g.big_r0,
@@ -718,6 +738,18 @@ void Plan::search(const uint8_t* d_in, hipStream_t st, float* dump) {
       }
       for (long long r0 = 0; r0 < n_enum; r0 += g.chunk_rows) {
         const long long r1 = std::min(n_enum, r0 + g.chunk_rows);
+        if (g.rows_cfg.variant == 5 && !g.big16) {
+          launch_half_pre(g.X, g.d_rep.as<float2>(), g.d_twh.as<float2>(), g.big_r0, g.n_prn,
+                          g.n_bin, n_ant, hs, g.rows_cfg.tile, r0, r1, g.d_Y.as<float2>(),
+                          g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(),
+                          g.d_rowmap.as<int>(), st);
+          launch_half_sub(g.d_tw.as<float2>(), g.big_r0, g.n_prn, g.n_bin, n_ant, hs,
+                          g.rows_cfg.tile, r0, r1, g.d_Y.as<float2>(),
+                          g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(),
+                          g.d_rowmap.as<int>(), g.d_rows.as<RowOut>(), st,
+                          dump ? dump + g.dump_off : nullptr);
+          continue;
+        }
         launch_big_pre(g.X, g.d_rep.as<float2>(), g.d_twb.as<float2>(), g.big_r0, g.n_prn,
                        g.n_bin, n_ant, hs, g.rows_cfg.tile, r0, r1, g.d_Y.p, g.big16,
                        g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(),

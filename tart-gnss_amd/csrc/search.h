@@ -173,12 +173,28 @@ void launch_big_pre(const float2* X, const float2* rep, const float2* twb, int R
                     hipStream_t st);
 // variant: the FP32 sub-row transforms (0: k_rows with the VALU radix-31 codelet, 1: k_rows with
 // the matrix-core radix 31 and the next sub-row prefetched into registers, 2: k_subrows, the
-// matrix-core form with pass B in place and each lane's best output pair); the FP16 form has one.
+// matrix-core form with pass B in place and each lane's best output pair); the FP16 form has one
+// (variant 5, the half sub-rows, has its own launchers below).
 // rowmap: the row index of each of the chunk's rows, written by launch_big_pre (r1 - r0 ints).
 void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, int hs,
                     int tile, long long r0, long long r1, const void* Y, bool f16,
                     unsigned long long* rowkey, int* rownan, const int* rowmap, RowOut* rows,
                     hipStream_t st, int variant, float* dump = nullptr);
+// Half sub-rows (FP32 split variant 5): the R0 x 16368-point rows as R = 2 R0 sub-rows of
+// 8184 = 33 x 8 x 31 points in their CRT order (two transforming workgroups per CU), read from
+// the same segment-wise prime-factor spectra.  twh = half_twiddles(P): W_P^m and W_P^{R0 m} for
+// the natural index m of each of the 8184 positions.  launch_half_pre writes a chunk's sub-rows
+// to Y and clears the rows' keys; launch_half_sub transforms them and writes the rows' RowOut;
+// dump as launch_big_sub's.
+std::vector<float2> half_twiddles(size_t P);
+void launch_half_pre(const float2* X, const float2* rep, const float2* twh, int R0, int n_prn,
+                     int n_bin, int n_ant, int hs, int tile, long long r0, long long r1,
+                     float2* Y, unsigned long long* rowkey, int* rownan, int* rowmap,
+                     hipStream_t st);
+void launch_half_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, int hs, int tile,
+                     long long r0, long long r1, const float2* Y, unsigned long long* rowkey,
+                     int* rownan, const int* rowmap, RowOut* rows, hipStream_t st,
+                     float* dump = nullptr);
 // surface form: v[i] = (sqrt(v[i]) * gain) * scale in place (the row values of k_select)
 void launch_surface_finish(float* v, long long n, float gain, float scale, hipStream_t st);
 // second peaks of the winning rows of sub-row split groups (spectra in the segment-wise

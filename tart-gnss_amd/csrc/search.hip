@@ -1434,14 +1434,16 @@ __global__ __launch_bounds__(T) void k_subrows(RowArgs a) {
   // transformed and stored at the top, and the registers reloaded with the next sub-row's
   // inputs, which then have a whole sub-row to arrive.
   float2 cur[33];
+  // the row index of the sub-row (big_row(a.r0 + rq, ...), from the pre-pass's map) is loaded
+  // one sub-row ahead like its inputs, so its latency is not exposed where the atomic needs it
+  int rmap = MASK ? 0 : a.rowmap[sidx / SUB];
   load_sub(sidx, cur);
   for (;;) {
     const long long s = sidx;
     const long long rq = s / SUB;
     const int k0 = static_cast<int>(s - rq * SUB);
-    // the row index (big_row(a.r0 + rq, ...), from the pre-pass), or in the masked mode the
-    // winner and the centre of its exclusion window
-    const long long r = MASK ? rq : a.rowmap[rq];
+    // the row index, or in the masked mode the winner and the centre of its exclusion window
+    const long long r = MASK ? rq : __builtin_amdgcn_readfirstlane(rmap);
     int centre = 0;
     if constexpr (MASK) {
       const int ant = static_cast<int>(rq / a.n_prn);
@@ -1460,7 +1462,10 @@ __global__ __launch_bounds__(T) void k_subrows(RowArgs a) {
         for (int q = 0; q < 33; ++q) buf[ta * 33 + q] = cur[q];
       }
     }
-    if (s + G < a.n_sub) load_sub(s + G, cur);
+    if (s + G < a.n_sub) {
+      if constexpr (!MASK) rmap = a.rowmap[(s + G) / SUB];
+      load_sub(s + G, cur);
+    }
     lds_barrier();
     // ---- pass B (radix 16) in place
     {
@@ -1598,6 +1603,286 @@ __global__ __launch_bounds__(T) void k_subrows(RowArgs a) {
         atomicMax(a.rowkey + r,
                   (static_cast<unsigned long long>(M) << 32) |
                       static_cast<unsigned int>(k0 + SUB * (static_cast<int>(iw) - 1)));
+        if (M > 0x7f800000u) atomicOr(a.rownan + r, 1);
+      }
+    }
+    sidx += G;
+    if (sidx >= a.n_sub) break;
+  }
+}
+
+// ---- Half sub-rows of the long rows (FP32 split variant 5) ------------------------------------
+// P = R0 x 16368 = R x 8184 with R = 2 R0.  With n = n1 + 8184 n0 and k = k0 + R k1,
+//   Z[k0 + R k1] = FFT_8184( W_P^{n1 k0} sum_{n0 < R} z[n1 + 8184 n0] W_R^{n0 k0} )[k1],
+// so a long row becomes R sub-rows of 8184 points: 65 KB of f32 complex values, so that two
+// transforming workgroups share one CU's LDS and overlap their passes (a 16368-point sub-row
+// takes the whole of it, and its workgroup's barriers leave the SIMDs idle).
+// 8184 = 33 x 8 x 31 with pairwise coprime factors.  A half sub-row is stored in the CRT order:
+// position s = 248 c1 + 31 c2 + c3 holds natural index m with m mod 33 = c1, m mod 8 = c2,
+// m mod 31 = c3, i.e. m = (496 c1 + 7161 c2 + 528 c3) mod 8184, and output (k1, k2, k3) of the
+// three passes (radix 33 over c1, 8 over c2, 31 over c3) is sample (248 k1 + 1023 k2 + 264 k3)
+// mod 8184 (Good-Thomas: no twiddles between the passes).  The spectra keep the 16368-segment
+// prime-factor order of the other split forms, where natural n1 + 8184 b of a segment sits at
+// 496 c1 + 31 ((-n1 - 8 b) mod 16) + c3: the two values of half position s in every segment are
+// the pair lo = 496 c1 + 31 ((-c2) mod 8) + c3 and lo + 248, b = 0 being lo exactly when
+// (-n1) mod 16 < 8, so the pre-pass reads runs of 31 contiguous values.
+constexpr int kHalfS = kFusedP / 2;  // 8184
+constexpr int TH = 256;
+constexpr int NWH = TH / 64;
+
+__device__ __forceinline__ int half_nat(int s) {
+  const int c1 = s / 248, b = s - 248 * c1, c2 = b / 31, c3 = b - 31 * c2;
+  return (496 * c1 + 7161 * c2 + 528 * c3) % kHalfS;
+}
+// sample of output k3 = 0 of pass-C butterfly q = 33 k2 + k1 (add 264 k3, wrap once)
+__device__ __forceinline__ int half_base(int q) {
+  const int k2 = q / 33, k1 = q - 33 * k2;
+  return (248 * k1 + 1023 * k2) % kHalfS;
+}
+__device__ __forceinline__ int half_wrap(int k) { return k >= kHalfS ? k - kHalfS : k; }
+// sample of the larger output of pair (butterfly q, row kk) with d = p1 - p2 (c31_pair_sample)
+__device__ __forceinline__ int half_pair_sample(int q, int kk, float d) {
+  const int i1 = half_wrap(half_base(q) + (kHalfS / 31) * kk);
+  const int i2 = kk > 0 ? half_wrap(half_base(q) + (kHalfS / 31) * (31 - kk)) : i1;
+  return d > 0.0f ? i1 : d < 0.0f ? i2 : max(i1, i2);
+}
+
+// Pre-pass of a chunk of rows into half sub-rows: 32 blocks of 256 positions per row (the XCD
+// dealing of k_big_pre); per position s, conj(X) R at the R = 2 R0 natural points m + 8184 n0
+// (the lo / lo + 248 pair of every segment), an R-point DFT, the twiddles W_P^{m k0} from two
+// table values W_P^m, W_P^{R0 m} (at most R0 products each: relative error below 1e-6), and R
+// non-temporal stores into the row's R sub-rows; position 0 clears the row's key and records it
+// in the chunk's row map.
+template <int R0>
+__global__ __launch_bounds__(256) void k_half_pre(const float2* __restrict__ X,
+                                                  const float2* __restrict__ rep,
+                                                  const float2* __restrict__ twh, int n_prn,
+                                                  int n_bin, int n_ant, int hs, int tile,
+                                                  long long r0, long long n_rows,
+                                                  float2* __restrict__ Y,
+                                                  unsigned long long* __restrict__ rowkey,
+                                                  int* __restrict__ rownan,
+                                                  int* __restrict__ rowmap) {
+  constexpr int R = 2 * R0;
+  constexpr long long PL = static_cast<long long>(R0) * kFusedP;
+  const long long rr = blockIdx.x >> 5;
+  if (rr >= n_rows) return;
+  const int chunk = 4 * static_cast<int>(blockIdx.x & 7) + static_cast<int>((blockIdx.x >> 3) & 3);
+  const int s = 256 * chunk + static_cast<int>(threadIdx.x);
+  if (s >= kHalfS) return;
+  const long long I = r0 + rr;
+  const BigItem it = big_item(I, n_prn, n_bin, n_ant, hs, tile);
+  const int h = static_cast<int>(it.xrow & 1);
+  const float2* xr = X + (hs == 1 ? it.xrow : it.xrow >> 1) * PL;
+  const float2* cr = rep + (static_cast<long long>(it.prn) * 2 + h) * PL;
+  const int c1 = s / 248, b = s - 248 * c1, c2 = b / 31, c3 = b - 31 * c2;
+  const int m = (496 * c1 + 7161 * c2 + 528 * c3) % kHalfS;
+  const int lo = 496 * c1 + 31 * ((8 - c2) & 7) + c3;
+  const bool swap = ((-m) & 15) >= 8;  // natural m of each segment sits at lo + 248
+  float2 v[R];
+#pragma unroll
+  for (int g = 0; g < R0; ++g) {
+    const float2 pl = cconjmul(xr[kFusedP * g + lo], cr[kFusedP * g + lo]);
+    const float2 ph = cconjmul(xr[kFusedP * g + lo + 248], cr[kFusedP * g + lo + 248]);
+    v[2 * g] = swap ? ph : pl;
+    v[2 * g + 1] = swap ? pl : ph;
+  }
+  Dft<R>::run(v);
+  const float2 w1 = twh[s], wh = twh[kHalfS + s];  // W_P^m, W_P^{R0 m}
+  v[R0] = cmul(v[R0], wh);
+  float2 wk = w1;
+#pragma unroll
+  for (int k0 = 1; k0 < R0; ++k0) {
+    v[k0] = cmul(v[k0], wk);
+    v[k0 + R0] = cmul(v[k0 + R0], cmul(wk, wh));
+    if (k0 + 1 < R0) wk = cmul(wk, w1);
+  }
+  float* y = reinterpret_cast<float*>(Y + rr * PL + s);
+#pragma unroll
+  for (int k0 = 0; k0 < R; ++k0) {
+    __builtin_nontemporal_store(v[k0].x, y + 2 * kHalfS * k0);
+    __builtin_nontemporal_store(v[k0].y, y + 2 * kHalfS * k0 + 1);
+  }
+  if (s == 0) {
+    const long long r = big_row(I, n_prn, n_bin, n_ant, hs, tile);
+    rowkey[r] = 0ull;
+    rownan[r] = 0;
+    rowmap[rr] = static_cast<int>(r);
+  }
+}
+
+// Transform of the half sub-rows (the passes of k_subrows on 8184 points): 256 threads, two
+// workgroups per CU within 256 VGPRs and 68 KB of LDS each.
+//   pass A  radix 33 (codelet), 248 butterflies ta = 31 c2 + c3 from the prefetched registers
+//           (inputs c1 at ta + 248 c1) -> LDS 33 ta + k1 = 1023 c2 + 33 c3 + k1
+//   pass B  radix 8 in place, butterflies j = 33 c3 + k1 < 1023 over c2 at j + 1023 c2
+//   pass C  radix 31 on the matrix cores (C31Lane), butterflies q = 33 k2 + k1 < 264 at
+//           c31_pos(q) + 33 c3: tiles w + 4 ts of wave w, and tile 16 (butterflies 256..263 in
+//           lanes 0..7; its other 8 columns are ignored) on wave 0
+// Each lane keeps its best output pair; an exact rescan runs only on ties (as k_subrows); one
+// atomic maximum per sub-row folds (power, k0 + R sample) into the row's key.
+template <int R, bool DUMP = false>
+__global__ __launch_bounds__(TH, 2) void k_halfrows(RowArgs a) {
+  constexpr int S = kHalfS;
+  constexpr int NBA = S / 33;  // 248
+  constexpr int NBB = S / 8;   // 1023
+  constexpr int NBC = S / 31;  // 264
+  __shared__ float2 buf[S];
+  __shared__ float2 stw[256];
+  __shared__ RowRed<NWH> red;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < 256; i += TH) stw[i] = a.tw[i];
+  __syncthreads();
+  const int wave = tid >> 6, lane = tid & 63, ln16 = lane & 15, grp = lane >> 4;
+  C31Lane c31;
+  c31.init(stw, ln16, grp);
+  // lanes beyond the 248 pass-A butterflies load past the end of the view: 0, no branch
+  const int voa = tid < NBA ? tid * 8 : S * 8;
+  auto load_sub = [&](long long s, float2 (&y)[33]) {
+    const __amdgpu_buffer_rsrc_t yv = buffer_view(a.Y + s * S, S * 8);
+#pragma unroll
+    for (int r = 0; r < 33; ++r) y[r] = buffer_load_f2(yv, voa, r * NBA * 8);
+  };
+  auto c_tile = [&](int ts) { return ts < 4 ? wave + 4 * ts : 16; };
+  const long long G = gridDim.x;
+  long long sidx = blockIdx.x;
+  if (sidx >= a.n_sub) return;
+  // the sub-row's row index (from the pre-pass's map) and inputs, loaded one sub-row ahead
+  int rmap = a.rowmap[sidx / R];
+  float2 cur[33];
+  load_sub(sidx, cur);
+  for (;;) {
+    const long long s = sidx;
+    const long long rq = s / R;
+    const int k0 = static_cast<int>(s - rq * R);
+    const long long r = __builtin_amdgcn_readfirstlane(rmap);
+    // ---- pass A (radix 33) -> LDS; the next sub-row's inputs in flight
+    Dft<33>::run(cur);
+    {
+      int ta = tid;
+      asm volatile("" : "+v"(ta));
+      if (ta < NBA) {
+#pragma unroll
+        for (int q = 0; q < 33; ++q) buf[ta * 33 + q] = cur[q];
+      }
+    }
+    if (s + G < a.n_sub) {
+      rmap = a.rowmap[(s + G) / R];
+      load_sub(s + G, cur);
+    }
+    lds_barrier();
+    // ---- pass B (radix 8) in place, butterflies tid + 256 i
+    {
+      int tb = tid;
+      asm volatile("" : "+v"(tb));
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int j = tb + i * TH;
+        if (i < 3 || j < NBB) {
+          float2 v[8];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) v[q] = buf[j + q * NBB];
+          Dft<8>::run(v);
+#pragma unroll
+          for (int q = 0; q < 8; ++q) buf[j + q * NBB] = v[q];
+        }
+      }
+    }
+    lds_barrier();
+    // ---- pass C (radix 31, matrix cores): each lane's best pair as in k_subrows; the columns
+    // of tile 16 beyond butterfly 263 take no part
+    unsigned int pmx = 0u;
+    int psl = 0;
+    float pdv = 0.0f;
+    unsigned long long tmask = 0ull;
+    auto dump_at = [&](int idx, float p) {
+      if constexpr (DUMP) a.dump[static_cast<size_t>(r) * (R * S) + k0 + R * idx] = p;
+    };
+    auto tile = [&](int ts) {
+      floatx4 acc[4];
+      const int q = 16 * c_tile(ts) + ln16;
+      const bool valid = q < NBC;
+      c31.tile(buf, valid ? q : NBC - 1, acc);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        float p1, p2;
+        c31_pair(acc, e, p1, p2);
+        const unsigned int bits = max(__float_as_uint(p1), __float_as_uint(p2));
+        const float d = p1 - p2;
+        const bool up = valid && bits > pmx;
+        tmask = (tmask | __ballot(valid && bits == pmx)) & ~__ballot(up);
+        pmx = up ? bits : pmx;
+        psl = up ? 4 * ts + e : psl;
+        pdv = up ? d : pdv;
+        if constexpr (DUMP) {
+          if (valid) {
+            const int kk = 4 * grp + e;
+            dump_at(half_wrap(half_base(q) + NBC * kk), p1);
+            if (kk > 0) dump_at(half_wrap(half_base(q) + NBC * (31 - kk)), p2);
+          }
+        }
+      }
+    };
+    tile(0);
+    tile(1);
+    tile(2);
+    tile(3);
+    if (wave == 0) tile(4);
+    // ---- end of the sub-row: block maximum, the waves holding it fold (M, largest sample with
+    // power M) into the row's key (the barrier also orders this sub-row's pass-C loads before
+    // the next sub-row's pass-A stores)
+    const unsigned int wmax = wave_umax(pmx);
+    const bool wtie = (__ballot(pmx == wmax) & tmask) != 0ull;
+    if (lane == 0) {
+      red.pk[wave] = wmax;
+      red.bad[wave] = wtie ? 1 : 0;
+    }
+    lds_barrier();
+    const bool mine = lane < NWH;
+    const unsigned int M = static_cast<unsigned int>(__builtin_amdgcn_readlane(
+        static_cast<int>(row16_umax(mine ? red.pk[lane] : 0u)), 0));
+    const bool tie = __ballot(mine && red.pk[mine ? lane : 0] == M &&
+                              red.bad[mine ? lane : 0] != 0) != 0ull;
+    unsigned int cand = 0u;  // 1 + sample, 0 for none
+    if (tie) {
+      // rare (block-uniform): a lane holding M reached it in more than one pair; the waves
+      // holding M recompute their tiles from the pass-B values still in LDS
+      if (wmax == M) {
+        int l16 = ln16;
+        asm volatile("" : "+v"(l16));
+        auto rescan = [&](int ts) {
+          floatx4 acc[4];
+          const int q = 16 * c_tile(ts) + l16;
+          const bool valid = q < NBC;
+          c31.tile(buf, valid ? q : NBC - 1, acc);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            float p1, p2;
+            c31_pair(acc, e, p1, p2);
+            if (valid && max(__float_as_uint(p1), __float_as_uint(p2)) == M)
+              cand = max(cand,
+                         static_cast<unsigned int>(half_pair_sample(q, 4 * grp + e, p1 - p2) + 1));
+          }
+        };
+        rescan(0);
+        rescan(1);
+        rescan(2);
+        rescan(3);
+        if (wave == 0) rescan(4);
+      }
+      lds_barrier();
+    } else if (pmx == M) {
+      int l16 = ln16;
+      asm volatile("" : "+v"(l16));
+      cand = static_cast<unsigned int>(
+          half_pair_sample(16 * c_tile(psl >> 2) + l16, 4 * grp + (psl & 3), pdv) + 1);
+    }
+    if (wmax == M) {
+      const unsigned int iw = wave_umax(cand);
+      if (lane == 0) {
+        atomicMax(a.rowkey + r,
+                  (static_cast<unsigned long long>(M) << 32) |
+                      static_cast<unsigned int>(k0 + R * (static_cast<int>(iw) - 1)));
         if (M > 0x7f800000u) atomicOr(a.rownan + r, 1);
       }
     }
@@ -3229,6 +3514,77 @@ void launch_big_pre(const float2* X, const float2* rep, const float2* twb, int R
                        n_ant, hs, tile, r0, total, Y, rowkey, rownan, rowmap);
   else
     fail(TGA_ERR_UNSUPPORTED, "long-period split supports 4 and 10 sub-rows");
+  TGA_HIP(hipGetLastError());
+}
+
+std::vector<float2> half_twiddles(size_t P) {
+  const int R0 = static_cast<int>(P / kFusedP);
+  std::vector<float2> t(2 * static_cast<size_t>(fused::kHalfS));
+  for (int s = 0; s < fused::kHalfS; ++s) {
+    // natural index of position s of a half sub-row (fused::half_nat)
+    const int c1 = s / 248, b = s - 248 * c1, c2 = b / 31, c3 = b - 31 * c2;
+    const long long m = (496LL * c1 + 7161LL * c2 + 528LL * c3) % fused::kHalfS;
+    for (int k = 0; k < 2; ++k) {
+      const long long e = (k == 0 ? m : m * R0) % static_cast<long long>(P);
+      const double a = -2.0 * kPiD * static_cast<double>(e) / static_cast<double>(P);
+      t[k * fused::kHalfS + s] =
+          make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+    }
+  }
+  return t;
+}
+
+void launch_half_pre(const float2* X, const float2* rep, const float2* twh, int R0, int n_prn,
+                     int n_bin, int n_ant, int hs, int tile, long long r0, long long r1,
+                     float2* Y, unsigned long long* rowkey, int* rownan, int* rowmap,
+                     hipStream_t st) {
+  const long long nr = r1 - r0;
+  const dim3 grid(static_cast<unsigned>(nr * 32));
+  if (R0 == 4)
+    hipLaunchKernelGGL((fused::k_half_pre<4>), grid, dim3(256), 0, st, X, rep, twh, n_prn, n_bin,
+                       n_ant, hs, tile, r0, nr, Y, rowkey, rownan, rowmap);
+  else if (R0 == 10)
+    hipLaunchKernelGGL((fused::k_half_pre<10>), grid, dim3(256), 0, st, X, rep, twh, n_prn,
+                       n_bin, n_ant, hs, tile, r0, nr, Y, rowkey, rownan, rowmap);
+  else
+    fail(TGA_ERR_UNSUPPORTED, "long-period split supports 4 and 10 sub-rows");
+  TGA_HIP(hipGetLastError());
+}
+
+void launch_half_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, int hs, int tile,
+                     long long r0, long long r1, const float2* Y, unsigned long long* rowkey,
+                     int* rownan, const int* rowmap, RowOut* rows, hipStream_t st,
+                     float* dump) {
+  const long long nr = r1 - r0;
+  fused::RowArgs a{};
+  a.hs = hs;
+  a.n_ant = n_ant;
+  a.tile = tile;
+  a.tw = tw;
+  a.n_prn = n_prn;
+  a.n_bin = n_bin;
+  a.Y = Y;
+  a.n_sub = nr * 2 * R0;
+  a.r0 = r0;
+  a.rowkey = rowkey;
+  a.rownan = rownan;
+  a.rowmap = rowmap;
+  a.dump = dump;
+  const unsigned blocks = static_cast<unsigned>(a.n_sub < 4096 ? a.n_sub : 4096);
+  const dim3 grid(blocks), block(fused::TH);
+  if (R0 == 4 && dump)
+    hipLaunchKernelGGL((fused::k_halfrows<8, true>), grid, block, 0, st, a);
+  else if (R0 == 4)
+    hipLaunchKernelGGL((fused::k_halfrows<8>), grid, block, 0, st, a);
+  else if (R0 == 10 && dump)
+    hipLaunchKernelGGL((fused::k_halfrows<20, true>), grid, block, 0, st, a);
+  else if (R0 == 10)
+    hipLaunchKernelGGL((fused::k_halfrows<20>), grid, block, 0, st, a);
+  else
+    fail(TGA_ERR_UNSUPPORTED, "long-period split supports 4 and 10 sub-rows");
+  TGA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_big_rowout, dim3(grid_for(nr, 256)), dim3(256), 0, st, rowkey, rownan,
+                     r0, nr, n_prn, n_bin, n_ant, hs, tile, rows);
   TGA_HIP(hipGetLastError());
 }
 

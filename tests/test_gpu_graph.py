@@ -44,6 +44,12 @@ CASES = {
                              "fp32", "split10x16368"),
     "subrows10": (FS, 2 * 163680, 1, 2, [("l1c", [5, 9], grid(IF, 500.0, 500.0), 0)],
                   "fp32", "split10x16368"),
+    "half4": (FS, 2 * 65472, 1, 1, [("galileo", [3, 11], grid(IF, 500.0, 500.0), 0)],
+              "fp32", "split4x16368"),
+    "half10_unfolded": (FS, 163680, 1, 1, [("beidou", [30], grid(IF, 500.0, 500.0), 0)],
+                        "fp32", "split10x16368"),
+    "half10": (FS, 2 * 163680, 1, 2, [("l1c", [5, 9], grid(IF, 500.0, 500.0), 0)],
+               "fp32", "split10x16368"),
     "fourstep4": (FS, 2 * 65472, 1, 1, [("galileo", [3, 11], grid(IF, 500.0, 500.0), 0)],
                   "fp32", "split132x496"),
     "fourstep10_unfolded": (FS, 163680, 1, 1, [("beidou", [30], grid(IF, 500.0, 500.0), 0)],
@@ -67,11 +73,11 @@ def signals_for(systems, fs, seed):
     return out
 
 
-# the split cases pin their FP32 form: the sub-row transforms of k_rows (matrix-core radix 31)
-# or k_subrows, or the four-step form
+# the split cases pin their FP32 form: the sub-row transforms of k_rows (matrix-core radix 31),
+# k_subrows, the half sub-rows of k_halfrows, or the four-step form
 SPLIT_FORM = {"split4": "1", "split10_unfolded": "1", "subrows4": "2",
-              "subrows10_unfolded": "2", "subrows10": "2", "fourstep4": "3",
-              "fourstep10_unfolded": "3"}
+              "subrows10_unfolded": "2", "subrows10": "2", "half4": "5", "half10_unfolded": "5",
+              "half10": "5", "fourstep4": "3", "fourstep10_unfolded": "3"}
 
 
 def make_plan(case):

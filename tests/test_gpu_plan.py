@@ -59,12 +59,14 @@ def fused_variant(request, monkeypatch):
     return request.param
 
 
-@pytest.fixture(params=[0, 1, 2, 3],
-                ids=["split-valu31", "split-mfma31", "split-subrows", "split-fourstep"])
+@pytest.fixture(params=[0, 1, 2, 3, 5],
+                ids=["split-valu31", "split-mfma31", "split-subrows", "split-fourstep",
+                     "split-half"])
 def split_variant(request, monkeypatch):
     """Every FP32 form of the long rows: R0 sub-rows of 16368 with either radix-31
     implementation (k_rows), the sub-row transform that keeps each lane's best output pair
-    (k_subrows), and the four-step N1 x 496 form."""
+    (k_subrows), 2 R0 half sub-rows of 8184 points (k_half_pre + k_halfrows), and the
+    four-step N1 x 496 form."""
     monkeypatch.setenv("TGA_SPLIT_VARIANT", str(request.param))
     return request.param
 
@@ -262,7 +264,7 @@ def test_split_tile_shapes_give_identical_peaks(gpu, monkeypatch):
     monkeypatch.delenv("TGA_SPLIT_VARIANT")
     desc = tga.Plan(fs, 2, L, [tga.SystemSearch(n, f, p) for n, p, f in systems]).describe()
     for c in ("v2t8x8:", "v2t32x2:", "v1t8x8:", "v1t4x16:", "v0t16x4:", "v0t2x32:", "v1t32x2:",
-              "v3t8x8:", "v3t4x16:"):
+              "v3t8x8:", "v3t4x16:", "v5t8x8:", "v5t32x2:"):
         assert c in desc, desc
     assert "v3t2x32:" not in desc  # the four-step form is timed on two shapes
 

@@ -127,3 +127,81 @@ def test_long_row_tiles_enumerate_every_row_once():
         assert len(rows) == n_ant * n_bin * 2 * n_prn // hs
         assert all(0 <= x < n_ant * n_bin * 2 and 0 <= p < n_prn for x, p in rows)
         assert hs == 1 or all(x % 2 == 0 for x, _ in rows)
+
+
+def _half_split_dft(z, R0):
+    """The FP32 split's half sub-rows (k_half_pre + k_halfrows in search.hip) in numpy, with the
+    kernels' index formulas: z (natural order, P = R0 x 16368) stored in the 16368-segment
+    prime-factor order, the pre-pass gather of the (lo, lo + 248) pair per segment, the R-point
+    DFT and the twiddles from the two table values, then the 8184-point transform's three passes
+    with their LDS positions and output map.  Returns the P-point DFT of z."""
+    S, R = 8184, 2 * R0
+    Pl = R0 * P
+    stored = np.empty(Pl, complex)
+    s16 = np.arange(P)
+    for g in range(R0):
+        stored[g * P + s16] = z[g * P + np.array([pfa_nat(int(s)) for s in s16])]
+    s8 = np.arange(S)
+    c1, b = np.divmod(s8, 248)
+    c2, c3 = np.divmod(b, 31)
+    m = (496 * c1 + 7161 * c2 + 528 * c3) % S
+    lo = 496 * c1 + 31 * ((8 - c2) & 7) + c3
+    swap = ((-m) & 15) >= 8
+    v = np.empty((R, S), complex)
+    for g in range(R0):
+        pl, ph = stored[g * P + lo], stored[g * P + lo + 248]
+        v[2 * g] = np.where(swap, ph, pl)
+        v[2 * g + 1] = np.where(swap, pl, ph)
+    v = np.fft.fft(v, axis=0)  # R-point DFT over n0
+    w1 = np.exp(-2j * np.pi * m / Pl)
+    wh = np.exp(-2j * np.pi * (m * R0 % Pl) / Pl)
+    wk = w1.copy()
+    v[R0] *= wh
+    for k0 in range(1, R0):
+        v[k0] *= wk
+        v[k0 + R0] *= wk * wh
+        wk = wk * w1
+    Z = np.empty(Pl, complex)
+    ta = np.arange(248)
+    j = np.arange(1023)
+    q = np.arange(264)
+    for k0 in range(R):
+        y = v[k0]
+        buf = np.empty(S, complex)
+        a = np.fft.fft(y[ta[:, None] + 248 * np.arange(33)[None, :]], axis=1)  # pass A
+        buf[(33 * ta)[:, None] + np.arange(33)[None, :]] = a
+        bb = np.fft.fft(buf[j[:, None] + 1023 * np.arange(8)[None, :]], axis=1)  # pass B
+        buf[j[:, None] + 1023 * np.arange(8)[None, :]] = bb
+        base = q + 990 * (q // 33)  # c31_pos
+        cc = np.fft.fft(buf[base[:, None] + 33 * np.arange(31)[None, :]], axis=1)  # pass C
+        k2, k1 = np.divmod(q, 33)
+        samp = (248 * k1[:, None] + 1023 * k2[:, None] + 264 * np.arange(31)[None, :]) % S
+        Z[k0 + R * samp] = cc
+    return Z
+
+
+def test_half_subrows_equal_the_long_dft():
+    # R0 = 4 (E1-C, 8 half sub-rows) and 10 (B1C / L1C, 20 half sub-rows)
+    rng = np.random.default_rng(7)
+    for R0 in (4, 10):
+        z = rng.standard_normal(R0 * P) + 1j * rng.standard_normal(R0 * P)
+        Z = _half_split_dft(z, R0)
+        ref = np.fft.fft(z)
+        assert np.max(np.abs(Z - ref)) < 1e-9 * np.max(np.abs(ref))
+
+
+def test_half_subrow_maps_are_bijections():
+    S = 8184
+    s8 = np.arange(S)
+    c1, b = np.divmod(s8, 248)
+    c2, c3 = np.divmod(b, 31)
+    m = (496 * c1 + 7161 * c2 + 528 * c3) % S
+    assert np.array_equal(np.sort(m), s8)
+    assert np.array_equal(m % 33, c1) and np.array_equal(m % 8, c2) and np.array_equal(m % 31, c3)
+    q = np.arange(264)
+    k2, k1 = np.divmod(q, 33)
+    samp = (248 * k1[:, None] + 1023 * k2[:, None] + 264 * np.arange(31)[None, :]) % S
+    assert np.array_equal(np.sort(samp.ravel()), s8)
+    # the pre-pass's (lo, lo + 248) pairs cover every position of a 16368 segment once
+    lo = 496 * c1 + 31 * ((8 - c2) & 7) + c3
+    assert np.array_equal(np.sort(np.concatenate([lo, lo + 248])), np.arange(P))
