@@ -658,6 +658,53 @@ static float select_gain(const Group& g) {
   return fold / (g.fp16 ? kF16Scale * kF16Scale : g.big16 ? kSplitF16Scale : 1.0f);
 }
 
+// Rows of a sub-row split group, chunk by chunk on st: the pre-pass writes the chunk's sub-rows,
+// the transform reduces them to the rows' keys and RowOut.  (Running the next chunk's pre-pass
+// on a second stream into a second buffer beside this chunk's transform was measured slower in
+// every form: the two kernels share the CUs and slow each other down.)
+void Plan::split_rows(Group& g, hipStream_t st, float* dump) {
+  const int hs = g.folded ? 1 : 2;
+  const long long n_enum = g.n_rows / hs;
+  const bool half = g.rows_cfg.variant == 5 && !g.big16;
+  float* dmp = dump ? dump + g.dump_off : nullptr;
+  auto pre = [&](long long r0, long long r1, hipStream_t s) {
+    void* Y = g.d_Y.p;
+    int* rowmap = g.d_rowmap.as<int>();
+    if (half)
+      launch_half_pre(g.X, g.d_rep.as<float2>(), g.d_twh.as<float2>(), g.big_r0, g.n_prn,
+                      g.n_bin, n_ant, hs, g.rows_cfg.tile, r0, r1, static_cast<float2*>(Y),
+                      g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(), rowmap, s);
+    else
+      launch_big_pre(g.X, g.d_rep.as<float2>(), g.d_twb.as<float2>(), g.big_r0, g.n_prn,
+                     g.n_bin, n_ant, hs, g.rows_cfg.tile, r0, r1, Y, g.big16,
+                     g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(), rowmap, s);
+  };
+  auto sub = [&](long long r0, long long r1, hipStream_t s) {
+    void* Y = g.d_Y.p;
+    const int* rowmap = g.d_rowmap.as<int>();
+    if (half)
+      launch_half_sub(g.d_tw.as<float2>(), g.big_r0, g.n_prn, g.n_bin, n_ant, hs,
+                      g.rows_cfg.tile, r0, r1, static_cast<const float2*>(Y),
+                      g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(), rowmap,
+                      g.d_rows.as<RowOut>(), s, dmp);
+    else
+      launch_big_sub(g.d_tw.as<float2>(), g.big_r0, g.n_prn, g.n_bin, n_ant, hs,
+                     g.rows_cfg.tile, r0, r1, Y, g.big16, g.d_rowkey.as<unsigned long long>(),
+                     g.d_rownan.as<int>(), rowmap, g.d_rows.as<RowOut>(), s,
+                     g.rows_cfg.variant, dmp);
+  };
+  const long long n_chunks = (n_enum + g.chunk_rows - 1) / g.chunk_rows;
+  auto bounds = [&](long long c) {
+    return std::pair<long long, long long>{c * g.chunk_rows,
+                                           std::min(n_enum, (c + 1) * g.chunk_rows)};
+  };
+  for (long long c = 0; c < n_chunks; ++c) {
+    const auto [r0, r1] = bounds(c);
+    pre(r0, r1, st);
+    sub(r0, r1, st);
+  }
+}
+
 void Plan::search(const uint8_t* d_in, hipStream_t st, float* dump) {
   if (st == nullptr) st = stream;
   launch_ant_sums(d_in, row_bytes, n_ant, L, fmt, d_partial.as<long long>(),
@@ -736,30 +783,7 @@ void Plan::search(const uint8_t* d_in, hipStream_t st, float* dump) {
                         g.d_rows.as<RowOut>(), st, dump ? dump + g.dump_off : nullptr);
         continue;
       }
-      for (long long r0 = 0; r0 < n_enum; r0 += g.chunk_rows) {
-        const long long r1 = std::min(n_enum, r0 + g.chunk_rows);
-        if (g.rows_cfg.variant == 5 && !g.big16) {
-          launch_half_pre(g.X, g.d_rep.as<float2>(), g.d_twh.as<float2>(), g.big_r0, g.n_prn,
-                          g.n_bin, n_ant, hs, g.rows_cfg.tile, r0, r1, g.d_Y.as<float2>(),
-                          g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(),
-                          g.d_rowmap.as<int>(), st);
-          launch_half_sub(g.d_tw.as<float2>(), g.big_r0, g.n_prn, g.n_bin, n_ant, hs,
-                          g.rows_cfg.tile, r0, r1, g.d_Y.as<float2>(),
-                          g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(),
-                          g.d_rowmap.as<int>(), g.d_rows.as<RowOut>(), st,
-                          dump ? dump + g.dump_off : nullptr);
-          continue;
-        }
-        launch_big_pre(g.X, g.d_rep.as<float2>(), g.d_twb.as<float2>(), g.big_r0, g.n_prn,
-                       g.n_bin, n_ant, hs, g.rows_cfg.tile, r0, r1, g.d_Y.p, g.big16,
-                       g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(),
-                       g.d_rowmap.as<int>(), st);
-        launch_big_sub(g.d_tw.as<float2>(), g.big_r0, g.n_prn, g.n_bin, n_ant, hs,
-                       g.rows_cfg.tile, r0, r1,
-                       g.d_Y.p, g.big16, g.d_rowkey.as<unsigned long long>(),
-                       g.d_rownan.as<int>(), g.d_rowmap.as<int>(), g.d_rows.as<RowOut>(), st,
-                       g.rows_cfg.variant, dump ? dump + g.dump_off : nullptr);
-      }
+      split_rows(g, st, dump);
     } else {
       const FftPlan* fp = fft_plan(g.M);
       for (long long r0 = 0; r0 < g.n_rows; r0 += g.chunk_rows) {

@@ -115,6 +115,8 @@ struct Plan {
   void autotune_split(Group& g);
   void rep_to_pfa(Group& g, long long n);
   void alloc_winners(Group& g);
+  // the rows of a sub-row split group (variants 0, 1, 2, 5), chunk by chunk, on st
+  void split_rows(Group& g, hipStream_t st, float* dump);
 };
 
 // Error-to-status wrapper shared by the C ABI translation units.
