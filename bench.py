@@ -9,7 +9,9 @@ SBAS + QZSS L1 C/A 38 + 39 + 23 PRNs x 41 bins (+-10 kHz, 500 Hz), E1-C 50 SVs x
 form, two hypotheses): 1.0746e11 cells per search, FP32, plus ACR C/N0 on the host.  The FP16
 matrix-core form (f16 spectra or f16 sub-rows, radix passes as f16 GEMMs, f64 carrier phase)
 then runs the same observation and is reported under extra.fp16_mfma with its difference to
-the FP32 peaks.
+the FP32 peaks.  extra.mfma (and extra.fp16_mfma.mfma) gives the matrix cores' achieved rate in
+the rows stage: the plan's MFMA FLOP per search over the stage's mean time, against the dense
+f32 / f16 peaks.
 
 Other workloads (same JSON line):
   ca24     GPS + SBAS + QZSS L1 C/A only (the C/A part of all24 at 10 ms: 1.611e9 cells).
@@ -50,6 +52,22 @@ METRIC = "search cells/s (ant×PRN×Doppler×phase): FP16-MFMA vs FP32 GPU vs CP
 FS = 16.368e6
 IF = 4.092e6
 HBM_PEAK_GBS = 8000.0
+# MI355X dense matrix-core peaks: f32 (v_mfma_f32_16x16x4f32) and f16 (v_mfma_f32_16x16x32_f16)
+MFMA_PEAK_TFLOPS = {"f32": 157.3, "f16": 2500.0}
+
+
+def mfma_block(plan, stage_s):
+    """Matrix-core rate of the rows stage: the plan's MFMA FLOP per search (counted from the
+    kernels its groups run) over the stage's measured mean time, per MFMA type."""
+    f32, f16 = plan.rows_matrix_flops()
+    out = {}
+    for kind, flops in (("f32", f32), ("f16", f16)):
+        if flops > 0:
+            rate = flops / stage_s / 1e12
+            out[kind] = {"flops_per_launch": flops, "achieved_tflops": rate,
+                         "peak_tflops": MFMA_PEAK_TFLOPS[kind],
+                         "frac": rate / MFMA_PEAK_TFLOPS[kind]}
+    return out
 PERIOD_S = {"gps": 0.001, "sbas": 0.001, "qzss": 0.001, "galileo": 0.004, "beidou": 0.010,
             "l1c": 0.010}
 PRNS = {"gps": range(1, 39), "sbas": range(120, 159), "qzss": range(184, 207),
@@ -396,6 +414,7 @@ def main():
                 "best_peak_rel_diff_vs_fp32": {"max": float(rel.max()),
                                                "median": float(np.median(rel))},
                 "same_bin_and_codephase_as_fp32": f"{same}/{rel.size}",
+                "mfma": mfma_block(plan16, avg16),
                 "plan": plan16.describe().splitlines()[0]}
 
     surface = None
@@ -487,6 +506,7 @@ def main():
                      "alg_bytes_per_launch": rows_bytes},
         "cpu_baseline": cpu,
         "extra": {"forward_ms": fwd_ms / max(fwd_n, 1), "select_ms": sel_ms / max(sel_n, 1),
+                  "mfma": mfma_block(plan, avg_rows_s),
                   "plan_create_s": t_plan, "synth_s": t_synth, "cn0_valid": n_cn0,
                   "injected_found": f"{found}/{len(signals)}", "rfi": rfi,
                   "observations_per_s": args.steps / dt if stream else None,

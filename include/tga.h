@@ -253,6 +253,13 @@ int tga_plan_set_graph(tga_plan* plan, int32_t enable);
 int tga_plan_set_timing(tga_plan* plan, int32_t enable);
 int tga_plan_get_timing(tga_plan* plan, int32_t kernel, double* total_ms, int64_t* launches,
                         double* bytes_per_launch);
+/* Matrix-core arithmetic of one search's correlation rows (timing slot 0), counted from the
+ * kernels each execution group runs after autotuning: f32_flops for v_mfma_f32_16x16x4f32
+ * (2048 FLOP each: the radix-31 pass of the FP32 rows and sub-rows, the four-step rows) and
+ * f16_flops for v_mfma_f32_16x16x32_f16 (16384 FLOP each: every pass of the FP16 form).  Divided
+ * by the slot's time it gives the matrix cores' achieved rate in the DFT-as-GEMM stages.
+ * No counterpart in the reference (a measurement aid). */
+int tga_plan_rows_matrix_flops(const tga_plan* plan, double* f32_flops, double* f16_flops);
 /* Short names of the kernels behind a timing slot, as rocprof lists them, joined by '+' over
  * the plan's execution groups (e.g. "k_rows+k_big_pre<4>+k_rows<0,4>" for a C/A group beside
  * an E1-C split group); "" for a slot out of range. */

@@ -882,6 +882,38 @@ void Plan::finish(tga_peak* out, hipStream_t st) {
                           : "called `Option::unwrap()` on a `None` value (NaN in the correlation)");
 }
 
+// Matrix-core instructions per transform, from the kernels' tile loops: the FP32 radix-31 pass
+// is 16 v_mfma_f32_16x16x4f32 per tile of 16 butterflies (33 tiles per 16368 points, 17 per 8184
+// including the half tile, 1 per 496-point four-step row); the FP16 rows kernel runs radix 33 as
+// 31 tiles x 5 x 3, radix 16 as 64 tiles x 2 and radix 31 as 33 tiles x 8
+// v_mfma_f32_16x16x32_f16.  The non-coherent form and the generic engine use no matrix cores.
+void Plan::rows_matrix_flops(double& f32, double& f16) const {
+  constexpr double kF32 = 2.0 * 16 * 16 * 4, kF16 = 2.0 * 16 * 16 * 32;
+  constexpr double kRows31 = 33 * 16, kHalf31 = 17 * 16, kFourStep31 = 16;
+  constexpr double kRows16 = 31 * 15 + 64 * 2 + 33 * 8;
+  f32 = f16 = 0.0;
+  for (const auto& g : groups) {
+    if (g.nc > 0) continue;
+    const double rows = static_cast<double>(g.n_rows);
+    if (g.fp16) {
+      f16 += rows * kRows16 * kF16;
+    } else if (g.fused) {
+      if (g.rows_cfg.variant == 1) f32 += rows * kRows31 * kF32;
+    } else if (g.big) {
+      const double long_rows = rows / (g.folded ? 1.0 : 2.0);  // unfolded: h = 0 rows only
+      const double r0 = g.big_r0;
+      if (g.big16)
+        f16 += long_rows * r0 * kRows16 * kF16;
+      else if (g.rows_cfg.variant == 1 || g.rows_cfg.variant == 2)
+        f32 += long_rows * r0 * kRows31 * kF32;
+      else if (g.rows_cfg.variant == 5)
+        f32 += long_rows * 2.0 * r0 * kHalf31 * kF32;
+      else if (g.rows_cfg.variant == 3)
+        f32 += long_rows * (static_cast<double>(g.P) / 496.0) * kFourStep31 * kF32;
+    }
+  }
+}
+
 std::string Plan::describe() const {
   std::string s;
   for (const auto& e : systems) {
@@ -1082,6 +1114,13 @@ int tga_plan_get_timing(tga_plan* plan, int32_t kernel, double* total_ms, int64_
     *total_ms = p->timers[kernel].total_ms;
     *launches = p->timers[kernel].launches;
     if (bytes_per_launch) *bytes_per_launch = p->timers[kernel].bytes;
+  });
+}
+
+int tga_plan_rows_matrix_flops(const tga_plan* plan, double* f32_flops, double* f16_flops) {
+  return plan_guard([&] {
+    if (!plan || !f32_flops || !f16_flops) fail(TGA_ERR_INVALID_ARGUMENT, "null argument");
+    reinterpret_cast<const Plan*>(plan)->rows_matrix_flops(*f32_flops, *f16_flops);
   });
 }
 

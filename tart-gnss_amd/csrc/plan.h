@@ -107,6 +107,8 @@ struct Plan {
   void drop_graph();
   void finish(tga_peak* out, hipStream_t st);
   std::string describe() const;
+  // matrix-core FLOP of the rows stage of one search (f32 and f16 MFMAs)
+  void rows_matrix_flops(double& f32, double& f16) const;
   void time_record(hipEvent_t ev, int slot, hipStream_t st, const char* what);
   void time_begin(int slot, hipStream_t st);
   void time_end(int slot, hipStream_t st, double bytes);

@@ -101,6 +101,7 @@ PROTOTYPES = {
     "tga_plan_set_graph": (C.c_int, [C.c_void_p, C.c_int32]),
     "tga_plan_get_timing": (C.c_int, [C.c_void_p, C.c_int32, _dp, C.POINTER(C.c_int64), _dp]),
     "tga_plan_kernel_name": (C.c_char_p, [C.c_void_p, C.c_int32]),
+    "tga_plan_rows_matrix_flops": (C.c_int, [C.c_void_p, _dp, _dp]),
     "tga_plan_describe": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
     "tga_plan_surface_size": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_size_t)]),
     "tga_plan_carrier": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_float), C.c_size_t,

@@ -395,6 +395,14 @@ class Plan:
                                         C.byref(b)))
         return ms.value, n.value, b.value
 
+    def rows_matrix_flops(self):
+        """(f32, f16) matrix-core FLOP of one search's correlation rows, from the kernels the
+        plan's groups run (v_mfma_f32_16x16x4f32 and v_mfma_f32_16x16x32_f16 instructions)."""
+        f32 = C.c_double(0)
+        f16 = C.c_double(0)
+        check(lib().tga_plan_rows_matrix_flops(self._h, C.byref(f32), C.byref(f16)))
+        return f32.value, f16.value
+
     def kernel_name(self, kernel=0):
         return lib().tga_plan_kernel_name(self._h, int(kernel)).decode()
 

@@ -392,3 +392,25 @@ def test_constant_observation_every_power_ties(gpu, split_variant, system):
         ref, *_ = oracle_search(levels[0], fs, P, L, fc, code)
         assert_peak_matches(peaks[0][0, pi], ref, P, (False, False), what=f"{system} {prn}")
         assert int(peaks[0][0, pi]["best_freq_idx"]) == len(fc) - 1
+
+
+def test_rows_matrix_flops_follow_the_plan(gpu, monkeypatch):
+    """tga_plan_rows_matrix_flops counts the matrix-core instructions of the kernels a plan's
+    groups run: 33 x 16 v_mfma_f32_16x16x4f32 per 16368-point (sub-)row with the matrix-core
+    radix 31, none with the VALU codelet, 17 x 16 per 8184-point half sub-row, 16 per 496-point
+    four-step row; the FP16 form 857 v_mfma_f32_16x16x32_f16 per (sub-)row."""
+    f32_op, f16_op = 2.0 * 16 * 16 * 4, 2.0 * 16 * 16 * 32
+    fc = grid(IF, 500.0, 500.0)
+    rows = 1 * 3 * len(fc) * 2
+    for v, per_row in ((0, 0), (1, 528)):
+        monkeypatch.setenv("TGA_FUSED_VARIANT", str(v))
+        plan = tga.Plan(FS, 1, 2 * 16368, [tga.SystemSearch("gps", fc, [1, 2, 3])])
+        assert plan.rows_matrix_flops() == (rows * per_row * f32_op, 0.0)
+    monkeypatch.delenv("TGA_FUSED_VARIANT")
+    plan = tga.Plan(FS, 1, 2 * 16368, [tga.SystemSearch("gps", fc, [1, 2, 3])], precision="fp16")
+    assert plan.rows_matrix_flops() == (0.0, rows * 857 * f16_op)
+    long_rows = 1 * 2 * len(fc) * 2  # galileo, 2 PRNs, folded
+    for v, per_row in (("2", 4 * 528), ("5", 8 * 272), ("3", 132 * 16), ("0", 0)):
+        monkeypatch.setenv("TGA_SPLIT_VARIANT", v)
+        plan = tga.Plan(FS, 1, 2 * 65472, [tga.SystemSearch("galileo", fc, [3, 11])])
+        assert plan.rows_matrix_flops() == (long_rows * per_row * f32_op, 0.0), v
