@@ -336,16 +336,17 @@ void Plan::build(const tga_plan_desc& d) {
       // sub-rows of a chunk of rows per pre-pass / transform pair, on the plan's stream (a
       // second stream for the pre-pass of the next chunk measured no faster: the two kernels
       // then slow each other down).  Larger chunks amortise each launch pair's ramp and tail
-      // (per all24 search: 1000 ms at 2 GiB, 894 ms at 16 GiB, 885 ms at 32 GiB of sub-rows),
-      // so a chunk holds up to 32 GiB, at most an eighth of the free device memory;
-      // TGA_SPLIT_CHUNK_MB sets it explicitly.
+      // (per all24 search: 1000 ms at 2 GiB, 894 ms at 16 GiB, 885 ms at 32 GiB of sub-rows
+      // at the time; later 826.5 / 817.9 / 816.9 ms at 32 / 48 / 64 GiB), so a chunk holds up
+      // to 48 GiB, at most a sixth of the free device memory; TGA_SPLIT_CHUNK_MB sets it
+      // explicitly.
       const size_t y_elem = g.big16 ? sizeof(uint32_t) : sizeof(float2);
       const char* env_mb = std::getenv("TGA_SPLIT_CHUNK_MB");
       size_t free_b = 0, total_b = 0;
       TGA_HIP(hipMemGetInfo(&free_b, &total_b));
       const long long chunk_bytes =
           env_mb ? std::max(1, std::atoi(env_mb)) * (1LL << 20)
-                 : std::max<long long>(256LL << 20, std::min<long long>(32LL << 30, free_b / 8));
+                 : std::max<long long>(256LL << 20, std::min<long long>(48LL << 30, free_b / 6));
       const long long cap = std::max<long long>(
           1, chunk_bytes / (static_cast<long long>(M) * static_cast<long long>(y_elem)));
       g.chunk_rows = std::min<long long>(cap, g.n_rows);
