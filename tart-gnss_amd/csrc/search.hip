@@ -2287,7 +2287,26 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
     for (int u = 0; u < 9; ++u)
       xk[u] = __builtin_amdgcn_raw_buffer_load_b32(xv, vo, (P / 33) * (16 * (u >> 2) + (u & 3)) * 4, 0);
   }
+  // SUB > 0: the next sub-row's pass-A words (tiles wave and wave + 16) and row index are
+  // loaded right after this sub-row's pass A, so their latency hides behind passes B and C
+  uint32_t ypf[SUB > 0 ? 2 : 1][9];
+  int rmap = 0;
+  auto load_next = [&](long long itx) {
+    if constexpr (SUB > 0) {
+      const __amdgpu_buffer_rsrc_t yv = buffer_view(a.Y + itx * P, P * 4);
+      const int w0 = tid >> 6, l16 = tid & 15, gp = (tid & 63) >> 4;
+#pragma unroll
+      for (int s2 = 0; s2 < 2; ++s2)
+        if (w0 + 16 * s2 < NBA / 16)
+          load_tile_y(ypf[s2], yv, (16 * (w0 + 16 * s2) + l16 + NBA * 4 * gp) * 4);
+      rmap = a.rowmap[itx / SUB];
+    }
+  };
+  if constexpr (SUB > 0) {
+    if (it0 < it1) load_next(it0);
+  }
   for (long long it = it0; it < it1; it += step) {
+    const int rcur = rmap;  // SUB > 0: this sub-row's row index
     const int prn = static_cast<int>(it);
     // surface form: output sample idx of this row (sub-row) is stored at dbase + dstride idx
     size_t dbase = 0;
@@ -2344,8 +2363,7 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
             b[kb] = __builtin_bit_cast(halfx8, w);
           }
         } else {
-          uint32_t y[9];
-          load_tile_y(y, rv, (j + NBA * 4 * grp) * 4);
+          const uint32_t(&y)[9] = ypf[s];
 #pragma unroll
           for (int kb = 0; kb < 3; ++kb) {
             u32x4 w;
@@ -2373,6 +2391,9 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
           if (t < 4) dst[8 * t + 1] = pack_f16(make_float2(acc[2], acc[3]), 1.0f);
         }
       }
+    }
+    if constexpr (SUB > 0) {
+      if (it + step < it1) load_next(it + step);
     }
     lds_barrier();
     // ---- pass B: radix 16, 64 tiles (the last holds 15 butterflies): wave + 16 s
@@ -2539,7 +2560,7 @@ __global__ __launch_bounds__(T16) void k_rows16(Row16Args a) {
       // M: several atomic maxima, the largest sample wins).  inf / NaN powers are flagged.
       const long long rq = it / SUB;
       const int k0 = static_cast<int>(it - rq * SUB);
-      const long long r = a.rowmap[rq];
+      const long long r = __builtin_amdgcn_readfirstlane(rcur);
       const unsigned int wmax = wave_umax(pmx);
       if (lane == 0) red.pk[wave] = wmax;
       lds_barrier();
