@@ -354,6 +354,7 @@ void Plan::build(const tga_plan_desc& d) {
       g.d_rowkey.alloc(static_cast<size_t>(g.n_rows) * sizeof(unsigned long long));
       g.d_rownan.alloc(static_cast<size_t>(g.n_rows) * sizeof(int));
       g.d_rowmap.alloc(static_cast<size_t>(g.chunk_rows) * sizeof(int));
+      g.d_rowtab.alloc(static_cast<size_t>(g.n_rows / (g.folded ? 1 : 2)) * sizeof(int4));
       if (g.n_rows > (1LL << 31) - 1)
         fail(TGA_ERR_UNSUPPORTED, "a long-period group holds more than 2^31 - 1 rows");
       alloc_winners(g);
@@ -575,9 +576,18 @@ void Plan::autotune_split(Group& g) {
     }
   g.rows_cfg = cands[0];
   g.tune.clear();
-  if (cands.size() == 1) return;
-  launch_fill_tune(g.d_X.p, g.d_X.bytes, false, stream);
   const int hs = g.folded ? 1 : 2;
+  // the row table of the chosen order, every enumerated row (the pre-passes read it)
+  auto build_rowtab = [&] {
+    if (!g.natural())
+      launch_big_rowtab(g.d_rowtab.as<int4>(), g.n_rows / hs, g.n_prn, g.n_bin, n_ant, hs,
+                        g.rows_cfg.tile, stream);
+  };
+  if (cands.size() == 1) {
+    build_rowtab();
+    return;
+  }
+  launch_fill_tune(g.d_X.p, g.d_X.bytes, false, stream);
   const long long r1 = std::min(g.n_rows / hs, g.chunk_rows);
   auto launch = [&](const FusedRowsConfig& c) {
     if (c.variant == 3) {
@@ -587,9 +597,11 @@ void Plan::autotune_split(Group& g) {
                     g.d_rownan.as<int>(), g.d_rows.as<RowOut>(), stream);
       return;
     }
+    // the candidate's row order for the timed chunk (a short kernel beside the chunk's pair)
+    launch_big_rowtab(g.d_rowtab.as<int4>(), r1, g.n_prn, g.n_bin, n_ant, hs, c.tile, stream);
     if (c.variant == 5) {
-      launch_half_pre(g.d_X.as<float2>(), g.d_rep.as<float2>(), g.d_twh.as<float2>(), g.big_r0,
-                      g.n_prn, g.n_bin, n_ant, hs, c.tile, 0, r1, g.d_Y.as<float2>(),
+      launch_half_pre(g.d_X.as<float2>(), g.d_rep.as<float2>(), g.d_twh.as<float2>(),
+                      g.d_rowtab.as<int4>(), g.big_r0, 0, r1, g.d_Y.as<float2>(),
                       g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(),
                       g.d_rowmap.as<int>(), stream);
       launch_half_sub(g.d_tw.as<float2>(), g.big_r0, g.n_prn, g.n_bin, n_ant, hs, c.tile, 0, r1,
@@ -598,8 +610,8 @@ void Plan::autotune_split(Group& g) {
                       stream);
       return;
     }
-    launch_big_pre(g.d_X.as<float2>(), g.d_rep.as<float2>(), g.d_twb.as<float2>(), g.big_r0,
-                   g.n_prn, g.n_bin, n_ant, hs, c.tile, 0, r1, g.d_Y.p, g.big16,
+    launch_big_pre(g.d_X.as<float2>(), g.d_rep.as<float2>(), g.d_twb.as<float2>(),
+                   g.d_rowtab.as<int4>(), g.big_r0, 0, r1, g.d_Y.p, g.big16,
                    g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(),
                    g.d_rowmap.as<int>(), stream);
     launch_big_sub(g.d_tw.as<float2>(), g.big_r0, g.n_prn, g.n_bin, n_ant, hs, c.tile, 0, r1,
@@ -607,6 +619,7 @@ void Plan::autotune_split(Group& g) {
                    g.d_rowmap.as<int>(), g.d_rows.as<RowOut>(), stream, c.variant);
   };
   tune_pick(g, cands, stream, launch);
+  build_rowtab();
 }
 
 // While a graph is being captured the records become event-record nodes of the graph, and
@@ -672,13 +685,13 @@ void Plan::split_rows(Group& g, hipStream_t st, float* dump) {
     void* Y = g.d_Y.p;
     int* rowmap = g.d_rowmap.as<int>();
     if (half)
-      launch_half_pre(g.X, g.d_rep.as<float2>(), g.d_twh.as<float2>(), g.big_r0, g.n_prn,
-                      g.n_bin, n_ant, hs, g.rows_cfg.tile, r0, r1, static_cast<float2*>(Y),
+      launch_half_pre(g.X, g.d_rep.as<float2>(), g.d_twh.as<float2>(), g.d_rowtab.as<int4>(),
+                      g.big_r0, r0, r1, static_cast<float2*>(Y),
                       g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(), rowmap, s);
     else
-      launch_big_pre(g.X, g.d_rep.as<float2>(), g.d_twb.as<float2>(), g.big_r0, g.n_prn,
-                     g.n_bin, n_ant, hs, g.rows_cfg.tile, r0, r1, Y, g.big16,
-                     g.d_rowkey.as<unsigned long long>(), g.d_rownan.as<int>(), rowmap, s);
+      launch_big_pre(g.X, g.d_rep.as<float2>(), g.d_twb.as<float2>(), g.d_rowtab.as<int4>(),
+                     g.big_r0, r0, r1, Y, g.big16, g.d_rowkey.as<unsigned long long>(),
+                     g.d_rownan.as<int>(), rowmap, s);
   };
   auto sub = [&](long long r0, long long r1, hipStream_t s) {
     void* Y = g.d_Y.p;

@@ -35,6 +35,7 @@ struct Group {
   DevBuf d_rep16, d_X16;  // FP16 form: f16 (re, im) pairs scaled by kF16Scale
   DevBuf d_Y, d_rowkey, d_rownan, d_rowmap, d_twb, d_win, d_Zw, d_Zws;  // long-period split
   DevBuf d_twh;  // half sub-row twiddles (FP32 split variant 5)
+  DevBuf d_rowtab;  // split groups: (X row, replica row, RowOut row) per enumerated row
   DevBuf d_tw4, d_c31;  // four-step split: W_P^{n2 k1} as [k1][n2], radix-31 cos / sin
   DevBuf d_twl;  // sub-row split: W_P^e two-level table of the long-period forward
   DevBuf d_m2;   // masked maximum (f32 bits) of each winning row (win_sub groups)

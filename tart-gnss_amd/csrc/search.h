@@ -157,8 +157,9 @@ bool big_supported(size_t P, size_t N, bool folded);
 // The pre-pass twiddles W_P^{pfa_nat(s)} for the positions s < 16368 of a segment in the
 // prime-factor order.
 std::vector<float2> big_twiddles(size_t P);
-// Rows [r0, r1) of the tiled enumeration (tiles of 8 forward spectra x 8 replica spectra, whose
-// rows reuse them from the caches); results land at the usual row index.  launch_big_pre writes the twiddled
+// Rows [r0, r1) of the tiled enumeration (tiles of TX forward spectra x TP replica spectra, whose
+// rows reuse them from the caches, as listed by the row table); results land at the usual row
+// index.  launch_big_pre writes the twiddled
 // sub-rows of the rows to Y ((r1 - r0) * P complex values) and clears their keys;
 // launch_big_sub transforms them and writes the rows' RowOut.
 // hs = 1: every (bin, h) row; hs = 2: unfolded rows, h = 0 only (the [c,-c] hypothesis equals
@@ -167,10 +168,13 @@ std::vector<float2> big_twiddles(size_t P);
 // run through the matrix-core rows kernel, whose powers then carry kSplitF16Scale^2; otherwise
 // Y holds float2 and the f32 rows kernel transforms them.
 // tile: TX | TP << 16, rows enumerated in tiles of TX forward spectra x TP replica spectra
-void launch_big_pre(const float2* X, const float2* rep, const float2* twb, int R0, int n_prn,
-                    int n_bin, int n_ant, int hs, int tile, long long r0, long long r1, void* Y,
-                    bool f16, unsigned long long* rowkey, int* rownan, int* rowmap,
-                    hipStream_t st);
+void launch_big_pre(const float2* X, const float2* rep, const float2* twb, const int4* rowtab,
+                    int R0, long long r0, long long r1, void* Y, bool f16,
+                    unsigned long long* rowkey, int* rownan, int* rowmap, hipStream_t st);
+// The row table the pre-passes read (rows [0, n) of the tiled order of `tile`): per row the
+// forward-spectrum row, the replica row and the RowOut row index, as int4 (x, y, z, 0).
+void launch_big_rowtab(int4* tab, long long n, int n_prn, int n_bin, int n_ant, int hs, int tile,
+                       hipStream_t st);
 // variant: the FP32 sub-row transforms (0: k_rows with the VALU radix-31 codelet, 1: k_rows with
 // the matrix-core radix 31 and the next sub-row prefetched into registers, 2: k_subrows, the
 // matrix-core form with pass B in place and each lane's best output pair); the FP16 form has one
@@ -187,10 +191,9 @@ void launch_big_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, i
 // to Y and clears the rows' keys; launch_half_sub transforms them and writes the rows' RowOut;
 // dump as launch_big_sub's.
 std::vector<float2> half_twiddles(size_t P);
-void launch_half_pre(const float2* X, const float2* rep, const float2* twh, int R0, int n_prn,
-                     int n_bin, int n_ant, int hs, int tile, long long r0, long long r1,
-                     float2* Y, unsigned long long* rowkey, int* rownan, int* rowmap,
-                     hipStream_t st);
+void launch_half_pre(const float2* X, const float2* rep, const float2* twh, const int4* rowtab,
+                     int R0, long long r0, long long r1, float2* Y, unsigned long long* rowkey,
+                     int* rownan, int* rowmap, hipStream_t st);
 void launch_half_sub(const float2* tw, int R0, int n_prn, int n_bin, int n_ant, int hs, int tile,
                      long long r0, long long r1, const float2* Y, unsigned long long* rowkey,
                      int* rownan, const int* rowmap, RowOut* rows, hipStream_t st,

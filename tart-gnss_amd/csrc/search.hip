@@ -501,13 +501,14 @@ __device__ __forceinline__ void big_pre_core(const float2* __restrict__ xrow_p,
   }
 }
 
-// Pre-pass of positions [p_begin, p_end) of row rr of a chunk (enumerated row r0 + rr): thread
-// lt takes the two consecutive positions p_begin + 2 lt (+1).
+// Pre-pass of positions [p_begin, p_end) of row rr of a chunk (enumerated row r0 + rr, whose
+// forward-spectrum row, replica row and RowOut row come from the plan's row table): thread lt
+// takes the two consecutive positions p_begin + 2 lt (+1).
 template <int R0, bool F16>
 __device__ __forceinline__ void big_pre_positions(const float2* __restrict__ X,
                                                   const float2* __restrict__ rep,
-                                                  const float2* __restrict__ twb, int n_prn,
-                                                  int n_bin, int n_ant, int hs, int tile,
+                                                  const float2* __restrict__ twb,
+                                                  const int4* __restrict__ rowtab,
                                                   long long r0, long long rr, int p_begin,
                                                   int p_end, int lt,
                                                   void* __restrict__ Yv,
@@ -517,20 +518,15 @@ __device__ __forceinline__ void big_pre_positions(const float2* __restrict__ X,
   constexpr int P = R0 * kFusedP;
   const int t = p_begin + 2 * lt;
   if (t >= p_end) return;
-  const long long I = r0 + rr;  // tiled order (big_item)
-  const BigItem it = big_item(I, n_prn, n_bin, n_ant, hs, tile);
-  const long long xrow = it.xrow;
-  const int prn = it.prn;
-  const int h = static_cast<int>(xrow & 1);
+  const int4 ref = rowtab[r0 + rr];  // (X row, replica row, RowOut row), block-uniform
   void* yrow = F16 ? static_cast<void*>(static_cast<uint32_t*>(Yv) + rr * P)
                    : static_cast<void*>(static_cast<float2*>(Yv) + rr * P);
-  big_pre_core<R0, F16>(X + (hs == 1 ? xrow : xrow >> 1) * P,
-                        rep + (static_cast<long long>(prn) * 2 + h) * P, twb, t, yrow);
+  big_pre_core<R0, F16>(X + static_cast<long long>(ref.x) * P,
+                        rep + static_cast<long long>(ref.y) * P, twb, t, yrow);
   if (t == 0) {
-    const long long r = big_row(I, n_prn, n_bin, n_ant, hs, tile);
-    rowkey[r] = 0ull;
-    rownan[r] = 0;
-    rowmap[rr] = static_cast<int>(r);  // the chunk's row index for its transforms
+    rowkey[ref.z] = 0ull;
+    rownan[ref.z] = 0;
+    rowmap[rr] = ref.z;  // the chunk's row index for its transforms
   }
 }
 
@@ -1656,8 +1652,8 @@ __device__ __forceinline__ int half_pair_sample(int q, int kk, float d) {
 template <int R0>
 __global__ __launch_bounds__(256) void k_half_pre(const float2* __restrict__ X,
                                                   const float2* __restrict__ rep,
-                                                  const float2* __restrict__ twh, int n_prn,
-                                                  int n_bin, int n_ant, int hs, int tile,
+                                                  const float2* __restrict__ twh,
+                                                  const int4* __restrict__ rowtab,
                                                   long long r0, long long n_rows,
                                                   float2* __restrict__ Y,
                                                   unsigned long long* __restrict__ rowkey,
@@ -1670,11 +1666,9 @@ __global__ __launch_bounds__(256) void k_half_pre(const float2* __restrict__ X,
   const int chunk = 4 * static_cast<int>(blockIdx.x & 7) + static_cast<int>((blockIdx.x >> 3) & 3);
   const int s = 256 * chunk + static_cast<int>(threadIdx.x);
   if (s >= kHalfS) return;
-  const long long I = r0 + rr;
-  const BigItem it = big_item(I, n_prn, n_bin, n_ant, hs, tile);
-  const int h = static_cast<int>(it.xrow & 1);
-  const float2* xr = X + (hs == 1 ? it.xrow : it.xrow >> 1) * PL;
-  const float2* cr = rep + (static_cast<long long>(it.prn) * 2 + h) * PL;
+  const int4 ref = rowtab[r0 + rr];  // (X row, replica row, RowOut row), block-uniform
+  const float2* xr = X + static_cast<long long>(ref.x) * PL;
+  const float2* cr = rep + static_cast<long long>(ref.y) * PL;
   const int c1 = s / 248, b = s - 248 * c1, c2 = b / 31, c3 = b - 31 * c2;
   const int m = (496 * c1 + 7161 * c2 + 528 * c3) % kHalfS;
   const int lo = 496 * c1 + 31 * ((8 - c2) & 7) + c3;
@@ -1704,10 +1698,9 @@ __global__ __launch_bounds__(256) void k_half_pre(const float2* __restrict__ X,
     __builtin_nontemporal_store(v[k0].y, y + 2 * kHalfS * k0 + 1);
   }
   if (s == 0) {
-    const long long r = big_row(I, n_prn, n_bin, n_ant, hs, tile);
-    rowkey[r] = 0ull;
-    rownan[r] = 0;
-    rowmap[rr] = static_cast<int>(r);
+    rowkey[ref.z] = 0ull;
+    rownan[ref.z] = 0;
+    rowmap[rr] = ref.z;
   }
 }
 
@@ -2995,8 +2988,8 @@ __global__ __launch_bounds__(kRedT) void k_row_reduce(const float2* __restrict__
 template <int R0, bool F16>
 __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
                                                  const float2* __restrict__ rep,
-                                                 const float2* __restrict__ twb, int n_prn,
-                                                 int n_bin, int n_ant, int hs, int tile,
+                                                 const float2* __restrict__ twb,
+                                                 const int4* __restrict__ rowtab,
                                                  long long r0, long long n_rows,
                                                  void* __restrict__ Yv,
                                                  unsigned long long* __restrict__ rowkey,
@@ -3006,9 +2999,23 @@ __global__ __launch_bounds__(256) void k_big_pre(const float2* __restrict__ X,
   if (rr >= n_rows) return;
   const int chunk = 4 * static_cast<int>(blockIdx.x & 7) + static_cast<int>((blockIdx.x >> 3) & 3);
   const int p0 = 512 * chunk;
-  fused::big_pre_positions<R0, F16>(X, rep, twb, n_prn, n_bin, n_ant, hs, tile, r0, rr, p0,
+  fused::big_pre_positions<R0, F16>(X, rep, twb, rowtab, r0, rr, p0,
                              p0 + 512 < kFusedP ? p0 + 512 : kFusedP,
                              static_cast<int>(threadIdx.x), Yv, rowkey, rownan, rowmap);
+}
+
+// The row table of a split group's tiled row order (big_item / big_row), enumerated rows [0, n):
+// (forward-spectrum row: xrow, or xrow / 2 for unfolded rows; replica row 2 prn + h; RowOut row)
+// per row, built once per plan, so a pre-pass block takes its row with one scalar load instead
+// of 64-bit divisions in every wave.
+__global__ void k_big_rowtab(int4* __restrict__ tab, long long n, int n_prn, int n_bin,
+                             int n_ant, int hs, int tile) {
+  const long long I = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (I >= n) return;
+  const BigItem it = big_item(I, n_prn, n_bin, n_ant, hs, tile);
+  const int h = static_cast<int>(it.xrow & 1);
+  tab[I] = make_int4(static_cast<int>(hs == 1 ? it.xrow : it.xrow >> 1), it.prn * 2 + h,
+                     static_cast<int>(big_row(I, n_prn, n_bin, n_ant, hs, tile)), 0);
 }
 
 // Pre-pass of the winning row of every (antenna, PRN) pair w (win[w] = 2 bin + h): its R0
@@ -3514,27 +3521,33 @@ std::vector<float2> big_twiddles(size_t P) {
   return t;
 }
 
-void launch_big_pre(const float2* X, const float2* rep, const float2* twb, int R0, int n_prn,
-                    int n_bin, int n_ant, int hs, int tile, long long r0, long long r1, void* Y,
-                    bool f16, unsigned long long* rowkey, int* rownan, int* rowmap,
-                    hipStream_t st) {
-  const long long nr = r1 - r0;
-  const long long total = nr;  // rows of this chunk, 32 blocks each
+void launch_big_pre(const float2* X, const float2* rep, const float2* twb, const int4* rowtab,
+                    int R0, long long r0, long long r1, void* Y, bool f16,
+                    unsigned long long* rowkey, int* rownan, int* rowmap, hipStream_t st) {
+  const long long nr = r1 - r0;  // rows of this chunk, 32 blocks each
   const dim3 grid(static_cast<unsigned>(nr * 32));
   if (R0 == 4 && !f16)
-    hipLaunchKernelGGL((k_big_pre<4, false>), grid, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
-                       n_ant, hs, tile, r0, total, Y, rowkey, rownan, rowmap);
+    hipLaunchKernelGGL((k_big_pre<4, false>), grid, dim3(256), 0, st, X, rep, twb, rowtab, r0, nr,
+                       Y, rowkey, rownan, rowmap);
   else if (R0 == 4)
-    hipLaunchKernelGGL((k_big_pre<4, true>), grid, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
-                       n_ant, hs, tile, r0, total, Y, rowkey, rownan, rowmap);
+    hipLaunchKernelGGL((k_big_pre<4, true>), grid, dim3(256), 0, st, X, rep, twb, rowtab, r0, nr,
+                       Y, rowkey, rownan, rowmap);
   else if (R0 == 10 && !f16)
-    hipLaunchKernelGGL((k_big_pre<10, false>), grid, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
-                       n_ant, hs, tile, r0, total, Y, rowkey, rownan, rowmap);
+    hipLaunchKernelGGL((k_big_pre<10, false>), grid, dim3(256), 0, st, X, rep, twb, rowtab, r0,
+                       nr, Y, rowkey, rownan, rowmap);
   else if (R0 == 10)
-    hipLaunchKernelGGL((k_big_pre<10, true>), grid, dim3(256), 0, st, X, rep, twb, n_prn, n_bin,
-                       n_ant, hs, tile, r0, total, Y, rowkey, rownan, rowmap);
+    hipLaunchKernelGGL((k_big_pre<10, true>), grid, dim3(256), 0, st, X, rep, twb, rowtab, r0,
+                       nr, Y, rowkey, rownan, rowmap);
   else
     fail(TGA_ERR_UNSUPPORTED, "long-period split supports 4 and 10 sub-rows");
+  TGA_HIP(hipGetLastError());
+}
+
+void launch_big_rowtab(int4* tab, long long n, int n_prn, int n_bin, int n_ant, int hs, int tile,
+                       hipStream_t st) {
+  if (n < 1) return;
+  hipLaunchKernelGGL(k_big_rowtab, dim3(grid_for(n, 256)), dim3(256), 0, st, tab, n, n_prn,
+                     n_bin, n_ant, hs, tile);
   TGA_HIP(hipGetLastError());
 }
 
@@ -3555,18 +3568,17 @@ std::vector<float2> half_twiddles(size_t P) {
   return t;
 }
 
-void launch_half_pre(const float2* X, const float2* rep, const float2* twh, int R0, int n_prn,
-                     int n_bin, int n_ant, int hs, int tile, long long r0, long long r1,
-                     float2* Y, unsigned long long* rowkey, int* rownan, int* rowmap,
-                     hipStream_t st) {
+void launch_half_pre(const float2* X, const float2* rep, const float2* twh, const int4* rowtab,
+                     int R0, long long r0, long long r1, float2* Y, unsigned long long* rowkey,
+                     int* rownan, int* rowmap, hipStream_t st) {
   const long long nr = r1 - r0;
   const dim3 grid(static_cast<unsigned>(nr * 32));
   if (R0 == 4)
-    hipLaunchKernelGGL((fused::k_half_pre<4>), grid, dim3(256), 0, st, X, rep, twh, n_prn, n_bin,
-                       n_ant, hs, tile, r0, nr, Y, rowkey, rownan, rowmap);
+    hipLaunchKernelGGL((fused::k_half_pre<4>), grid, dim3(256), 0, st, X, rep, twh, rowtab, r0, nr,
+                       Y, rowkey, rownan, rowmap);
   else if (R0 == 10)
-    hipLaunchKernelGGL((fused::k_half_pre<10>), grid, dim3(256), 0, st, X, rep, twh, n_prn,
-                       n_bin, n_ant, hs, tile, r0, nr, Y, rowkey, rownan, rowmap);
+    hipLaunchKernelGGL((fused::k_half_pre<10>), grid, dim3(256), 0, st, X, rep, twh, rowtab, r0,
+                       nr, Y, rowkey, rownan, rowmap);
   else
     fail(TGA_ERR_UNSUPPORTED, "long-period split supports 4 and 10 sub-rows");
   TGA_HIP(hipGetLastError());
