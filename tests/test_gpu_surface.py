@@ -45,11 +45,17 @@ def truth(levels, system, mode):
     return out
 
 
-@pytest.mark.parametrize("precision", ["fp16", "fp32"])
-def test_surface_error_per_system(gpu, precision):
+@pytest.mark.parametrize("precision,split", [("fp16", None), ("fp32", None), ("fp32", "5")],
+                         ids=["fp16", "fp32", "fp32-half-subrows"])
+def test_surface_error_per_system(gpu, precision, split, monkeypatch):
+    # split = "5": the FP32 split groups pinned to the half sub-rows (k_halfrows' surface form)
+    if split is not None:
+        monkeypatch.setenv("TGA_SPLIT_VARIANT", split)
     packed = synth.synth_observation(N_ANT, L, FS, signals(77), fmt=2, seed=77)
     plan = make_plan(N_ANT, L, 2, precision)
     print(plan.describe(), flush=True)
+    if split is not None:
+        assert plan.describe().count(f"rows_variant={split}") == 3  # E1-C, B1C, L1C
     surf, peaks = plan.surface(packed)
     assert [s.shape for s in surf] == [(N_ANT, len(PRNS[s]), 3, 2, PERIOD[s]) for s in ORDER]
     plain = plan.search(packed)
